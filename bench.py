@@ -1,0 +1,248 @@
+"""Benchmark: dielectric sphere (eps = 12+0.01i, radius 1 um) at omega = 1,
+icosphere level 4, PMCHWT N = 15360 -- assemble + LU + solve per frequency.
+
+A step is one frequency of the mixed-precision path: complex128 assembly
+of the BEM matrix, complex64 LU on the matrix cores, complex128 iterative
+refinement of the plane-wave solve, and the power.  The plain complex128
+path (assembly, complex128 LU, solve) runs the same number of steps
+beside it for the speedup and the power error, which must stay <= 1e-8.
+The frequency-independent FIPPI table and the panel geometry are built
+once, before the timed region, as the reference caches them.
+
+Prints one JSON line (see --help for the knobs)."""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "scuff-em_amd"))
+sys.path.insert(0, ROOT)
+
+PEAK = {"mfma_f64": 78.6e12, "mfma_f32": 157.3e12, "hbm": 8.0e12}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=4, help="icosphere level (4: N=15360)")
+    ap.add_argument("--omega", type=float, default=1.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-lu-n", type=int, default=2048, help="oracle LU sample size")
+    ap.add_argument("--cpu-rows", type=int, default=6, help="oracle assembly sample (edge rows)")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured with rocprofv3 counters (committed
+    summary), or None."""
+    fn = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        d = json.load(f)
+    v = d.get(kernel)
+    return None if v is None else v.get("bytes_per_launch")
+
+
+def cpu_baseline(level, omega, nrows, nlu, eps):
+    """The oracle (a scalar CPU restatement of the reference's path) on a
+    bounded sample of the same workload, scaled to one frequency:
+    assembly rows of a few edges (warm FIPPI cache) scaled by the number
+    of edge pairs, and an unblocked LU of an nlu x nlu matrix scaled by
+    (N / nlu)^3."""
+    import numpy as np
+    from oracle import oracle as O
+    V, T = O.icosphere(level)
+    Go = O.Geometry()
+    Go.add_surface(V, T, kind=1, eps=eps)
+    N = Go.N
+    NE = N // 2
+    M = np.zeros((N, N), dtype=np.complex128, order="F")
+    edges = [int(round(i * (NE - 1) / max(1, nrows - 1))) for i in range(nrows)]
+    for e in edges:                       # cold: fills the FIPPI cache
+        Go.assemble_rows(omega, e, e + 1, M)
+    t0 = time.perf_counter()
+    for e in edges:
+        Go.assemble_rows(omega, e, e + 1, M)
+    t_rows = time.perf_counter() - t0
+    sampled_pairs = sum(NE - e for e in edges)
+    all_pairs = NE * (NE + 1) / 2
+    t_asm = t_rows * all_pairs / sampled_pairs
+    rng = np.random.default_rng(20261202)
+    A = rng.standard_normal((nlu, nlu)) + 1j * rng.standard_normal((nlu, nlu))
+    t0 = time.perf_counter()
+    O.zgetrf(A)
+    t_lu = (time.perf_counter() - t0) * (N / nlu) ** 3
+    return {
+        "value": t_asm + t_lu, "unit": "s/freq", "cores": 1, "kind": "port",
+        "sample": f"oracle on 1 core: assembly rows of {nrows} of {NE} edges (warm FIPPI cache, "
+                  f"{t_rows:.2f} s, scaled by edge pairs to {t_asm:.1f} s) + unblocked zgetrf at "
+                  f"N={nlu} scaled by (N/{nlu})^3 to {t_lu:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import scuffx
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    eps = 12 + 0.01j
+    omega = args.omega
+    V, T = scuffx.icosphere(args.level)
+    G = scuffx.RWGGeometry()
+    G.AddMesh(V, T, "CONST_EPS_12+0.01i")
+    N = G.TotalBFs
+    print(f"# N={N}: building FIPPI table", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    G.BuildFIPPITable()
+    t_fippi = time.perf_counter() - t0
+    M = G.AllocateBEMMatrix()
+    LU32 = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ipiv = torch.zeros(N, dtype=torch.int32, device="cuda")
+    pw = scuffx.PlaneWave([1.0, 0.0, 0.0], [0.0, 0.0, 1.0])
+    RHS = G.AssembleRHSVector(omega, pw)
+    KN = scuffx.HVector(N)
+    x = scuffx.HVector(N)
+    L = scuffx.lib()
+    import ctypes as C
+
+    phases = {"mixed": {}, "fp64": {}}
+
+    def tick(path, name, t0):
+        # every library call returns after its work is complete
+        phases[path][name] = phases[path].get(name, 0.0) + time.perf_counter() - t0
+        return time.perf_counter()
+
+    def step_mixed():
+        scuffx._sync_stream()
+        t = time.perf_counter()
+        G.AssembleBEMMatrix(omega, M)
+        t = tick("mixed", "assemble", t)
+        info = L.scuffx_lu32_from_c128(scuffx._ptr(M.t), N, N, scuffx._ptr(LU32), N, scuffx._ptr(ipiv))
+        if info != 0:
+            raise RuntimeError(f"complex64 LU failed ({info})")
+        t = tick("mixed", "lu_c64", t)
+        it, rel = C.c_int(), C.c_double()
+        scuffx._check(L.scuffx_refine(scuffx._ptr(M.t), N, N, scuffx._ptr(LU32), N, scuffx._ptr(ipiv),
+                                      scuffx._ptr(RHS.t), scuffx._ptr(x.t), 1e-12, 30, C.byref(it), C.byref(rel)))
+        t = tick("mixed", "solve_refine", t)
+        P = G.GetPower(omega, x, RHS)
+        tick("mixed", "power", t)
+        return P, it.value, rel.value
+
+    def step_fp64():
+        t = time.perf_counter()
+        G.AssembleBEMMatrix(omega, M)
+        t = tick("fp64", "assemble", t)
+        KN.t.copy_(RHS.t)
+        if M.LUFactorize() != 0:
+            raise RuntimeError("complex128 LU reported a zero pivot")
+        t = tick("fp64", "lu_c128", t)
+        M.LUSolve(KN)
+        t = tick("fp64", "solve", t)
+        P = G.GetPower(omega, KN, RHS)
+        tick("fp64", "power", t)
+        return P
+
+    def timed(fn, steps, label):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        for s in range(steps):
+            out = fn()
+            print(f"# {label} step {s + 1}/{steps}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / max(1, steps), out
+
+    for w in range(args.warmup):
+        step_mixed()
+        step_fp64()
+        print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+
+    # timed region: mixed path, every hot-kernel launch bracketed by HIP
+    # events on its stream
+    phases["mixed"].clear()
+    phases["fp64"].clear()
+    scuffx.profile(True)
+    t_mixed, (Pm, iters, relres) = timed(step_mixed, args.steps, "mixed")
+    prof_mixed = scuffx.profile_read()
+    asm_stats = G.AssemblyStats()
+    scuffx.profile(True)
+    t_fp64, Pd = timed(step_fp64, args.steps, "fp64")
+    prof_fp64 = scuffx.profile_read()
+    scuffx.profile(False)
+
+    perr = max(abs(Pm[k] - Pd[k]) / abs(Pd[k]) for k in ("PAbs", "PExt", "PScat"))
+
+    # dominant kernel of the mixed step
+    cand = {k: v for k, v in prof_mixed.items() if v[2] > 0}
+    dom = max(cand, key=lambda k: cand[k][0])
+    ms, work, launches = cand[dom]
+    if dom.startswith("gemm"):
+        bound, unit = "mfma", "TFLOP/s"
+        peak = PEAK["mfma_f32"] if dom.endswith("c64") else PEAK["mfma_f64"]
+        achieved = (work / launches) / (ms / launches * 1e-3)
+        scale = 1e12
+    elif dom in ("resid", "stamp"):
+        bound, unit, peak = "hbm", "GB/s", PEAK["hbm"]
+        achieved = work / (ms * 1e-3)
+        scale = 1e9
+    else:
+        bound, unit, peak = "mfma", "TFLOP/s", PEAK["mfma_f64"]
+        achieved = work / (ms * 1e-3)
+        scale = 1e12
+    roofline = {
+        "kernel": dom, "bound": bound, "achieved": achieved / scale, "peak": peak / scale, "unit": unit,
+        "frac": achieved / peak, "traffic": traffic_from_profiles(dom),
+        "launches": launches / args.steps, "avg_launch_ms": ms / launches,
+        "work_per_launch": work / launches,
+    }
+
+    def kern_summary(p):
+        return {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[2] / args.steps,
+                    "rate": (v[1] / (v[0] * 1e-3)) if v[0] > 0 and v[1] > 0 else None}
+                for k, v in p.items() if v[2] > 0}
+
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline (oracle sample)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.level, omega, args.cpu_rows, args.cpu_lu_n, eps)
+
+    W_lu = 8.0 / 3.0 * N ** 3
+    line = {
+        "metric": "N=15360 assemble+LU+solve s/freq, mixed vs FP64, at <=1e-8 power rel err",
+        "value": t_mixed, "unit": "s/freq", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 assembly + c64 LU + c128 refinement",
+        "data": "synthetic: icosphere mesh generated in-process",
+        "config": {"workload": f"dielectric sphere eps=12+0.01i R=1um omega={omega}, icosphere level "
+                               f"{args.level}, PMCHWT N={N}, x-polarized plane wave along +z",
+                   "N": N, "parallelism": "single GPU"},
+        "fp64_s_per_freq": t_fp64, "speedup_vs_fp64": t_fp64 / t_mixed,
+        "power_rel_err_vs_fp64": perr, "refine_iters": iters, "refine_relres": relres,
+        "power_mixed": Pm, "power_fp64": Pd,
+        "fippi_build_s": t_fippi, "assembly_ms": {k: asm_stats[k] for k in
+                                                  ("self_ms", "far_ms", "near_ms", "stamp_ms", "mirror_ms")},
+        "phase_s_per_freq": {p: {k: v / args.steps for k, v in d.items()} for p, d in phases.items()},
+        "lu_tflops": {"c64": W_lu / (phases["mixed"]["lu_c64"] / args.steps) / 1e12,
+                      "c128": W_lu / (phases["fp64"]["lu_c128"] / args.steps) / 1e12},
+        "kernels_mixed": kern_summary(prof_mixed), "kernels_fp64": kern_summary(prof_fp64),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/*
+ * scuffx.h -- C ABI of the MI355X BEM system path (assemble -> LU -> solve).
+ *
+ * Drop-in replacement, behind plain pointers and sizes, for the
+ * reference entry points of the path:
+ *   RWGGeometry::RWGGeometry         libs/libscuff/RWGGeometry.cc:353
+ *   RWGGeometry::AssembleBEMMatrix   libs/libscuff/libscuff.h:364-365,
+ *                                    AssembleBEMMatrix.cc:684-773
+ *   RWGGeometry::AssembleRHSVector   libs/libscuff/AssembleRHSVector.cc:365-427
+ *   HMatrix::LUFactorize             libs/libhmat/libhmat.h:311, LBWrappers.cc:151-170
+ *   HMatrix::LUSolve                 libs/libhmat/libhmat.h:312-317, LBWrappers.cc:175-245
+ *
+ * Conventions
+ *  - Matrices are column-major, complex numbers interleaved (re, im):
+ *    element (r, c) of M is at M[r + c*ldm] as in HMatrix (HMatrix.cc:325-342).
+ *  - Pointers named *_dev are HIP device pointers on the current device.
+ *  - Every call runs on the stream set by scuffx_set_stream (default: the
+ *    null stream) and returns after its work is complete.
+ *  - Frequencies are in units of c / 1 micron (3e14 rad/s), lengths in microns
+ *    (libs/libMatProp/libMatProp.cc:44).
+ *  - Return value: 0 or a LAPACK-style info (> 0) on success paths that
+ *    have one; negative SCUFFX_ERR_* on failure, with a message available
+ *    from scuffx_last_error().
+ */
+#ifndef SCUFFX_H
+#define SCUFFX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCUFFX_OK               0
+#define SCUFFX_ERR_ARG         -1
+#define SCUFFX_ERR_ALLOC       -2
+#define SCUFFX_ERR_HIP         -3
+#define SCUFFX_ERR_NUMERIC     -4
+#define SCUFFX_ERR_UNSUPPORTED -5
+#define SCUFFX_ERR_IO          -6
+
+typedef struct scuffx_geom scuffx_geom;
+
+/* message of the last failed call on this thread */
+const char *scuffx_last_error(void);
+int scuffx_version(void);
+/* HIP stream (hipStream_t) used by subsequent calls on this thread */
+int scuffx_set_stream(void *hip_stream);
+
+/* ---- geometry: RWGGeometry::RWGGeometry (RWGGeometry.cc:353-807) ---- */
+/* .scuffgeo subset: OBJECT / MESHFILE / MATERIAL / ENDOBJECT */
+int scuffx_geom_open(const char *scuffgeo_path, scuffx_geom **G);
+int scuffx_geom_new(scuffx_geom **G);
+/* material: "PEC", "CONST_EPS_<re>[+/-<im>i][_MU_..][_SIGMA_..]", "DRUDE_<wp>_<gamma>" */
+int scuffx_geom_add_mesh(scuffx_geom *G, const double *V, int nV, const int *T, int nT,
+                         const char *material);
+int scuffx_geom_add_msh(scuffx_geom *G, const char *msh_path, const char *material);
+int scuffx_geom_free(scuffx_geom *G);
+int scuffx_geom_total_bfs(const scuffx_geom *G);
+int scuffx_geom_num_surfaces(const scuffx_geom *G);
+int scuffx_geom_counts(const scuffx_geom *G, int ns, int *nV, int *nP, int *nE);
+/* per edge: iV1 iV2 iQP iQM iPPanel iMPanel PIndex MIndex Index (libscuff.h:120-146) */
+int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out9, double *lengths);
+/* per panel: centroid[3] zhat[3] radius area (libscuff.h:100-112) */
+int scuffx_geom_panels(const scuffx_geom *G, int ns, double *out8);
+/* test-mesh generator: V[3*(10*4^L+2)], T[3*20*4^L] */
+int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T);
+
+/* ---- frequency-independent panel-pair integrals: FIPPICache
+        (FIPPICache.cc:135-187), built once per geometry on the GPU ---- */
+int scuffx_fippi_build(scuffx_geom *G);
+int scuffx_fippi_count(const scuffx_geom *G);
+/* host copy, 36 doubles per record (QIFIPPIData, libscuffInternals.h:282-292) */
+int scuffx_fippi_records(scuffx_geom *G, double *out);
+
+/* ---- RWGGeometry::AssembleBEMMatrix(Omega, M) (AssembleBEMMatrix.cc:684) ----
+   fills all N x N entries of the complex128 matrix at M_dev */
+int scuffx_assemble(scuffx_geom *G, double omega_re, double omega_im, void *M_dev, long ldm);
+/* timings of the last assembly in ms: fippi self far near stamp mirror;
+   counts: far pairs, near pairs, self jobs, fippi records */
+int scuffx_assemble_stats(const scuffx_geom *G, double *ms6, long *counts4);
+
+/* ---- HMatrix::LUFactorize (LBWrappers.cc:151-170) ----
+   prec 64: complex128 (zgetrf), prec 32: complex64 (cgetrf).
+   ipiv_dev: int[n], 1-based row interchanges.  Returns LAPACK info. */
+int scuffx_lu_factor(void *A_dev, long n, long lda, int *ipiv_dev, int prec);
+/* ---- HMatrix::LUSolve (LBWrappers.cc:175-245); trans 'N', 'T' or 'C' ---- */
+int scuffx_lu_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev, char trans,
+                    long nrhs, void *B_dev, long ldb, int prec);
+
+/* ---- mixed precision: complex64 LU of a complex128 matrix plus
+        complex128 iterative refinement (no counterpart in the reference;
+        the plain path above is the drop-in) ---- */
+int scuffx_lu32_from_c128(const void *M_dev, long n, long ldm, void *LU32_dev, long ld32,
+                          int *ipiv_dev);
+/* x = M^{-1} b refined until |b - M x|_inf <= tol |b|_inf or maxiter steps */
+int scuffx_refine(const void *M_dev, long n, long ldm, const void *LU32_dev, long ld32,
+                  const int *ipiv_dev, const void *b_dev, void *x_dev, double tol, int maxiter,
+                  int *iters, double *relres);
+
+/* ---- callers either side of the path ---- */
+/* AssembleRHSVector for one plane wave in the exterior medium
+   (AssembleRHSVector.cc:365-427, PlaneWave.cc:71-87); E0 = 3 complex
+   as 6 doubles, nhat = propagation direction */
+int scuffx_rhs_planewave(scuffx_geom *G, double omega_re, double omega_im, const double *E0,
+                         const double *nhat, void *rhs_dev);
+/* absorbed power (crossed overlaps, OPFT.cc:310-364) and extinction
+   power from the solution KN and the RHS */
+int scuffx_power(scuffx_geom *G, double omega_re, double omega_im, const void *KN_dev,
+                 const void *rhs_dev, double *PAbs, double *PExt);
+
+/* ---- measurement (no counterpart in the reference) ----
+   on != 0: every launch of the hot kernels is bracketed by HIP events on
+   its stream; totals are reset.  Kernel names: gemm_c128 gemm_c64
+   panel_c128 panel_c64 trsm_c128 trsm_c64 far near self stamp fippi resid.
+   work = algorithmic flops (gemm, trsm, far, near) or bytes (stamp, resid). */
+int scuffx_profile(int on);
+int scuffx_profile_read(const char *kernel, double *ms, double *work, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

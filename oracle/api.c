@@ -132,7 +132,7 @@ void o_api_qifippi(const double *Va9, const double *Vb9, int ncv, double *out, s
 /* Taylor-Duffy with user kernels (known-answer tests) */
 void o_api_taylor_duffy(int ncv, int npk, const int *PIndex, const int *KIndex,
                         const double *KParam, const double *V, const double *Q,
-                        double *Result, int *nCalls)
+                        double *Result, int *nCalls, int maxeval)
 {
   double v[15], q[6];
   memcpy(v, V, sizeof(v));
@@ -145,6 +145,7 @@ void o_api_taylor_duffy(int ncv, int npk, const int *PIndex, const int *KIndex,
   T.V1 = v; T.V2 = v+3; T.V3 = v+6; T.V2P = v+9; T.V3P = v+12;
   T.Q = Q ? q : NULL; T.QP = Q ? q+3 : NULL;
   T.Result = R; T.Error = E;
+  if (maxeval > 0) T.MaxEval = maxeval;
   o_taylor_duffy(&T);
   for (int i = 0; i < npk; i++) { Result[2*i] = creal(R[i]); Result[2*i+1] = cimag(R[i]); }
   if (nCalls) *nCalls = T.nCalls;
@@ -196,3 +197,13 @@ int o_api_icosphere(int level, double radius, const double *center, double *V, i
 
 int o_api_write_gmsh(const char *fn, const double *V, int nV, const int *T, int nT)
 { return o_write_gmsh(fn, V, nV, T, nT); }
+
+/* triangle rule of order 4 or 20 as (u, v, w) triples; returns the count */
+int o_api_tcr(int order, double *out)
+{
+  int n = 0;
+  const double *r = o_tcr(order, &n);
+  if (!r) return 0;
+  if (out) memcpy(out, r, 3*sizeof(double)*n);
+  return n;
+}

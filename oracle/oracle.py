@@ -55,7 +55,7 @@ def lib():
     L.o_api_fippi_count.restype = C.c_size_t
     L.o_api_qifippi.argtypes = [_dp, _dp, C.c_int, _dp, C.POINTER(C.c_size_t)]
     L.o_api_taylor_duffy.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_void_p, _dp,
-                                     C.POINTER(C.c_int)]
+                                     C.POINTER(C.c_int), C.c_int]
     L.o_api_zgetrf.argtypes = [C.c_int, _zp, C.c_int, _ip]
     L.o_api_zgetrs.argtypes = [C.c_char, C.c_int, C.c_int, _zp, C.c_int, _ip, _zp, C.c_int]
     L.o_api_rhs.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp, _zv]
@@ -63,8 +63,54 @@ def lib():
     L.o_api_mie.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
     L.o_api_icosphere.argtypes = [C.c_int, C.c_double, C.c_void_p, _dp, _ip]
     L.o_api_write_gmsh.argtypes = [C.c_char_p, _dp, C.c_int, _ip, C.c_int]
+    L.o_api_tcr.argtypes = [C.c_int, C.c_void_p]
+    _CB = C.CFUNCTYPE(None, C.c_uint, C.POINTER(C.c_double), C.c_void_p, C.c_uint,
+                      C.POINTER(C.c_double))
+    L._integrand_type = _CB
+    for f in ("o_api_hcubature", "o_api_pcubature"):
+        getattr(L, f).argtypes = [C.c_uint, _CB, C.c_uint, _dp, _dp, C.c_size_t, C.c_double,
+                                  C.c_double, _dp, _dp]
     _LIB = L
     return L
+
+
+def tcr(order):
+    """Triangle cubature rule (order 4: 6 points, order 20: 78 points) as an
+    (n, 3) array of (u, v, weight)."""
+    n = lib().o_api_tcr(order, None)
+    out = np.zeros(3 * n)
+    lib().o_api_tcr(order, out.ctypes.data_as(C.c_void_p))
+    return out.reshape(n, 3)
+
+
+def _cubature(which, f, fdim, xmin, xmax, maxeval, abstol, reltol):
+    L = lib()
+    ncalls = [0]
+
+    def cb(ndim, x, _p, fd, fval):
+        xs = np.ctypeslib.as_array(x, shape=(ndim,))
+        v = f(xs.copy())
+        ncalls[0] += 1
+        for i in range(fd):
+            fval[i] = v[i]
+    cfn = L._integrand_type(cb)
+    xmin = np.ascontiguousarray(xmin, dtype=np.float64)
+    xmax = np.ascontiguousarray(xmax, dtype=np.float64)
+    val = np.zeros(fdim)
+    err = np.zeros(fdim)
+    getattr(L, which)(fdim, cfn, len(xmin), xmin, xmax, maxeval, abstol, reltol, val, err)
+    return val, err, ncalls[0]
+
+
+def hcubature(f, fdim, xmin, xmax, maxeval=0, abstol=0.0, reltol=1e-8):
+    """h-adaptive cubature of a vector integrand f(x) -> fdim values;
+    returns (value, error estimate, number of integrand calls)."""
+    return _cubature("o_api_hcubature", f, fdim, xmin, xmax, maxeval, abstol, reltol)
+
+
+def pcubature(f, fdim, xmin, xmax, maxeval=0, abstol=0.0, reltol=1e-8):
+    """p-adaptive Clenshaw-Curtis cubature; same conventions as hcubature."""
+    return _cubature("o_api_pcubature", f, fdim, xmin, xmax, maxeval, abstol, reltol)
 
 
 def icosphere(level, radius=1.0, center=None):
@@ -211,7 +257,7 @@ def qifippi(Va, Vb, ncv):
     return out, ne.value
 
 
-def taylor_duffy(ncv, PIndex, KIndex, KParam, V, Q=None):
+def taylor_duffy(ncv, PIndex, KIndex, KParam, V, Q=None, maxeval=0):
     """Known-answer interface to the Taylor-Duffy integrator.
     V = [V1, V2, V3, V2P, V3P] (5x3), Q = [Q, QP] (2x3) or None."""
     npk = len(PIndex)
@@ -227,5 +273,5 @@ def taylor_duffy(ncv, PIndex, KIndex, KParam, V, Q=None):
                              np.asarray(KIndex, dtype=np.int32), kp,
                              np.ascontiguousarray(V, dtype=np.float64).ravel(),
                              qarr.ctypes.data_as(C.c_void_p) if qarr is not None else None,
-                             res, C.byref(nc))
+                             res, C.byref(nc), maxeval)
     return res[0::2] + 1j * res[1::2], nc.value

@@ -1,0 +1,554 @@
+// assemble.cpp -- host orchestration of the BEM matrix assembly.
+//
+// Call structure restated from the reference:
+//   AssembleBEMMatrix (AssembleBEMMatrix.cc:684-773): surface pairs
+//   ns <= nsp, each block filled by GetSurfaceSurfaceInteractions
+//   (SurfaceSurfaceInteractions.cc:426-585), then M(r,c) = M(c,r) below
+//   the diagonal.  Inside a block every RWG pair sums four panel-pair
+//   integrals (EdgeEdgeInteractions.cc:154-188) whose algorithm is
+//   chosen per panel pair (PanelPanelInteractions.cc:357-561).
+// Here each class of panel pair is a separate kernel: far pairs (fixed
+// rule), near pairs (desingularized rule + cached singular terms), self
+// pairs (Taylor-Duffy); a final kernel combines them per edge pair.
+#include "assemble.h"
+#include "prof.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+namespace scuffx {
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { err_ = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x; return -3; } } while (0)
+
+static double dist3h(const double *a, const double *b)
+{
+  double d = 0;
+  for (int i = 0; i < 3; i++) d += (a[i] - b[i]) * (a[i] - b[i]);
+  return std::sqrt(d);
+}
+
+// relative centroid distance and common-vertex count with the common
+// vertices moved to the front (AssessPanelPair.cc:52-172).  ia/ib
+// receive the mesh-local vertex slots in call order.
+static int assess_pair(const Surface &Sa, int pa, const Surface &Sb, int pb, double *rRel, int ia[3], int ib[3])
+{
+  const Panel &Pa = Sa.Panels[pa], &Pb = Sb.Panels[pb];
+  for (int i = 0; i < 3; i++) { ia[i] = i; ib[i] = i; }
+  double DC, r2, rMax = std::fmax(Pa.Radius, Pb.Radius);
+  DC = Pa.Centroid[0] - Pb.Centroid[0]; r2 = DC * DC;
+  DC = Pa.Centroid[1] - Pb.Centroid[1]; r2 += DC * DC;
+  DC = Pa.Centroid[2] - Pb.Centroid[2]; r2 += DC * DC;
+  *rRel = std::sqrt(r2) / rMax;
+  if (*rRel > 2.0) return 0;
+  double thr = std::fabs(1.0e-6 * rMax);
+  int ncv = 0, CVa[3], CVb[3];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      const double *x = Sa.V.data() + 3 * Pa.VI[a], *y = Sb.V.data() + 3 * Pb.VI[b];
+      if (std::fabs(x[0] - y[0]) + std::fabs(x[1] - y[1]) + std::fabs(x[2] - y[2]) <= 3 * thr) {
+        CVa[ncv] = a; CVb[ncv] = b; ncv++;
+      }
+    }
+  if (ncv == 1) {
+    for (int i = 0; i < 3; i++) { ia[i] = (CVa[0] + i) % 3; ib[i] = (CVb[0] + i) % 3; }
+  } else if (ncv == 2) {
+    ia[0] = CVa[0]; ia[1] = CVa[1]; ia[2] = 3 - CVa[0] - CVa[1];
+    ib[0] = CVb[0]; ib[1] = CVb[1]; ib[2] = 3 - CVb[0] - CVb[1];
+  } else if (ncv == 3) {
+    for (int i = 0; i < 3; i++) { ia[i] = CVa[i]; ib[i] = CVb[i]; }
+  }
+  return ncv;
+}
+
+// vertex order with 1e-6 relative coordinate tolerance (AssessPanelPair.cc:250-267)
+static int vlt(const double *V1, const double *V2)
+{
+  double DV;
+  DV = V1[0] - V2[0]; if (std::fabs(DV) > 1.0e-6 * std::fabs(V1[0])) return DV < 0.0;
+  DV = V1[1] - V2[1]; if (std::fabs(DV) > 1.0e-6 * std::fabs(V1[1])) return DV < 0.0;
+  DV = V1[2] - V2[2]; if (std::fabs(DV) > 1.0e-6 * std::fabs(V1[2])) return DV < 0.0;
+  return 1;
+}
+
+// canonical ordering for the FIPPI key (AssessPanelPair.cc:342-455).
+// For ncv == 0 the reference compares Va[iMina] with Va[iMinb] and, in
+// the swapped case, takes OVa from Vb with panel a's sort indices; that
+// behaviour is kept because it decides which cached record a pair uses.
+static int canonical(const double *Va[3], const double *Vb[3], int ncv, const double *OVa[3], const double *OVb[3])
+{
+  int iMina, iMeda = 0, iMaxa, iMinb, iMaxb;
+  if (ncv == 3 || ncv == 0) {
+    if (vlt(Va[0], Va[1])) { iMina = 0; iMaxa = 1; } else { iMina = 1; iMaxa = 0; }
+    if (vlt(Va[2], Va[iMina])) { iMeda = iMina; iMina = 2; }
+    else if (vlt(Va[2], Va[iMaxa])) iMeda = 2;
+    else { iMeda = iMaxa; iMaxa = 2; }
+    if (ncv == 3) {
+      OVa[0] = Va[iMina]; OVa[1] = Va[iMeda]; OVa[2] = Va[iMaxa];
+      OVb[0] = Vb[iMina]; OVb[1] = Vb[iMeda]; OVb[2] = Vb[iMaxa];
+      return 0;
+    }
+    if (vlt(Vb[0], Vb[1])) { iMinb = 0; iMaxb = 1; } else { iMinb = 1; iMaxb = 0; }
+    if (vlt(Vb[2], Vb[iMinb])) iMinb = 2;
+    else if (vlt(Vb[2], Vb[iMaxb])) { }
+    else iMaxb = 2;
+    if (vlt(Va[iMina], Va[iMinb])) {
+      OVa[0] = Va[iMina]; OVa[1] = Va[iMeda]; OVa[2] = Va[iMaxa];
+      OVb[0] = Vb[iMina]; OVb[1] = Vb[iMeda]; OVb[2] = Vb[iMaxa];
+      return 0;
+    }
+    OVb[0] = Va[iMina]; OVb[1] = Va[iMeda]; OVb[2] = Va[iMaxa];
+    OVa[0] = Vb[iMina]; OVa[1] = Vb[iMeda]; OVa[2] = Vb[iMaxa];
+    return 1;
+  }
+  if (ncv == 2) {
+    int a01 = vlt(Va[0], Va[1]) ? 0 : 1;
+    if (vlt(Va[2], Vb[2])) {
+      OVa[0] = Va[a01]; OVa[1] = Va[1 - a01]; OVa[2] = Va[2];
+      OVb[0] = Vb[a01]; OVb[1] = Vb[1 - a01]; OVb[2] = Vb[2];
+      return 0;
+    }
+    OVa[0] = Vb[a01]; OVa[1] = Vb[1 - a01]; OVa[2] = Vb[2];
+    OVb[0] = Va[a01]; OVb[1] = Va[1 - a01]; OVb[2] = Va[2];
+    return 1;
+  }
+  int a12 = vlt(Va[1], Va[2]) ? 1 : 2, b12 = vlt(Vb[1], Vb[2]) ? 1 : 2;
+  if (vlt(Va[a12], Vb[b12])) {
+    OVa[0] = Va[0]; OVa[1] = Va[a12]; OVa[2] = Va[3 - a12];
+    OVb[0] = Vb[0]; OVb[1] = Vb[b12]; OVb[2] = Vb[3 - b12];
+    return 0;
+  }
+  OVa[0] = Vb[0]; OVa[1] = Vb[b12]; OVa[2] = Vb[3 - b12];
+  OVb[0] = Va[0]; OVb[1] = Va[a12]; OVb[2] = Va[3 - a12];
+  return 1;
+}
+
+struct Key15 {
+  float k[15];
+  bool operator==(const Key15 &o) const { return std::memcmp(k, o.k, sizeof(k)) == 0; }
+};
+struct Key15Hash {
+  size_t operator()(const Key15 &K) const
+  {
+    const unsigned char *p = (const unsigned char *)K.k;
+    uint64_t h = 1469598103934665603ULL;
+    for (int i = 0; i < 60; i++) { h ^= p[i]; h *= 1099511628211ULL; }
+    return (size_t)h;
+  }
+};
+
+Assembler::Assembler(const Geometry &G) : G_(G)
+{
+  ds_.resize(G.Surfaces.size());
+  for (size_t s = 0; s < G.Surfaces.size(); s++) {
+    const Surface &S = G.Surfaces[s];
+    DevSurface &D = ds_[s];
+    D.nV = S.NumVertices(); D.nP = S.NumPanels(); D.nE = S.NumEdges();
+    std::vector<int> pvi(3 * D.nP), ei(4 * D.nE);
+    std::vector<double> pc(4 * D.nP), el(D.nE);
+    for (int p = 0; p < D.nP; p++) {
+      for (int j = 0; j < 3; j++) { pvi[3 * p + j] = S.Panels[p].VI[j]; pc[4 * p + j] = S.Panels[p].Centroid[j]; }
+      pc[4 * p + 3] = S.Panels[p].Radius;
+    }
+    for (int e = 0; e < D.nE; e++) {
+      const Edge &E = S.Edges[e];
+      ei[4 * e] = E.iPPanel; ei[4 * e + 1] = E.iMPanel; ei[4 * e + 2] = E.PIndex; ei[4 * e + 3] = E.MIndex;
+      el[e] = E.Length;
+    }
+    if (hipMalloc(&D.V, sizeof(double) * 3 * D.nV) != hipSuccess ||
+        hipMalloc(&D.PVI, sizeof(int) * 3 * D.nP) != hipSuccess ||
+        hipMalloc(&D.PC, sizeof(double) * 4 * D.nP) != hipSuccess ||
+        hipMalloc(&D.EI, sizeof(int) * 4 * std::max(D.nE, 1)) != hipSuccess ||
+        hipMalloc(&D.EL, sizeof(double) * std::max(D.nE, 1)) != hipSuccess) {
+      err_ = "device allocation failed for the geometry";
+      return;
+    }
+    hipMemcpy(D.V, S.V.data(), sizeof(double) * 3 * D.nV, hipMemcpyHostToDevice);
+    hipMemcpy(D.PVI, pvi.data(), sizeof(int) * 3 * D.nP, hipMemcpyHostToDevice);
+    hipMemcpy(D.PC, pc.data(), sizeof(double) * 4 * D.nP, hipMemcpyHostToDevice);
+    if (D.nE) {
+      hipMemcpy(D.EI, ei.data(), sizeof(int) * 4 * D.nE, hipMemcpyHostToDevice);
+      hipMemcpy(D.EL, el.data(), sizeof(double) * D.nE, hipMemcpyHostToDevice);
+    }
+  }
+  hipMalloc(&d_regs_, sizeof(RegionK) * 4);
+  build_near();
+}
+
+Assembler::~Assembler()
+{
+  for (auto &D : ds_) { hipFree(D.V); hipFree(D.PVI); hipFree(D.PC); hipFree(D.EI); hipFree(D.EL); }
+  for (auto &B : blocks_) { hipFree(B.d_rowPtr); hipFree(B.d_col); hipFree(B.d_idx); hipFree(B.d_pairs); }
+  hipFree(d_jobs_); hipFree(d_records_);
+  hipFree(d_HP_); hipFree(d_HN_); hipFree(d_HS_); hipFree(d_scratch_);
+  hipFree(d_regs_);
+}
+
+int Assembler::ensure(double **p, size_t *cap, size_t need)
+{
+  if (need <= *cap && *p) return 0;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (need == 0) need = 1;
+  if (hipMalloc(p, need * sizeof(double)) != hipSuccess) { *p = nullptr; err_ = "device allocation failed"; return -2; }
+  *cap = need;
+  return 0;
+}
+
+// Near pairs of every surface-pair block, the order in which the
+// reference's serial edge-pair traversal first touches each ordered
+// pair, and the deduplicated FIPPI job list (first touch wins, as in
+// FIPPICache::GetQIFIPPIData, FIPPICache.cc:135-187).
+void Assembler::build_near()
+{
+  const int NS = (int)G_.Surfaces.size();
+  struct Occ { uint64_t pos; int blk, pa, pb; };
+  std::vector<Occ> occ;
+  int bi = 0;
+  for (int sa = 0; sa < NS; sa++)
+    for (int sb = sa; sb < NS; sb++, bi++) {
+      const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
+      NearBlock B;
+      B.sa = sa; B.sb = sb;
+      const int nPa = Sa.NumPanels(), nPb = Sb.NumPanels();
+      // uniform grid on panel centroids, cell = 4 x largest radius
+      double Rmax = 0;
+      for (auto &P : Sa.Panels) Rmax = std::max(Rmax, P.Radius);
+      for (auto &P : Sb.Panels) Rmax = std::max(Rmax, P.Radius);
+      double cell = 4.0 * Rmax * 1.000001 + 1e-300;
+      double lo[3] = {1e300, 1e300, 1e300};
+      for (auto &P : Sb.Panels) for (int j = 0; j < 3; j++) lo[j] = std::min(lo[j], P.Centroid[j]);
+      auto cidx = [&](const double *c, int j) { return (long)std::floor((c[j] - lo[j]) / cell); };
+      std::unordered_map<long long, std::vector<int>> grid;
+      auto gkey = [](long x, long y, long z) { return ((long long)(x + 1048576) << 42) ^ ((long long)(y + 1048576) << 21) ^ (long long)(z + 1048576); };
+      for (int p = 0; p < nPb; p++) {
+        const double *c = Sb.Panels[p].Centroid;
+        grid[gkey(cidx(c, 0), cidx(c, 1), cidx(c, 2))].push_back(p);
+      }
+      B.rowPtr.assign(nPa + 1, 0);
+      std::vector<std::vector<int>> rows(nPa);
+      for (int pa = 0; pa < nPa; pa++) {
+        const double *c = Sa.Panels[pa].Centroid;
+        long cx = cidx(c, 0), cy = cidx(c, 1), cz = cidx(c, 2);
+        for (long dx = -1; dx <= 1; dx++) for (long dy = -1; dy <= 1; dy++) for (long dz = -1; dz <= 1; dz++) {
+          auto it = grid.find(gkey(cx + dx, cy + dy, cz + dz));
+          if (it == grid.end()) continue;
+          for (int pb : it->second) {
+            if (sa == sb && pb == pa) continue;
+            int ia[3], ib[3];
+            double rRel;
+            int ncv = assess_pair(Sa, pa, Sb, pb, &rRel, ia, ib);
+            if (rRel > 4.0) continue;
+            if (ncv == 3) continue;   // coincident distinct panels: not supported
+            rows[pa].push_back(pb);
+          }
+        }
+        std::sort(rows[pa].begin(), rows[pa].end());
+      }
+      for (int pa = 0; pa < nPa; pa++) B.rowPtr[pa + 1] = B.rowPtr[pa] + (int)rows[pa].size();
+      B.col.resize(B.rowPtr[nPa]);
+      B.idx.assign(B.rowPtr[nPa], -1);
+      for (int pa = 0; pa < nPa; pa++) std::copy(rows[pa].begin(), rows[pa].end(), B.col.begin() + B.rowPtr[pa]);
+      blocks_.push_back(B);
+      // first touch of each ordered pair in the (ea, eb>=ea|0, PP/PM/MP/MM) walk
+      NearBlock &BB = blocks_.back();
+      const int nEa = Sa.NumEdges(), nEb = Sb.NumEdges();
+      std::vector<uint64_t> first(BB.col.size(), ~0ULL);
+      for (int ea = 0; ea < nEa; ea++) {
+        const Edge &Ea = Sa.Edges[ea];
+        int pas[2] = {Ea.iPPanel, Ea.iMPanel};
+        for (int s = 0; s < 2; s++) {
+          int pa = pas[s];
+          if (pa < 0) continue;
+          for (int q = BB.rowPtr[pa]; q < BB.rowPtr[pa + 1]; q++) {
+            int pb = BB.col[q];
+            for (int le = 0; le < 3; le++) {
+              int eb = Sb.Panels[pb].EI[le];
+              if (eb < 0 || eb >= nEb) continue;
+              if (Sb.Edges[eb].iPPanel != pb && Sb.Edges[eb].iMPanel != pb) continue;
+              if (sa == sb && eb < ea) continue;
+              int t = (Sb.Edges[eb].iPPanel == pb) ? 0 : 1;
+              uint64_t pos = ((uint64_t)ea * (uint64_t)nEb + (uint64_t)eb) * 4 + (uint64_t)(2 * s + t);
+              if (pos < first[q]) first[q] = pos;
+            }
+          }
+        }
+      }
+      for (size_t q = 0; q < BB.col.size(); q++) {
+        int pa = (int)(std::upper_bound(BB.rowPtr.begin(), BB.rowPtr.end(), (int)q) - BB.rowPtr.begin()) - 1;
+        occ.push_back({first[q], bi, pa, (int)q});
+      }
+    }
+  // global first-use order: blocks in order, then traversal position
+  std::stable_sort(occ.begin(), occ.end(), [](const Occ &x, const Occ &y) {
+    if (x.blk != y.blk) return x.blk < y.blk;
+    return x.pos < y.pos;
+  });
+  std::unordered_map<Key15, int, Key15Hash> keymap;
+  for (auto &B : blocks_) B.pairs.clear();
+  for (const Occ &o : occ) {
+    NearBlock &B = blocks_[o.blk];
+    const Surface &Sa = G_.Surfaces[B.sa], &Sb = G_.Surfaces[B.sb];
+    int q = o.pb;   // CSR slot
+    int pa = o.pa, pb = B.col[q];
+    int ia[3], ib[3];
+    double rRel;
+    int ncv = assess_pair(Sa, pa, Sb, pb, &rRel, ia, ib);
+    NearPair P;
+    std::memset(&P, 0, sizeof(P));
+    const double *Va[3], *Vb[3];
+    for (int i = 0; i < 3; i++) {
+      Va[i] = Sa.V.data() + 3 * Sa.Panels[pa].VI[ia[i]];
+      Vb[i] = Sb.V.data() + 3 * Sb.Panels[pb].VI[ib[i]];
+      for (int c = 0; c < 3; c++) {
+        P.Va[3 * i + c] = Va[i][c];
+        P.Vb[3 * i + c] = Vb[i][c];
+        P.Qa[3 * i + c] = Sa.V[3 * Sa.Panels[pa].VI[i] + c];
+        P.Qb[3 * i + c] = Sb.V[3 * Sb.Panels[pb].VI[i] + c];
+      }
+    }
+    const double *OVa[3], *OVb[3];
+    int fl = canonical(Va, Vb, ncv, OVa, OVb);
+    for (int c = 0; c < 3; c++) {
+      P.OVa0[c] = OVa[0][c]; P.OVa1[c] = OVa[1][c]; P.OVa2[c] = OVa[2][c];
+      P.OVb0[c] = OVb[0][c]; P.OVb1[c] = OVb[1][c]; P.OVb2[c] = OVb[2][c];
+    }
+    P.flipped = fl;
+    P.ncv = ncv;
+    Key15 K;
+    const double *src[5] = {OVa[1], OVa[2], OVb[0], OVb[1], OVb[2]};
+    for (int s = 0; s < 5; s++)
+      for (int m = 0; m < 3; m++) K.k[3 * s + m] = ((float)src[s][m]) - ((float)OVa[0][m]);
+    auto it = keymap.find(K);
+    if (it == keymap.end()) {
+      FippiJob J;
+      std::memset(&J, 0, sizeof(J));
+      for (int i = 0; i < 3; i++) for (int c = 0; c < 3; c++) { J.OVa[3 * i + c] = OVa[i][c]; J.OVb[3 * i + c] = OVb[i][c]; }
+      J.ncv = ncv;
+      int id = (int)jobs_.size();
+      jobs_.push_back(J);
+      keymap.emplace(K, id);
+      P.rec = id;
+    } else P.rec = it->second;
+    B.idx[q] = (int)B.pairs.size();
+    B.pairs.push_back(P);
+  }
+  for (auto &B : blocks_) {
+    hipMalloc(&B.d_rowPtr, sizeof(int) * B.rowPtr.size());
+    hipMalloc(&B.d_col, sizeof(int) * std::max<size_t>(1, B.col.size()));
+    hipMalloc(&B.d_idx, sizeof(int) * std::max<size_t>(1, B.idx.size()));
+    hipMalloc(&B.d_pairs, sizeof(NearPair) * std::max<size_t>(1, B.pairs.size()));
+    hipMemcpy(B.d_rowPtr, B.rowPtr.data(), sizeof(int) * B.rowPtr.size(), hipMemcpyHostToDevice);
+    if (!B.col.empty()) {
+      hipMemcpy(B.d_col, B.col.data(), sizeof(int) * B.col.size(), hipMemcpyHostToDevice);
+      hipMemcpy(B.d_idx, B.idx.data(), sizeof(int) * B.idx.size(), hipMemcpyHostToDevice);
+      hipMemcpy(B.d_pairs, B.pairs.data(), sizeof(NearPair) * B.pairs.size(), hipMemcpyHostToDevice);
+    }
+  }
+  stats_.nfippi = (long)jobs_.size();
+}
+
+int Assembler::build_fippi(hipStream_t stream)
+{
+  if (fippi_ready_) return 0;
+  if (!err_.empty()) return -3;
+  const int nj = (int)jobs_.size();
+  auto t0 = std::chrono::steady_clock::now();
+  if (nj > 0) {
+    HIPCHK(hipMalloc(&d_jobs_, sizeof(FippiJob) * nj));
+    HIPCHK(hipMalloc(&d_records_, sizeof(double) * SCUFFX_QIF_LEN * nj));
+    HIPCHK(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(FippiJob) * nj, hipMemcpyHostToDevice, stream));
+    int *d_counter = nullptr, *d_ovf = nullptr;
+    HIPCHK(hipMalloc(&d_counter, sizeof(int)));
+    HIPCHK(hipMalloc(&d_ovf, sizeof(int) * nj));
+    HIPCHK(hipMemsetAsync(d_ovf, 0, sizeof(int) * nj, stream));
+    HIPCHK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
+    // pass 1: small heaps for every job
+    int cap1 = 2048, slots1 = 1024;
+    double *hd = nullptr, *hk = nullptr;
+    int *hs = nullptr;
+    HIPCHK(hipMalloc(&hd, sizeof(double) * (size_t)cap1 * 76 * slots1));
+    HIPCHK(hipMalloc(&hk, sizeof(double) * (size_t)cap1 * slots1));
+    HIPCHK(hipMalloc(&hs, sizeof(int) * (size_t)cap1 * slots1));
+    int tok = prof_begin(stream, PK_FIPPI);
+    HIPCHK(launch_fippi(stream, d_jobs_, nj, d_records_, hd, hs, hk, cap1, slots1, d_counter, d_ovf, nullptr));
+    prof_end(stream, tok, 0.0);
+    std::vector<int> ovf(nj);
+    HIPCHK(hipMemcpyAsync(ovf.data(), d_ovf, sizeof(int) * nj, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    hipFree(hd); hipFree(hk); hipFree(hs);
+    std::vector<int> redo;
+    for (int j = 0; j < nj; j++) if (ovf[j]) redo.push_back(j);
+    if (!redo.empty()) {
+      // pass 2: the few jobs that outgrew the small heap, full-size heaps
+      int cap2 = 32768, slots2 = std::min<int>(128, (int)redo.size());
+      int *d_list = nullptr;
+      HIPCHK(hipMalloc(&d_list, sizeof(int) * redo.size()));
+      HIPCHK(hipMemcpyAsync(d_list, redo.data(), sizeof(int) * redo.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
+      HIPCHK(hipMemsetAsync(d_ovf, 0, sizeof(int) * nj, stream));
+      HIPCHK(hipMalloc(&hd, sizeof(double) * (size_t)cap2 * 76 * slots2));
+      HIPCHK(hipMalloc(&hk, sizeof(double) * (size_t)cap2 * slots2));
+      HIPCHK(hipMalloc(&hs, sizeof(int) * (size_t)cap2 * slots2));
+      tok = prof_begin(stream, PK_FIPPI);
+      HIPCHK(launch_fippi(stream, d_jobs_, (int)redo.size(), d_records_, hd, hs, hk, cap2, slots2, d_counter, d_ovf, d_list));
+      prof_end(stream, tok, 0.0);
+      HIPCHK(hipMemcpyAsync(ovf.data(), d_ovf, sizeof(int) * nj, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      hipFree(hd); hipFree(hk); hipFree(hs); hipFree(d_list);
+      for (int j = 0; j < nj; j++) if (ovf[j]) { err_ = "FIPPI cubature exceeded its region budget"; hipFree(d_counter); hipFree(d_ovf); return -4; }
+    }
+    hipFree(d_counter); hipFree(d_ovf);
+    if (prof_enabled()) prof_collect();
+  }
+  fippi_ready_ = true;
+  stats_.fippi_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+std::vector<double> Assembler::fippi_records()
+{
+  std::vector<double> r(jobs_.size() * SCUFFX_QIF_LEN);
+  if (!r.empty() && d_records_) hipMemcpy(r.data(), d_records_, sizeof(double) * r.size(), hipMemcpyDeviceToHost);
+  return r;
+}
+
+static cdouble csqrt2(cdouble z)
+{
+  cdouble r = std::sqrt(z);
+  return r.imag() < 0.0 ? -r : r;
+}
+
+// common regions of two surfaces and the relative sign of their
+// currents there (SurfaceSurfaceInteractions.cc:62-90)
+static int common_regions(const Surface &Sa, const Surface &Sb, int CR[2], double Sg[2])
+{
+  int n = 0;
+  if (Sa.RegionIndices[0] == Sb.RegionIndices[0]) { CR[n] = Sa.RegionIndices[0]; Sg[n] = +1.0; n++; }
+  else if (Sa.RegionIndices[0] == Sb.RegionIndices[1]) { CR[n] = Sa.RegionIndices[0]; Sg[n] = -1.0; n++; }
+  if (Sa.RegionIndices[1] == Sb.RegionIndices[0]) { CR[n] = Sa.RegionIndices[1]; Sg[n] = -1.0; n++; }
+  else if (!Sa.IsPEC && !Sb.IsPEC && Sa.RegionIndices[1] == Sb.RegionIndices[1]) { CR[n] = Sa.RegionIndices[1]; Sg[n] = +1.0; n++; }
+  return n;
+}
+
+int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
+{
+  if (!err_.empty()) return -3;
+  if (ldm < G_.TotalBFs) { err_ = "leading dimension smaller than the matrix"; return -1; }
+  int rc = build_fippi(stream);
+  if (rc) return rc;
+  const int NS = (int)G_.Surfaces.size();
+  std::vector<cdouble> eps(G_.Regions.size()), mu(G_.Regions.size());
+  for (size_t r = 0; r < G_.Regions.size(); r++) G_.Regions[r].eps_mu(omega, &eps[r], &mu[r]);
+  hipEvent_t ev[6];
+  for (auto &e : ev) hipEventCreate(&e);
+  double t_self = 0, t_far = 0, t_near = 0, t_stamp = 0;
+  int bi = 0;
+  stats_.nfar = stats_.nnear = stats_.nself = 0;
+  for (int sa = 0; sa < NS; sa++)
+    for (int sb = sa; sb < NS; sb++, bi++) {
+      const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
+      const DevSurface &Da = ds_[sa], &Db = ds_[sb];
+      NearBlock &B = blocks_[bi];
+      const int same = (sa == sb);
+      int CR[2]; double Sg[2];
+      int nreg = common_regions(Sa, Sb, CR, Sg);
+      RegionK rk[2];
+      int used = 0;
+      for (int r = 0; r < nreg; r++) {
+        cdouble E = eps[CR[r]], Mu = mu[CR[r]];
+        if (E == 0.0) continue;
+        cdouble k = csqrt2(E * Mu) * omega, I(0, 1);
+        cdouble pf1 = Sg[r] * I * Mu * omega, pf2 = -Sg[r] * I * k, pf3 = -Sg[r] * I * E * omega;
+        rk[used].k = mk(k.real(), k.imag());
+        rk[used].pf1 = mk(pf1.real(), pf1.imag());
+        rk[used].pf2 = mk(pf2.real(), pf2.imag());
+        rk[used].pf3 = mk(pf3.real(), pf3.imag());
+        // regimes handled on the GPU: every near pair in the
+        // desingularization regime |k| R_max <= pi, self pairs with
+        // |k| R <= 20 (PanelPanelInteractions.cc:429-491)
+        double Rmax = 0;
+        for (auto &P : Sa.Panels) Rmax = std::max(Rmax, P.Radius);
+        for (auto &P : Sb.Panels) Rmax = std::max(Rmax, P.Radius);
+        if (std::abs(k) * Rmax > M_PI) {
+          err_ = "short-wavelength regime (|k| R_panel > pi) is not implemented on the GPU path";
+          for (auto &e : ev) hipEventDestroy(e);
+          return -5;
+        }
+        used++;
+      }
+      nreg = used;
+      const int needH1 = !(Sa.IsPEC && Sb.IsPEC);
+      const int stride = needH1 ? 4 : 2;
+      HIPCHK(hipMemcpyAsync(d_regs_ + 2 * (bi % 2), rk, sizeof(RegionK) * std::max(nreg, 1), hipMemcpyHostToDevice, stream));
+      RegionK *dreg = d_regs_ + 2 * (bi % 2);
+      long npairs = same ? (long)Da.nP * (Da.nP - 1) / 2 : (long)Da.nP * Db.nP;
+      if (ensure(&d_HP_, &cap_HP_, (size_t)npairs * nreg * 9 * stride)) return -2;
+      if (ensure(&d_HN_, &cap_HN_, (size_t)B.pairs.size() * nreg * 9 * stride)) return -2;
+      if (same) {
+        if (ensure(&d_HS_, &cap_HS_, (size_t)Da.nP * nreg * 18)) return -2;
+        int chunk = 16384;
+        if (ensure(&d_scratch_, &cap_scratch_, (size_t)chunk * 1025 * 4)) return -2;
+        hipEventRecord(ev[0], stream);
+        int tok = prof_begin(stream, PK_SELF);
+        if (nreg > 0) HIPCHK(launch_self(stream, Da.V, Da.PVI, Da.nP, nreg, dreg, d_HS_, d_scratch_, chunk));
+        prof_end(stream, tok, 0.0);
+        hipEventRecord(ev[1], stream);
+        stats_.nself += (long)Da.nP * 9 * nreg;
+      } else {
+        hipEventRecord(ev[0], stream);
+        hipEventRecord(ev[1], stream);
+      }
+      // algorithmic flops per (panel pair, region): 36 point pairs of 98
+      // (PMCHWT) or 56 (EFIE) flops plus the 9-combination epilogue
+      const double wpair = needH1 ? 3780.0 : 2160.0;
+      const long nfar_blk = npairs - (same ? (long)B.col.size() / 2 : (long)B.col.size());
+      if (nreg > 0) {
+        int tok = prof_begin(stream, PK_FAR);
+        HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, Da.nP, Db.V, Db.PVI, Db.PC, Db.nP, same,
+                          B.d_rowPtr, B.d_col, nreg, dreg, needH1, d_HP_));
+        prof_end(stream, tok, wpair * (double)nfar_blk * nreg);
+      }
+      hipEventRecord(ev[2], stream);
+      if (nreg > 0 && !B.pairs.empty()) {
+        int tok = prof_begin(stream, PK_NEAR);
+        HIPCHK(launch_near(stream, B.d_pairs, (int)B.pairs.size(), d_records_, nreg, dreg, needH1, d_HN_));
+        prof_end(stream, tok, wpair * (double)B.pairs.size() * nreg);
+      }
+      hipEventRecord(ev[3], stream);
+      {
+        // bytes of matrix written by the block (upper triangle when same)
+        double nbf = (double)Sa.NumBFs * (double)Sb.NumBFs;
+        int tok = prof_begin(stream, PK_STAMP);
+        HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same,
+                            Sa.IsPEC, Sb.IsPEC, nreg, dreg, needH1, d_HP_, d_HN_, d_HS_,
+                            B.d_rowPtr, B.d_col, B.d_idx, M, ldm, G_.BFIndexOffset[sa], G_.BFIndexOffset[sb]));
+        prof_end(stream, tok, 16.0 * (same ? 0.5 * nbf : nbf));
+      }
+      hipEventRecord(ev[4], stream);
+      HIPCHK(hipEventSynchronize(ev[4]));
+      float ms;
+      hipEventElapsedTime(&ms, ev[0], ev[1]); t_self += ms;
+      hipEventElapsedTime(&ms, ev[1], ev[2]); t_far += ms;
+      hipEventElapsedTime(&ms, ev[2], ev[3]); t_near += ms;
+      hipEventElapsedTime(&ms, ev[3], ev[4]); t_stamp += ms;
+      stats_.nfar += nfar_blk;
+      stats_.nnear += (long)B.pairs.size();
+    }
+  hipEventRecord(ev[0], stream);
+  HIPCHK(launch_mirror(stream, M, ldm, G_.TotalBFs));
+  hipEventRecord(ev[1], stream);
+  HIPCHK(hipEventSynchronize(ev[1]));
+  float ms;
+  hipEventElapsedTime(&ms, ev[0], ev[1]);
+  stats_.mirror_ms = ms;
+  stats_.self_ms = t_self; stats_.far_ms = t_far; stats_.near_ms = t_near; stats_.stamp_ms = t_stamp;
+  for (auto &e : ev) hipEventDestroy(e);
+  if (prof_enabled()) prof_collect();
+  return 0;
+}
+
+}  // namespace scuffx

@@ -1,0 +1,68 @@
+// assemble.h -- host orchestration of the BEM matrix assembly on one GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "bem.h"
+#include "geometry.h"
+
+namespace scuffx {
+
+struct DevSurface {
+  double *V = nullptr;
+  int *PVI = nullptr;
+  double *PC = nullptr;
+  int *EI = nullptr;
+  double *EL = nullptr;
+  int nV = 0, nP = 0, nE = 0;
+};
+
+// near-pair bookkeeping of one surface-pair block
+struct NearBlock {
+  int sa = 0, sb = 0;
+  std::vector<int> rowPtr, col, idx;   // CSR over row panels, sorted columns
+  std::vector<NearPair> pairs;         // first-use order
+  int *d_rowPtr = nullptr, *d_col = nullptr, *d_idx = nullptr;
+  NearPair *d_pairs = nullptr;
+};
+
+struct AsmStats {
+  double fippi_ms = 0, self_ms = 0, far_ms = 0, near_ms = 0, stamp_ms = 0, mirror_ms = 0;
+  long nfar = 0, nnear = 0, nself = 0, nfippi = 0;
+};
+
+class Assembler {
+ public:
+  explicit Assembler(const Geometry &G);
+  ~Assembler();
+  // fills the full N x N matrix (column-major complex128, device memory)
+  int assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream);
+  int build_fippi(hipStream_t stream);
+  const AsmStats &stats() const { return stats_; }
+  const DevSurface &dev_surface(int s) const { return ds_[s]; }
+  std::string error() const { return err_; }
+  int num_fippi() const { return (int)jobs_.size(); }
+  // host copy of the FIPPI records (36 doubles each) for checks
+  std::vector<double> fippi_records();
+
+ private:
+  const Geometry &G_;
+  std::vector<DevSurface> ds_;
+  std::vector<NearBlock> blocks_;
+  std::vector<FippiJob> jobs_;
+  FippiJob *d_jobs_ = nullptr;
+  double *d_records_ = nullptr;
+  bool fippi_ready_ = false;
+  double *d_HP_ = nullptr, *d_HN_ = nullptr, *d_HS_ = nullptr, *d_scratch_ = nullptr;
+  size_t cap_HP_ = 0, cap_HN_ = 0, cap_HS_ = 0, cap_scratch_ = 0;
+  RegionK *d_regs_ = nullptr;
+  AsmStats stats_;
+  std::string err_;
+  void build_near();
+  int ensure(double **p, size_t *cap, size_t need);
+};
+
+}  // namespace scuffx

@@ -1,0 +1,75 @@
+// bem.h -- device data layout and kernel entry points of the BEM matrix
+// assembly (host orchestration in assemble.cpp).
+//
+// Layout in HBM, per surface:
+//   V      double[3*nV]                vertex coordinates
+//   PVI    int[3*nP]                   panel vertex indices (mesh order)
+//   PC     double[4*nP]                centroid xyz, bounding radius
+//   EI     int[4*nE]                   iPPanel, iMPanel, PIndex, MIndex
+//   EL     double[nE]                  edge length
+// Panel-pair integrals, one record per (panel pair, region):
+//   far    unordered pairs pa<pb (same surface, packed upper triangle)
+//          or all pa x pb (distinct surfaces); 9 (Qa,Qb) combinations x
+//          {H0, H1} complex = 36 doubles (H1 omitted for PEC-PEC blocks)
+//   near   ordered pairs (pa as the row panel), same record layout
+//   self   panel p with itself, 9 combinations, H0 only
+// The matrix M is column-major complex128, M[r + c*ldm].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "devmath.h"
+
+namespace scuffx {
+
+// one unique frequency-independent panel-pair integral job
+struct FippiJob {
+  double OVa[9];      // canonically ordered vertices of panel a
+  double OVb[9];      // canonically ordered vertices of panel b
+  int ncv;
+  int pad;
+};
+#define SCUFFX_QIF_LEN 36   // xMxpRM3[3] xXxpRM3[3] RM1[9] R1[9] R2[9] x0[3]
+
+// one ordered near pair (0 < ncv < 3 or ncv == 0 with rRel <= 4)
+struct NearPair {
+  double Va[9];       // vertices ordered common-first (AssessPanelPair)
+  double Vb[9];
+  double Qa[9];       // panel a vertices in mesh order (Q candidates)
+  double Qb[9];
+  double OVa0[3], OVa1[3], OVa2[3];   // canonical order (for QDFIPPI)
+  double OVb0[3], OVb1[3], OVb2[3];
+  int flipped;
+  int ncv;
+  int rec;            // FIPPI record index
+  int pad;
+};
+
+struct RegionK {
+  cplx k;             // wavenumber in the region
+  cplx pf1, pf2, pf3; // i*mu*w*s, -i*k*s, -i*eps*w*s
+};
+
+// launch wrappers (kern_*.hip); all asynchronous on `stream`
+hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int nPa,
+                      const double *Vb, const int *PVIb, const double *PCb, int nPb, int same,
+                      const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
+                      int needH1, double *HP);
+hipError_t launch_near(hipStream_t stream, const NearPair *np, int nnear, const double *records,
+                       int nreg, const RegionK *regs, int needH1, double *HN);
+hipError_t launch_self(hipStream_t stream, const double *V, const int *PVI, int nP, int nreg,
+                       const RegionK *regs, double *HS, double *scratch, int chunk);
+hipError_t launch_fippi(hipStream_t stream, const FippiJob *jobs, int njobs, double *records,
+                        double *heapData, int *heapSlot, double *heapKey, int cap, int nslots,
+                        int *counter, int *overflow, const int *jobList);
+hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const double *ELa, int nPa,
+                        int nEb, const int *EIb, const double *ELb, int nPb, int same,
+                        int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
+                        const double *HP, const double *HN, const double *HS,
+                        const int *nearRowPtr, const int *nearCol, const int *nearIdx,
+                        double2 *M, long ldm, int rowOff, int colOff);
+hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n);
+
+}  // namespace scuffx

@@ -1,0 +1,364 @@
+// capi.cpp -- extern "C" boundary (include/scuffx.h) over the host
+// geometry, the assembly orchestrator and the LU module.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/scuffx.h"
+#include "assemble.h"
+#include "geometry.h"
+#include "lu.h"
+#include "prof.h"
+#include "rhs.h"
+
+using namespace scuffx;
+
+static thread_local std::string g_err;
+static thread_local hipStream_t g_stream = nullptr;
+
+static int fail(int code, const std::string &msg) { g_err = msg; return code; }
+
+struct DevRhs {
+  double *PA = nullptr, *PZ = nullptr, *work = nullptr;
+  int *PEI = nullptr, *EV = nullptr;
+};
+
+struct scuffx_geom {
+  Geometry G;
+  std::unique_ptr<Assembler> A;
+  std::vector<DevRhs> rhs;
+  double *d_out = nullptr;
+  ~scuffx_geom()
+  {
+    for (auto &r : rhs) { hipFree(r.PA); hipFree(r.PZ); hipFree(r.work); hipFree(r.PEI); hipFree(r.EV); }
+    hipFree(d_out);
+  }
+  Assembler *assembler()
+  {
+    if (!A) A.reset(new Assembler(G));
+    return A.get();
+  }
+};
+
+extern "C" {
+
+const char *scuffx_last_error(void) { return g_err.c_str(); }
+int scuffx_version(void) { return 100; }
+int scuffx_set_stream(void *s) { g_stream = (hipStream_t)s; return 0; }
+
+int scuffx_geom_new(scuffx_geom **G)
+{
+  if (!G) return fail(SCUFFX_ERR_ARG, "null handle pointer");
+  *G = new scuffx_geom();
+  return 0;
+}
+
+int scuffx_geom_open(const char *path, scuffx_geom **G)
+{
+  if (!path || !G) return fail(SCUFFX_ERR_ARG, "null argument");
+  scuffx_geom *g = new scuffx_geom();
+  std::string e = read_scuffgeo(path, &g->G);
+  if (!e.empty()) { delete g; return fail(SCUFFX_ERR_IO, e); }
+  *G = g;
+  return 0;
+}
+
+static int add_surface_checked(scuffx_geom *G, const double *V, int nV, const int *T, int nT, const char *mat)
+{
+  if (G->A) return fail(SCUFFX_ERR_ARG, "geometry is frozen after its first assembly");
+  Material m;
+  bool pec;
+  std::string e = parse_material(mat ? mat : "PEC", &m, &pec);
+  if (!e.empty()) return fail(SCUFFX_ERR_ARG, e);
+  Surface S;
+  e = build_surface(V, nV, T, nT, pec, &S);
+  if (!e.empty()) return fail(SCUFFX_ERR_ARG, e);
+  return add_surface(&G->G, std::move(S), m);
+}
+
+int scuffx_geom_add_mesh(scuffx_geom *G, const double *V, int nV, const int *T, int nT, const char *mat)
+{
+  if (!G || !V || !T || nV <= 0 || nT <= 0) return fail(SCUFFX_ERR_ARG, "bad mesh arguments");
+  return add_surface_checked(G, V, nV, T, nT, mat);
+}
+
+int scuffx_geom_add_msh(scuffx_geom *G, const char *path, const char *mat)
+{
+  if (!G || !path) return fail(SCUFFX_ERR_ARG, "null argument");
+  std::vector<double> V;
+  std::vector<int> T;
+  std::string e = read_gmsh(path, &V, &T);
+  if (!e.empty()) return fail(SCUFFX_ERR_IO, e);
+  return add_surface_checked(G, V.data(), (int)V.size() / 3, T.data(), (int)T.size() / 3, mat);
+}
+
+int scuffx_geom_free(scuffx_geom *G) { delete G; return 0; }
+int scuffx_geom_total_bfs(const scuffx_geom *G) { return G ? G->G.TotalBFs : fail(SCUFFX_ERR_ARG, "null geometry"); }
+int scuffx_geom_num_surfaces(const scuffx_geom *G) { return G ? (int)G->G.Surfaces.size() : fail(SCUFFX_ERR_ARG, "null geometry"); }
+
+int scuffx_geom_counts(const scuffx_geom *G, int ns, int *nV, int *nP, int *nE)
+{
+  if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
+  const Surface &S = G->G.Surfaces[ns];
+  if (nV) *nV = S.NumVertices();
+  if (nP) *nP = S.NumPanels();
+  if (nE) *nE = S.NumEdges();
+  return 0;
+}
+
+int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out, double *len)
+{
+  if (!G || !out || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  const Surface &S = G->G.Surfaces[ns];
+  for (int e = 0; e < S.NumEdges(); e++) {
+    const Edge &E = S.Edges[e];
+    int *o = out + 9 * e;
+    o[0] = E.iV1; o[1] = E.iV2; o[2] = E.iQP; o[3] = E.iQM; o[4] = E.iPPanel; o[5] = E.iMPanel;
+    o[6] = E.PIndex; o[7] = E.MIndex; o[8] = E.Index;
+    if (len) len[e] = E.Length;
+  }
+  return 0;
+}
+
+int scuffx_geom_panels(const scuffx_geom *G, int ns, double *out)
+{
+  if (!G || !out || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  const Surface &S = G->G.Surfaces[ns];
+  for (int p = 0; p < S.NumPanels(); p++) {
+    const Panel &P = S.Panels[p];
+    double *o = out + 8 * p;
+    for (int j = 0; j < 3; j++) { o[j] = P.Centroid[j]; o[3 + j] = P.ZHat[j]; }
+    o[6] = P.Radius; o[7] = P.Area;
+  }
+  return 0;
+}
+
+int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T)
+{
+  if (level < 0 || level > 9 || !V || !T) return fail(SCUFFX_ERR_ARG, "bad icosphere arguments");
+  std::vector<double> v;
+  std::vector<int> t;
+  icosphere(level, radius, center, &v, &t);
+  std::memcpy(V, v.data(), sizeof(double) * v.size());
+  std::memcpy(T, t.data(), sizeof(int) * t.size());
+  return 0;
+}
+
+int scuffx_fippi_build(scuffx_geom *G)
+{
+  if (!G || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "empty geometry");
+  Assembler *A = G->assembler();
+  int rc = A->build_fippi(g_stream);
+  if (rc) return fail(rc, A->error());
+  return 0;
+}
+
+int scuffx_fippi_count(const scuffx_geom *G)
+{
+  if (!G) return fail(SCUFFX_ERR_ARG, "null geometry");
+  return G->A ? G->A->num_fippi() : 0;
+}
+
+int scuffx_fippi_records(scuffx_geom *G, double *out)
+{
+  if (!G || !out) return fail(SCUFFX_ERR_ARG, "null argument");
+  if (!G->A) return 0;
+  std::vector<double> r = G->A->fippi_records();
+  std::memcpy(out, r.data(), sizeof(double) * r.size());
+  return 0;
+}
+
+int scuffx_assemble(scuffx_geom *G, double wr, double wi, void *M, long ldm)
+{
+  if (!G || !M || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
+  Assembler *A = G->assembler();
+  int rc = A->assemble(cdouble(wr, wi), (double2 *)M, ldm, g_stream);
+  if (rc) return fail(rc, A->error());
+  return 0;
+}
+
+int scuffx_assemble_stats(const scuffx_geom *G, double *ms, long *counts)
+{
+  if (!G || !G->A) return fail(SCUFFX_ERR_ARG, "no assembly yet");
+  const AsmStats &s = G->A->stats();
+  if (ms) { ms[0] = s.fippi_ms; ms[1] = s.self_ms; ms[2] = s.far_ms; ms[3] = s.near_ms; ms[4] = s.stamp_ms; ms[5] = s.mirror_ms; }
+  if (counts) { counts[0] = s.nfar; counts[1] = s.nnear; counts[2] = s.nself; counts[3] = s.nfippi; }
+  return 0;
+}
+
+int scuffx_lu_factor(void *A, long n, long lda, int *ipiv, int prec)
+{
+  if (!A || !ipiv || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad LU arguments");
+  int info = 0, rc;
+  if (prec == 64) rc = lu_factor<double2>((double2 *)A, n, lda, ipiv, &info, g_stream);
+  else if (prec == 32) rc = lu_factor<float2>((float2 *)A, n, lda, ipiv, &info, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  if (rc) return fail(rc, "LU factorization failed on the device");
+  return info;
+}
+
+int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char trans, long nrhs, void *B, long ldb, int prec)
+{
+  if (!LU || !ipiv || !B) return fail(SCUFFX_ERR_ARG, "null argument");
+  int rc;
+  if (prec == 64) rc = lu_solve<double2>((const double2 *)LU, n, lda, ipiv, trans, nrhs, (double2 *)B, ldb, g_stream);
+  else if (prec == 32) rc = lu_solve<float2>((const float2 *)LU, n, lda, ipiv, trans, nrhs, (float2 *)B, ldb, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  if (rc == -1) return fail(SCUFFX_ERR_ARG, "bad LU solve arguments");
+  if (rc) return fail(rc, "LU solve failed on the device");
+  if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
+  return 0;
+}
+
+int scuffx_lu32_from_c128(const void *M, long n, long ldm, void *LU32, long ld32, int *ipiv)
+{
+  if (!M || !LU32 || !ipiv || ldm < n || ld32 < n) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (convert_c128_to_c64((const double2 *)M, (float2 *)LU32, n, n, ldm, ld32, g_stream)) return fail(SCUFFX_ERR_HIP, "conversion failed");
+  return scuffx_lu_factor(LU32, n, ld32, ipiv, 32);
+}
+
+int scuffx_refine(const void *M, long n, long ldm, const void *LU32, long ld32, const int *ipiv,
+                  const void *b, void *x, double tol, int maxiter, int *iters, double *relres)
+{
+  if (!M || !LU32 || !ipiv || !b || !x || n <= 0) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  int nch = (int)((n + 511) / 512);
+  double2 *r = nullptr, *work = nullptr;
+  float2 *r32 = nullptr;
+  double *scal = nullptr;
+  if (hipMalloc(&r, sizeof(double2) * n) != hipSuccess || hipMalloc(&work, sizeof(double2) * n * nch) != hipSuccess ||
+      hipMalloc(&r32, sizeof(float2) * n) != hipSuccess || hipMalloc(&scal, sizeof(double)) != hipSuccess)
+    return fail(SCUFFX_ERR_ALLOC, "workspace allocation failed");
+  hipStream_t s = g_stream;
+  const double2 *bb = (const double2 *)b;
+  double2 *xx = (double2 *)x;
+  double bn = absmax_c128(bb, n, scal, s);
+  // x0 from the complex64 factors
+  convert_c128_to_c64(bb, r32, n, 1, n, n, s);
+  lu_solve<float2>((const float2 *)LU32, n, ld32, ipiv, 'N', 1, r32, n, s);
+  convert_c64_to_c128(r32, xx, n, s);
+  int it = 0;
+  double rel = 0, prev = 1e300;
+  for (;;) {
+    int tok = prof_begin(s, PK_RESID);
+    residual_c128((const double2 *)M, n, ldm, xx, bb, r, work, s);
+    prof_end(s, tok, 16.0 * (double)n * (double)n);
+    rel = absmax_c128(r, n, scal, s) / (bn > 0 ? bn : 1.0);
+    if (rel <= tol || it >= maxiter || rel >= prev) break;
+    prev = rel;
+    convert_c128_to_c64(r, r32, n, 1, n, n, s);
+    lu_solve<float2>((const float2 *)LU32, n, ld32, ipiv, 'N', 1, r32, n, s);
+    axpy_c64_into_c128(r32, xx, n, s);
+    it++;
+  }
+  hipStreamSynchronize(s);
+  if (prof_enabled()) prof_collect();
+  hipFree(r); hipFree(work); hipFree(r32); hipFree(scal);
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  return 0;
+}
+
+static int ensure_rhs_data(scuffx_geom *G)
+{
+  if (!G->rhs.empty()) return 0;
+  for (size_t s = 0; s < G->G.Surfaces.size(); s++) {
+    const Surface &S = G->G.Surfaces[s];
+    DevRhs R;
+    std::vector<double> pa(S.NumPanels()), pz(3 * S.NumPanels());
+    std::vector<int> pei(3 * S.NumPanels()), ev(4 * S.NumEdges());
+    for (int p = 0; p < S.NumPanels(); p++) {
+      pa[p] = S.Panels[p].Area;
+      for (int j = 0; j < 3; j++) { pz[3 * p + j] = S.Panels[p].ZHat[j]; pei[3 * p + j] = S.Panels[p].EI[j]; }
+    }
+    for (int e = 0; e < S.NumEdges(); e++) {
+      ev[4 * e] = S.Edges[e].iV1; ev[4 * e + 1] = S.Edges[e].iV2; ev[4 * e + 2] = S.Edges[e].iQP; ev[4 * e + 3] = S.Edges[e].iQM;
+    }
+    if (hipMalloc(&R.PA, sizeof(double) * pa.size()) || hipMalloc(&R.PZ, sizeof(double) * pz.size()) ||
+        hipMalloc(&R.PEI, sizeof(int) * pei.size()) || hipMalloc(&R.EV, sizeof(int) * std::max<size_t>(1, ev.size())) ||
+        hipMalloc(&R.work, sizeof(double) * 2 * std::max(1, S.NumEdges())))
+      return fail(SCUFFX_ERR_ALLOC, "allocation failed");
+    hipMemcpy(R.PA, pa.data(), sizeof(double) * pa.size(), hipMemcpyHostToDevice);
+    hipMemcpy(R.PZ, pz.data(), sizeof(double) * pz.size(), hipMemcpyHostToDevice);
+    hipMemcpy(R.PEI, pei.data(), sizeof(int) * pei.size(), hipMemcpyHostToDevice);
+    if (!ev.empty()) hipMemcpy(R.EV, ev.data(), sizeof(int) * ev.size(), hipMemcpyHostToDevice);
+    G->rhs.push_back(R);
+  }
+  if (!G->d_out && hipMalloc(&G->d_out, sizeof(double) * 2)) return fail(SCUFFX_ERR_ALLOC, "allocation failed");
+  return 0;
+}
+
+int scuffx_rhs_planewave(scuffx_geom *G, double wr, double wi, const double *E0, const double *nhat, void *rhs)
+{
+  if (!G || !E0 || !nhat || !rhs) return fail(SCUFFX_ERR_ARG, "null argument");
+  Assembler *A = G->assembler();
+  if (!A->error().empty()) return fail(SCUFFX_ERR_HIP, A->error());
+  int rc = ensure_rhs_data(G);
+  if (rc) return rc;
+  cdouble omega(wr, wi), eps, mu;
+  G->G.Regions[0].eps_mu(omega, &eps, &mu);
+  cdouble K = std::sqrt(eps * mu) * omega, Z = 376.73031346177 * std::sqrt(mu / eps);
+  cdouble e0[3] = {cdouble(E0[0], E0[1]), cdouble(E0[2], E0[3]), cdouble(E0[4], E0[5])};
+  hipMemsetAsync(rhs, 0, sizeof(double2) * G->G.TotalBFs, g_stream);
+  for (size_t s = 0; s < G->G.Surfaces.size(); s++) {
+    const Surface &S = G->G.Surfaces[s];
+    double sgn;
+    if (S.RegionIndices[0] == 0) sgn = -1.0;
+    else if (S.RegionIndices[1] == 0) sgn = 1.0;
+    else continue;
+    const DevSurface &D = A->dev_surface((int)s);
+    if (launch_rhs(g_stream, D.V, D.PVI, G->rhs[s].PA, G->rhs[s].EV, D.EI, D.EL, D.nE, S.IsPEC, sgn, e0, nhat, K, Z,
+                   (double2 *)rhs + G->G.BFIndexOffset[s]) != hipSuccess)
+      return fail(SCUFFX_ERR_HIP, "RHS kernel failed");
+  }
+  if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "RHS kernel failed");
+  return 0;
+}
+
+int scuffx_power(scuffx_geom *G, double wr, double wi, const void *KN, const void *rhs, double *PAbs, double *PExt)
+{
+  (void)wr; (void)wi;
+  if (!G || !KN || !rhs) return fail(SCUFFX_ERR_ARG, "null argument");
+  Assembler *A = G->assembler();
+  int rc = ensure_rhs_data(G);
+  if (rc) return rc;
+  double tot[2] = {0, 0};
+  for (size_t s = 0; s < G->G.Surfaces.size(); s++) {
+    const Surface &S = G->G.Surfaces[s];
+    const DevSurface &D = A->dev_surface((int)s);
+    int off = G->G.BFIndexOffset[s];
+    if (launch_power(g_stream, D.V, D.PVI, G->rhs[s].PA, G->rhs[s].PZ, G->rhs[s].PEI, D.EI, D.EL, D.nE, S.IsPEC,
+                     (const double2 *)KN + off, (const double2 *)rhs + off, G->rhs[s].work, G->d_out) != hipSuccess)
+      return fail(SCUFFX_ERR_HIP, "power kernel failed");
+    double h[2];
+    hipMemcpyAsync(h, G->d_out, sizeof(h), hipMemcpyDeviceToHost, g_stream);
+    hipStreamSynchronize(g_stream);
+    tot[0] += h[0]; tot[1] += h[1];
+  }
+  if (PExt) *PExt = tot[0];
+  if (PAbs) *PAbs = tot[1];
+  return 0;
+}
+
+int scuffx_profile(int on)
+{
+  prof_collect();
+  prof_reset();
+  prof_enable(on != 0);
+  return 0;
+}
+
+int scuffx_profile_read(const char *kernel, double *ms, double *work, long *launches)
+{
+  if (!kernel) return fail(SCUFFX_ERR_ARG, "null kernel name");
+  prof_collect();
+  for (int k = 0; k < PK_COUNT; k++)
+    if (!std::strcmp(kernel, prof_name(k))) { prof_read(k, ms, work, launches); return 0; }
+  return fail(SCUFFX_ERR_ARG, std::string("unknown kernel name ") + kernel);
+}
+
+}  // extern "C"

@@ -1,0 +1,479 @@
+// geometry.cpp -- host-side mesh topology (see geometry.h).
+#include "geometry.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <strings.h>
+#include <unordered_map>
+
+namespace scuffx {
+
+static double dist3(const double *a, const double *b)
+{
+  double d = 0.0;
+  for (int i = 0; i < 3; i++) d += (a[i] - b[i]) * (a[i] - b[i]);
+  return std::sqrt(d);
+}
+
+void Material::eps_mu(cdouble omega, cdouble *e, cdouble *m) const
+{
+  switch (kind) {
+    case MAT_VACUUM: *e = 1.0; *m = 1.0; break;
+    case MAT_CONST: *e = eps; *m = mu; break;
+    case MAT_SIGMA: *e = eps.real() + cdouble(0.0, 1.0) * sigma / omega; *m = mu; break;
+    default: *e = 1.0 - wp * wp / (omega * (omega + cdouble(0.0, gamma))); *m = 1.0; break;
+  }
+}
+
+// Panel geometry: centroid, bounding radius, normal and area
+// (InitRWGPanel, RWGSurface.cc:55-97).
+static void init_panel(Panel *P, const double *V)
+{
+  const double *X[3] = {V + 3 * P->VI[0], V + 3 * P->VI[1], V + 3 * P->VI[2]};
+  for (int i = 0; i < 3; i++) P->Centroid[i] = (X[0][i] + X[1][i] + X[2][i]) / 3.0;
+  P->Radius = dist3(P->Centroid, X[0]);
+  P->Radius = std::fmax(P->Radius, dist3(P->Centroid, X[1]));
+  P->Radius = std::fmax(P->Radius, dist3(P->Centroid, X[2]));
+  double A[3], B[3];
+  for (int i = 0; i < 3; i++) { A[i] = X[1][i] - X[0][i]; B[i] = X[2][i] - X[0][i]; }
+  P->ZHat[0] = A[1] * B[2] - A[2] * B[1];
+  P->ZHat[1] = A[2] * B[0] - A[0] * B[2];
+  P->ZHat[2] = A[0] * B[1] - A[1] * B[0];
+  double n = std::sqrt(P->ZHat[0] * P->ZHat[0] + P->ZHat[1] * P->ZHat[1] + P->ZHat[2] * P->ZHat[2]);
+  double s = 1.0 / n;
+  for (int i = 0; i < 3; i++) P->ZHat[i] *= s;
+  P->Area = 0.5 * n;
+}
+
+// RWG edges.  Edges are met panel by panel, local edge ne spanning
+// VI[ne]..VI[(ne+1)%3].  The first panel to touch an edge is its
+// positive panel (QP = opposite vertex); the second is its negative
+// panel, and the edge receives the next interior index at that moment
+// (InitEdgeList.cc:126-245).  Exterior edges are numbered -(i+1) in the
+// order of a walk over vertices, newest edge first.
+std::string build_surface(const double *Vin, int nV, const int *T, int nT,
+                          bool pec, Surface *S)
+{
+  S->V.assign(Vin, Vin + 3 * (size_t)nV);
+  S->Panels.resize(nT);
+  S->IsPEC = pec;
+  for (int p = 0; p < nT; p++) {
+    Panel &P = S->Panels[p];
+    for (int j = 0; j < 3; j++) {
+      P.VI[j] = T[3 * p + j];
+      P.EI[j] = 0;
+      if (P.VI[j] < 0 || P.VI[j] >= nV) return "panel vertex index out of range";
+    }
+    init_panel(&P, S->V.data());
+  }
+  // per-vertex chains of edges keyed by their lesser vertex; a chain
+  // is walked newest-first
+  struct Tmp { Edge e; int next; };
+  std::vector<Tmp> all;
+  all.reserve(3 * (size_t)nT / 2 + 16);
+  std::vector<int> head(nV, -1);
+  int numInterior = 0;
+  for (int p = 0; p < nT; p++)
+    for (int ne = 0; ne < 3; ne++) {
+      const Panel &P = S->Panels[p];
+      int lo = P.VI[ne], hi = P.VI[(ne + 1) % 3];
+      if (lo > hi) std::swap(lo, hi);
+      int k = head[lo];
+      while (k >= 0 && all[k].e.iV2 != hi) k = all[k].next;
+      if (k >= 0) {
+        Edge &E = all[k].e;
+        if (E.iMPanel != -1) return "invalid mesh topology: an edge belongs to more than two panels";
+        E.iQM = P.VI[(ne + 2) % 3];
+        E.iMPanel = p;
+        E.MIndex = (ne + 2) % 3;
+        E.Radius = std::fmax(E.Radius, dist3(E.Centroid, S->V.data() + 3 * E.iQM));
+        E.Index = numInterior++;
+      } else {
+        Tmp t;
+        Edge &E = t.e;
+        E.iV1 = lo; E.iV2 = hi;
+        E.iQP = P.VI[(ne + 2) % 3];
+        E.iQM = -1;
+        const double *VL = S->V.data() + 3 * lo, *VG = S->V.data() + 3 * hi;
+        for (int i = 0; i < 3; i++) E.Centroid[i] = (VL[i] + VG[i]) / 2.0;
+        E.Length = dist3(VL, VG);
+        E.Radius = dist3(E.Centroid, S->V.data() + 3 * E.iQP);
+        E.Radius = std::fmax(E.Radius, dist3(E.Centroid, VL));
+        E.Radius = std::fmax(E.Radius, dist3(E.Centroid, VG));
+        E.iPPanel = p;
+        E.PIndex = (ne + 2) % 3;
+        E.iMPanel = -1;
+        E.MIndex = (ne + 2) % 3;
+        E.Index = -1;
+        t.next = head[lo];
+        head[lo] = (int)all.size();
+        all.push_back(t);
+      }
+    }
+  S->Edges.assign(numInterior, Edge());
+  int numExt = 0;
+  for (int v = 0; v < nV; v++)
+    for (int k = head[v]; k >= 0; k = all[k].next) {
+      Edge &E = all[k].e;
+      if (E.Index == -1) E.Index = -(++numExt);
+      else S->Edges[E.Index] = E;
+    }
+  S->NumExteriorEdges = numExt;
+  for (int e = 0; e < numInterior; e++) {
+    const Edge &E = S->Edges[e];
+    S->Panels[E.iPPanel].EI[E.PIndex] = e;
+    if (E.iMPanel != -1) S->Panels[E.iMPanel].EI[E.MIndex] = e;
+  }
+  S->NumBFs = pec ? numInterior : 2 * numInterior;
+  return "";
+}
+
+int add_surface(Geometry *G, Surface &&S, const Material &interior)
+{
+  S.RegionIndices[0] = 0;
+  if (S.IsPEC) S.RegionIndices[1] = -1;
+  else {
+    S.RegionIndices[1] = (int)G->Regions.size();
+    G->Regions.push_back(interior);
+  }
+  int ns = (int)G->Surfaces.size();
+  G->BFIndexOffset.push_back(G->TotalBFs);
+  G->TotalBFs += S.NumBFs;
+  G->Surfaces.push_back(std::move(S));
+  return ns;
+}
+
+// ---------------------------------------------------------------------
+// GMSH reader.  Node numbers are remapped to storage order; vertices
+// closer than 1e-6 are merged onto the earlier one (O(V^2), as the
+// reference does); only triangles (element type 2) are kept.
+// ---------------------------------------------------------------------
+std::string read_gmsh(const std::string &fn, std::vector<double> *V, std::vector<int> *T)
+{
+  FILE *f = std::fopen(fn.c_str(), "r");
+  if (!f) return "could not open " + fn;
+  char Line[100];
+  int legacy = -1;
+  while (legacy < 0) {
+    if (!std::fgets(Line, 100, f)) { std::fclose(f); return fn + ": no node section"; }
+    if (!std::strncmp(Line, "$NOD", 4)) legacy = 1;
+    else if (!std::strncmp(Line, "$Nodes", 6)) legacy = 0;
+  }
+  int NumNodes = 0;
+  if (!std::fgets(Line, 100, f) || !(NumNodes = std::atoi(Line))) { std::fclose(f); return fn + ": invalid number of nodes"; }
+  V->assign(3 * (size_t)NumNodes, 0.0);
+  std::vector<int> map(2 * (size_t)NumNodes, -1);
+  for (int nn = 0; nn < NumNodes; nn++) {
+    int idx; double x[3];
+    if (!std::fgets(Line, 100, f) || std::sscanf(Line, "%i %le %le %le", &idx, x, x + 1, x + 2) != 4) { std::fclose(f); return fn + ": invalid node"; }
+    if (idx < 0 || idx >= 2 * NumNodes) { std::fclose(f); return fn + ": node index out of range"; }
+    for (int j = 0; j < 3; j++) (*V)[3 * nn + j] = x[j];
+    map[idx] = nn;
+  }
+  for (int i = 0; i < NumNodes; i++)
+    for (int j = i + 1; j < NumNodes; j++)
+      if (dist3(V->data() + 3 * i, V->data() + 3 * j) < 1.0e-6)
+        for (auto &m : map) if (m == j) m = i;
+  if (!std::fgets(Line, 100, f) || std::strncmp(Line, legacy ? "$ENDNOD" : "$EndNodes", legacy ? 7 : 9)) { std::fclose(f); return fn + ": nodes section not terminated"; }
+  if (!std::fgets(Line, 100, f) || std::strncmp(Line, legacy ? "$ELM" : "$Elements", legacy ? 4 : 9)) { std::fclose(f); return fn + ": no elements section"; }
+  int NumElements = 0;
+  if (!std::fgets(Line, 100, f) || std::sscanf(Line, "%i", &NumElements) != 1 || NumElements < 0) { std::fclose(f); return fn + ": invalid number of elements"; }
+  T->clear();
+  for (int ne = 0; ne < NumElements; ne++) {
+    if (!std::fgets(Line, 100, f)) { std::fclose(f); return fn + ": too few elements"; }
+    int VI[3] = {0, 0, 0}, ElType = 0;
+    if (legacy) {
+      int a, b, c, d;
+      if (std::sscanf(Line, "%i %i %i %i %i %i %i %i", &a, &ElType, &b, &c, &d, VI, VI + 1, VI + 2) < 5) { std::fclose(f); return fn + ": invalid element"; }
+    } else {
+      int ElNum, nTags, nRead, pos, dummy;
+      if (std::sscanf(Line, "%i %i %i%n", &ElNum, &ElType, &nTags, &nRead) < 3) { std::fclose(f); return fn + ": invalid element"; }
+      pos = nRead;
+      for (int t = 0; t < nTags; t++) { std::sscanf(Line + pos, "%i%n", &dummy, &nRead); pos += nRead; }
+      int nc = std::sscanf(Line + pos, "%i %i %i", VI, VI + 1, VI + 2);
+      if (ElType == 2 && nc != 3) { std::fclose(f); return fn + ": invalid triangle"; }
+    }
+    if (ElType == 2)
+      for (int j = 0; j < 3; j++) {
+        if (VI[j] < 0 || VI[j] >= 2 * NumNodes || map[VI[j]] < 0) { std::fclose(f); return fn + ": triangle references an unknown node"; }
+        T->push_back(map[VI[j]]);
+      }
+  }
+  std::fclose(f);
+  return "";
+}
+
+std::string parse_material(const std::string &s, Material *m, bool *pec)
+{
+  *pec = false;
+  m->kind = MAT_CONST; m->eps = 1.0; m->mu = 1.0;
+  if (s == "PEC") { *pec = true; return ""; }
+  if (s == "VACUUM") { m->kind = MAT_VACUUM; return ""; }
+  if (!strncasecmp(s.c_str(), "DRUDE_", 6)) {
+    m->kind = MAT_DRUDE;
+    if (std::sscanf(s.c_str() + 6, "%le_%le", &m->wp, &m->gamma) != 2) return "bad DRUDE material " + s;
+    return "";
+  }
+  if (!strncasecmp(s.c_str(), "CONST_", 6)) {
+    // CONST_EPS_x[_MU_y][_SIGMA_z] (libs/libMatProp/libMatProp.cc:201-290)
+    std::stringstream ss(s.substr(6));
+    std::string tok;
+    std::vector<std::string> toks;
+    while (std::getline(ss, tok, '_')) toks.push_back(tok);
+    double sigma = 0.0;
+    for (size_t i = 0; i < toks.size(); i++) {
+      if (i + 1 >= toks.size()) return "no value in material " + s;
+      double r, im; char c1, c2;
+      int n = std::sscanf(toks[i + 1].c_str(), "%le%c%le%c", &r, &c1, &im, &c2);
+      cdouble val;
+      if (n == 1) val = r;
+      else if (n == 4 && c1 == '+' && std::tolower(c2) == 'i') val = cdouble(r, im);
+      else if (n == 4 && c1 == '-' && std::tolower(c2) == 'i') val = cdouble(r, -im);
+      else if (!strcasecmp(toks[i].c_str(), "SIGMA") && n >= 1) val = r;
+      else return "bad value in material " + s;
+      if (!strcasecmp(toks[i].c_str(), "EPS")) m->eps = val;
+      else if (!strcasecmp(toks[i].c_str(), "MU")) m->mu = val;
+      else if (!strcasecmp(toks[i].c_str(), "SIGMA")) sigma = val.real();
+      else return "invalid token in material " + s;
+      i++;
+    }
+    if (sigma != 0.0) {
+      // eps(w) = Re(eps) + i*sigma/w in units of c/1um; the reference
+      // prints Re(eps) and sigma with %e before re-parsing them
+      char buf[64];
+      std::snprintf(buf, sizeof buf, "%e", m->eps.real());
+      m->eps = std::atof(buf);
+      std::snprintf(buf, sizeof buf, "%e", sigma);
+      m->sigma = std::atof(buf);
+      m->kind = MAT_SIGMA;
+    }
+    return "";
+  }
+  return "unsupported material " + s;
+}
+
+// One-time rigid transformation of an OBJECT at birth: X' = R X + D
+// (libs/libscuff/GTransformation.cc:107-142, :186-235, :273-315).  Later
+// lines compose on the left, as the reference's running transformation
+// does (RWGSurface.cc:165-186), and the vertices are transformed before
+// the panels and edges are built (RWGSurface.cc:427-435).
+struct BirthTransform {
+  double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double D[3] = {0, 0, 0};
+  // v <- (u.v) u + cos(t) (v - (u.v) u) + sin(t) (u x v), same operation
+  // order as the reference so that rotated meshes agree to the bit
+  static void rotate_vec(const double u[3], double c, double sn, double v[3])
+  {
+    double ud = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    double par[3] = {u[0] * ud, u[1] * ud, u[2] * ud};
+    double perp[3] = {v[0] - par[0], v[1] - par[1], v[2] - par[2]};
+    double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    for (int i = 0; i < 3; i++) v[i] = par[i] + (perp[i] * c + cr[i] * sn);
+  }
+  void displace(const double d[3]) { for (int i = 0; i < 3; i++) D[i] += d[i]; }
+  void rotate(const double axis[3], double degrees)
+  {
+    double n = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    double u[3] = {axis[0] / n, axis[1] / n, axis[2] / n};
+    double t = degrees * (3.14159265358979323846 / 180.0);
+    double c = std::cos(t), sn = std::sin(t);
+    rotate_vec(u, c, sn, D);
+    for (int j = 0; j < 3; j++) {
+      double col[3] = {R[0][j], R[1][j], R[2][j]};
+      rotate_vec(u, c, sn, col);
+      for (int i = 0; i < 3; i++) R[i][j] = col[i];
+    }
+  }
+  // this <- B o this
+  void compose(const BirthTransform &B)
+  {
+    double d[3];
+    for (int i = 0; i < 3; i++) d[i] = B.R[i][0] * D[0] + B.R[i][1] * D[1] + B.R[i][2] * D[2] + B.D[i];
+    double r[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        r[i][j] = 0;
+        for (int k = 0; k < 3; k++) r[i][j] += B.R[i][k] * R[k][j];
+      }
+    std::memcpy(D, d, sizeof D);
+    std::memcpy(R, r, sizeof R);
+  }
+  void apply(std::vector<double> *V) const
+  {
+    for (size_t n = 0; n + 2 < V->size(); n += 3) {
+      double x0 = (*V)[n], x1 = (*V)[n + 1], x2 = (*V)[n + 2];
+      for (int i = 0; i < 3; i++) (*V)[n + i] = R[i][0] * x0 + R[i][1] * x1 + R[i][2] * x2 + D[i];
+    }
+  }
+};
+
+// parses "DISPLACED x y z" or "ROTATED t ABOUT x y z"
+static std::string parse_transform(const std::vector<std::string> &tok, BirthTransform *out)
+{
+  char *end;
+  auto num = [&](const std::string &s, double *v) {
+    *v = std::strtod(s.c_str(), &end);
+    return end != s.c_str() && *end == 0;
+  };
+  if (!strcasecmp(tok[0].c_str(), "DISPLACED") || !strcasecmp(tok[0].c_str(), "DISP")) {
+    double d[3];
+    if (tok.size() != 4) return "DISPLACED needs 3 values";
+    for (int i = 0; i < 3; i++)
+      if (!num(tok[1 + i], d + i)) return "bad value in DISPLACED";
+    out->displace(d);
+    return "";
+  }
+  double t, a[3];
+  if (tok.size() != 6 || strcasecmp(tok[2].c_str(), "ABOUT")) return "ROTATED needs: ROTATED angle ABOUT x y z";
+  if (!num(tok[1], &t)) return "bad angle in ROTATED";
+  for (int i = 0; i < 3; i++)
+    if (!num(tok[3 + i], a + i)) return "bad axis in ROTATED";
+  out->rotate(a, t);
+  return "";
+}
+
+std::string read_scuffgeo(const std::string &fn, Geometry *G)
+{
+  std::ifstream in(fn);
+  if (!in) return "could not open " + fn;
+  std::string dir;
+  size_t sl = fn.rfind('/');
+  if (sl != std::string::npos) dir = fn.substr(0, sl + 1);
+  std::string line, mesh, mat = "PEC";
+  bool inobj = false, moved = false;
+  BirthTransform T;
+  int lineno = 0;
+  while (std::getline(in, line)) {
+    lineno++;
+    size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::stringstream ss(line);
+    std::vector<std::string> tok;
+    std::string w;
+    while (ss >> w) tok.push_back(w);
+    if (tok.empty()) continue;
+    const std::string &kw = tok[0];
+    std::string where = fn + ":" + std::to_string(lineno) + ": ";
+    if (!strcasecmp(kw.c_str(), "OBJECT")) {
+      if (inobj) return where + "OBJECT inside OBJECT";
+      inobj = true; mesh.clear(); mat = "PEC"; moved = false; T = BirthTransform();
+    } else if (!inobj) {
+      return where + "unsupported keyword " + kw + " (only OBJECT sections are handled)";
+    } else if (!strcasecmp(kw.c_str(), "MESHFILE")) {
+      if (tok.size() != 2) return where + "MESHFILE needs one file name";
+      mesh = tok[1];
+    } else if (!strcasecmp(kw.c_str(), "MATERIAL")) {
+      if (tok.size() != 2) return where + "MATERIAL needs one name";
+      mat = tok[1];
+    } else if (!strcasecmp(kw.c_str(), "DISPLACED") || !strcasecmp(kw.c_str(), "DISP") ||
+               !strcasecmp(kw.c_str(), "ROTATED") || !strcasecmp(kw.c_str(), "ROT")) {
+      BirthTransform B;
+      std::string e = parse_transform(tok, &B);
+      if (!e.empty()) return where + e;
+      if (moved) T.compose(B); else T = B;
+      moved = true;
+    } else if (!strcasecmp(kw.c_str(), "ENDOBJECT")) {
+      if (mesh.empty()) return where + "OBJECT without MESHFILE";
+      std::vector<double> V; std::vector<int> Tri;
+      std::string err = read_gmsh(mesh[0] == '/' ? mesh : dir + mesh, &V, &Tri);
+      if (!err.empty()) return err;
+      if (moved) T.apply(&V);
+      Material m; bool pec;
+      err = parse_material(mat, &m, &pec);
+      if (!err.empty()) return where + err;
+      Surface S;
+      err = build_surface(V.data(), (int)V.size() / 3, Tri.data(), (int)Tri.size() / 3, pec, &S);
+      if (!err.empty()) return where + err;
+      add_surface(G, std::move(S), m);
+      inobj = false;
+    } else {
+      return where + "unsupported keyword " + kw + " in OBJECT";
+    }
+  }
+  if (inobj) return fn + ": OBJECT without ENDOBJECT";
+  if (G->Surfaces.empty()) return fn + ": no objects";
+  return "";
+}
+
+// ---------------------------------------------------------------------
+// icosphere: level 0 is the icosahedron; every level splits each
+// triangle (a,b,c) into (a,ab,ca), (ab,b,bc), (ca,bc,c), (ab,bc,ca) with
+// unit-sphere midpoints u=(ua+ub)/|ua+ub|.  Panel 4t+i is child i of
+// panel t of the previous level, so neighbouring panels have nearby
+// indices.
+// ---------------------------------------------------------------------
+void icosphere(int level, double radius, const double *center,
+               std::vector<double> *Vout, std::vector<int> *Tout)
+{
+  static const int faces[20][3] = {
+      {0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11},
+      {1, 5, 9}, {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+      {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8}, {3, 8, 9},
+      {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double base[12][3] = {
+      {-1, phi, 0}, {1, phi, 0}, {-1, -phi, 0}, {1, -phi, 0},
+      {0, -1, phi}, {0, 1, phi}, {0, -1, -phi}, {0, 1, -phi},
+      {phi, 0, -1}, {phi, 0, 1}, {-phi, 0, -1}, {-phi, 0, 1}};
+  std::vector<double> U;
+  std::vector<int> T;
+  for (int i = 0; i < 12; i++) {
+    double n = std::sqrt(base[i][0] * base[i][0] + base[i][1] * base[i][1] + base[i][2] * base[i][2]);
+    for (int j = 0; j < 3; j++) U.push_back(base[i][j] / n);
+  }
+  for (int t = 0; t < 20; t++) for (int j = 0; j < 3; j++) T.push_back(faces[t][j]);
+  for (int l = 0; l < level; l++) {
+    std::unordered_map<long long, int> mid;
+    mid.reserve(T.size());
+    long long nVmax = 10LL * (1LL << (2 * level)) + 2;
+    std::vector<int> T2(4 * T.size());
+    size_t nt = T.size() / 3;
+    for (size_t t = 0; t < nt; t++) {
+      int v[3] = {T[3 * t], T[3 * t + 1], T[3 * t + 2]}, m[3];
+      for (int e = 0; e < 3; e++) {
+        int a = v[e], b = v[(e + 1) % 3];
+        long long key = (long long)std::min(a, b) * nVmax + std::max(a, b);
+        auto it = mid.find(key);
+        if (it == mid.end()) {
+          double s[3];
+          for (int j = 0; j < 3; j++) s[j] = U[3 * a + j] + U[3 * b + j];
+          double n = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+          int idx = (int)(U.size() / 3);
+          for (int j = 0; j < 3; j++) U.push_back(s[j] / n);
+          mid.emplace(key, idx);
+          m[e] = idx;
+        } else m[e] = it->second;
+      }
+      int *c = T2.data() + 12 * t;
+      c[0] = v[0]; c[1] = m[0]; c[2] = m[2];
+      c[3] = m[0]; c[4] = v[1]; c[5] = m[1];
+      c[6] = m[2]; c[7] = m[1]; c[8] = v[2];
+      c[9] = m[0]; c[10] = m[1]; c[11] = m[2];
+    }
+    T.swap(T2);
+  }
+  Vout->resize(U.size());
+  for (size_t i = 0; i < U.size() / 3; i++)
+    for (int j = 0; j < 3; j++) (*Vout)[3 * i + j] = (center ? center[j] : 0.0) + radius * U[3 * i + j];
+  *Tout = T;
+}
+
+std::string write_gmsh(const std::string &fn, const std::vector<double> &V, const std::vector<int> &T)
+{
+  FILE *f = std::fopen(fn.c_str(), "w");
+  if (!f) return "could not write " + fn;
+  size_t nV = V.size() / 3, nT = T.size() / 3;
+  std::fprintf(f, "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%zu\n", nV);
+  for (size_t i = 0; i < nV; i++) std::fprintf(f, "%zu %.17g %.17g %.17g\n", i + 1, V[3 * i], V[3 * i + 1], V[3 * i + 2]);
+  std::fprintf(f, "$EndNodes\n$Elements\n%zu\n", nT);
+  for (size_t t = 0; t < nT; t++) std::fprintf(f, "%zu 2 2 0 0 %d %d %d\n", t + 1, T[3 * t] + 1, T[3 * t + 1] + 1, T[3 * t + 2] + 1);
+  std::fprintf(f, "$EndElements\n");
+  std::fclose(f);
+  return "";
+}
+
+}  // namespace scuffx

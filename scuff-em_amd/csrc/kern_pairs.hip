@@ -1,0 +1,434 @@
+// kern_pairs.hip -- panel-pair integrals for far and near pairs, the
+// edge-pair combination and stamping into the BEM matrix, and the
+// symmetric mirror of the upper triangle.
+//
+// Far pairs (relative centroid distance > 4) use the 6x6-point fixed
+// rule of GetPPIs_Cubature (PanelPanelInteractions.cc:211-350, reached at
+// :420-424).  Instead of re-running the 36-point sum for each of the 9
+// (Qa,Qb) source-vertex choices, one pass accumulates moments in panel-
+// local coordinates xi = x - c_a, xi' = x' - c_b (R = x - x'):
+//   H0(Qa,Qb) = S_xx' - Qb.S_x - Qa.S_x' + (Qa.Qb + 4/(ik)^2) S_1
+//   H1(Qa,Qb) = s - Qb.u - Qa.v + (Qa x Qb).w
+// with S_1 = sum Phi, S_x = sum xi Phi, S_x' = sum xi' Phi,
+// S_xx' = sum (xi.xi') Phi, s = sum ((xi x xi').R) Psi,
+// u = sum (R x xi) Psi, v = sum (xi' x R) Psi, w = sum R Psi, and
+// Qa, Qb measured from the same panel centroids.  This is an algebraic
+// rearrangement of (x-Qa).(x'-Qb) and ((x-Qa)x(x'-Qb)).R; the 9 results
+// equal the reference's per-Q sums up to rounding.
+#include <hip/hip_runtime.h>
+
+#include "bem.h"
+
+namespace scuffx {
+
+__constant__ double c_tcr4[18] = SCUFFX_TCR4_INIT;
+
+#define SCUFFX_INV4PI 0.079577471545947667884   // 1/(4 pi)
+
+// binary search of sorted nearCol[lo..hi) for pb; index or -1
+__device__ inline int near_find(const int *rowPtr, const int *col, int pa, int pb)
+{
+  int lo = rowPtr[pa], hi = rowPtr[pa + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    int c = col[mid];
+    if (c == pb) return mid;
+    if (c < pb) lo = mid + 1; else hi = mid;
+  }
+  return -1;
+}
+
+__device__ inline long long pair_index(int pa, int pb, int nPa, int nPb, int same)
+{
+  if (same) return (long long)pa * nPa - (long long)pa * (pa + 1) / 2 + (pb - pa - 1);
+  return (long long)pa * nPb + pb;
+}
+
+// moments of one panel pair for one wavenumber (far: Phi = e^{ikr}/4pi r)
+struct Moments {
+  cplx S1, Sx[3], Sy[3], Sxy;
+  cplx s, u[3], v[3], w[3];
+};
+
+template <int DESING>
+__device__ inline void pair_moments(const double *xi, const double *eta, const double *D,
+                                    cplx k, int needH1, Moments &m)
+{
+#pragma clang fp contract(fast)
+  m.S1 = mk(0, 0); m.Sxy = mk(0, 0); m.s = mk(0, 0);
+  for (int c = 0; c < 3; c++) { m.Sx[c] = mk(0, 0); m.Sy[c] = mk(0, 0); m.u[c] = mk(0, 0); m.v[c] = mk(0, 0); m.w[c] = mk(0, 0); }
+  cplx ik = IMUL(k);
+  for (int i = 0; i < 6; i++) {
+    const double wi = c_tcr4[3 * i + 2];
+    const double *x = xi + 3 * i;
+    for (int j = 0; j < 6; j++) {
+      const double wj = c_tcr4[3 * j + 2];
+      const double *y = eta + 3 * j;
+      double R[3] = {D[0] + x[0] - y[0], D[1] + x[1] - y[1], D[2] + x[2] - y[2]};
+      double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      double r = sqrt(r2);
+      double ir = 1.0 / r;
+      cplx Phi;
+      if (DESING) {
+        // ExpRel(ikr,4) = sum_{m>=4} (ikr)^m/m!, stopped at |term|<1e-8|sum|
+        // (PanelPanelInteractions.cc:84-103)
+        cplx z = ik * r, term = mk(1, 0), sum = mk(0, 0);
+        int q;
+        for (q = 1; q < 4; q++) term = term * (z / (double)q);
+        for (; q < 100; q++) {
+          term = term * (z / (double)q);
+          sum += term;
+          if (norm2(term) < 1.0e-16 * norm2(sum)) break;
+        }
+        Phi = sum * (SCUFFX_INV4PI * ir);
+      } else {
+        double e = exp(-k.i * r), sn, cs;
+        sincos(k.r * r, &sn, &cs);
+        Phi = mk(e * cs, e * sn) * (SCUFFX_INV4PI * ir);
+      }
+      Phi = Phi * (wi * wj);
+      m.S1 += Phi;
+      for (int c = 0; c < 3; c++) { m.Sx[c] += x[c] * Phi; m.Sy[c] += y[c] * Phi; }
+      m.Sxy += (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) * Phi;
+      if (needH1) {
+        cplx Psi = Phi * (ik - mk(ir, 0)) * ir;
+        double xy[3], Rx[3], yR[3];
+        cross3(x, y, xy);
+        cross3(R, x, Rx);
+        cross3(y, R, yR);
+        m.s += (xy[0] * R[0] + xy[1] * R[1] + xy[2] * R[2]) * Psi;
+        for (int c = 0; c < 3; c++) { m.u[c] += Rx[c] * Psi; m.v[c] += yR[c] * Psi; m.w[c] += R[c] * Psi; }
+      }
+    }
+  }
+}
+
+__device__ inline void combos_from_moments(const Moments &m, const double *Qa, const double *Qb,
+                                           cplx k, int needH1, cplx *H0, cplx *H1)
+{
+  cplx ik = IMUL(k);
+  cplx c4 = 4.0 / (ik * ik);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      const double *qa = Qa + 3 * a, *qb = Qb + 3 * b;
+      cplx h = m.Sxy;
+      for (int c = 0; c < 3; c++) h = h - qb[c] * m.Sx[c] - qa[c] * m.Sy[c];
+      h += (mk(qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2], 0) + c4) * m.S1;
+      H0[3 * a + b] = h;
+      if (needH1) {
+        double qq[3];
+        cross3(qa, qb, qq);
+        cplx t = m.s;
+        for (int c = 0; c < 3; c++) t = t - qb[c] * m.u[c] - qa[c] * m.v[c] + qq[c] * m.w[c];
+        H1[3 * a + b] = t;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------
+// K1: far panel pairs.  grid.y = row panel pa, threads over pb.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_far(const double *Va, const int *PVIa, const double *PCa, int nPa,
+      const double *Vb, const int *PVIb, const double *PCb, int nPb, int same,
+      const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
+      int needH1, double *HP)
+{
+  int pa = blockIdx.y;
+  int pb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pa >= nPa || pb >= nPb) return;
+  if (same && pb <= pa) return;
+  if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) return;
+  double ca[3] = {PCa[4 * pa], PCa[4 * pa + 1], PCa[4 * pa + 2]};
+  double cb[3] = {PCb[4 * pb], PCb[4 * pb + 1], PCb[4 * pb + 2]};
+  const double *A0 = Va + 3 * PVIa[3 * pa], *A1 = Va + 3 * PVIa[3 * pa + 1], *A2 = Va + 3 * PVIa[3 * pa + 2];
+  const double *B0 = Vb + 3 * PVIb[3 * pb], *B1 = Vb + 3 * PVIb[3 * pb + 1], *B2 = Vb + 3 * PVIb[3 * pb + 2];
+  double xi[18], eta[18], Qa[9], Qb[9], D[3];
+  for (int c = 0; c < 3; c++) {
+    double a0 = A0[c] - ca[c], aA = A1[c] - A0[c], aB = A2[c] - A0[c];
+    double b0 = B0[c] - cb[c], bA = B1[c] - B0[c], bB = B2[c] - B0[c];
+    for (int i = 0; i < 6; i++) {
+      xi[3 * i + c] = a0 + c_tcr4[3 * i] * aA + c_tcr4[3 * i + 1] * aB;
+      eta[3 * i + c] = b0 + c_tcr4[3 * i] * bA + c_tcr4[3 * i + 1] * bB;
+    }
+    Qa[c] = A0[c] - ca[c]; Qa[3 + c] = A1[c] - ca[c]; Qa[6 + c] = A2[c] - ca[c];
+    Qb[c] = B0[c] - cb[c]; Qb[3 + c] = B1[c] - cb[c]; Qb[6 + c] = B2[c] - cb[c];
+    D[c] = ca[c] - cb[c];
+  }
+  long long pidx = pair_index(pa, pb, nPa, nPb, same);
+  int stride = needH1 ? 4 : 2;
+  double *out = HP + pidx * (long long)(nreg * 9 * stride);
+  for (int r = 0; r < nreg; r++) {
+    Moments m;
+    cplx k = regs[r].k;
+    pair_moments<0>(xi, eta, D, k, needH1, m);
+    cplx H0[9], H1[9];
+    combos_from_moments(m, Qa, Qb, k, needH1, H0, H1);
+    for (int q = 0; q < 9; q++) {
+      double *o = out + (r * 9 + q) * stride;
+      o[0] = H0[q].r; o[1] = H0[q].i;
+      if (needH1) { o[2] = H1[q].r; o[3] = H1[q].i; }
+    }
+  }
+}
+
+hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int nPa,
+                      const double *Vb, const int *PVIb, const double *PCb, int nPb, int same,
+                      const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
+                      int needH1, double *HP)
+{
+  if (nPa <= 0 || nPb <= 0 || nreg <= 0) return hipSuccess;
+  dim3 grid((nPb + 255) / 256, nPa);
+  hipLaunchKernelGGL(k_far, grid, dim3(256), 0, stream, Va, PVIa, PCa, nPa, Vb, PVIb, PCb, nPb, same,
+                     nearRowPtr, nearCol, nreg, regs, needH1, HP);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// K3: near pairs -- desingularized fixed-rule cubature plus the analytic
+// singular terms rebuilt from the cached frequency-independent integrals
+// (PanelPanelInteractions.cc:504-559, QDFIPPI.cc:67-260).
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(128)
+k_near(const NearPair *nps, int nnear, const double *records, int nreg, const RegionK *regs,
+       int needH1, double *HN)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnear) return;
+  const NearPair &P = nps[t];
+  const double *Q = records + (long long)P.rec * SCUFFX_QIF_LEN;
+  // reordered vertices: x = Va0 + u(Va1-Va0) + v(Va2-Va0); local origin Va0
+  double xi[18], eta[18], Qal[9], Qbl[9], D[3];
+  for (int c = 0; c < 3; c++) {
+    double aA = P.Va[3 + c] - P.Va[c], aB = P.Va[6 + c] - P.Va[c];
+    double bA = P.Vb[3 + c] - P.Vb[c], bB = P.Vb[6 + c] - P.Vb[c];
+    for (int i = 0; i < 6; i++) {
+      xi[3 * i + c] = c_tcr4[3 * i] * aA + c_tcr4[3 * i + 1] * aB;
+      eta[3 * i + c] = c_tcr4[3 * i] * bA + c_tcr4[3 * i + 1] * bB;
+    }
+    for (int q = 0; q < 3; q++) { Qal[3 * q + c] = P.Qa[3 * q + c] - P.Va[c]; Qbl[3 * q + c] = P.Qb[3 * q + c] - P.Vb[c]; }
+    D[c] = P.Va[c] - P.Vb[c];
+  }
+  // Q-dependent FIPPI recombination, shared pieces
+  const double *OVa[3] = {P.OVa0, P.OVa1, P.OVa2}, *OVb[3] = {P.OVb0, P.OVb1, P.OVb2};
+  double A[3], B[3], AP[3], BP[3];
+  sub3(OVa[1], OVa[0], A); sub3(OVa[2], OVa[1], B);
+  sub3(OVb[1], OVb[0], AP); sub3(OVb[2], OVb[1], BP);
+  const double *xMxp = Q, *xXxp = Q + 3, *RM1 = Q + 6, *R1 = Q + 15, *R2 = Q + 24, *x0 = Q + 33;
+  const double r0m[9] = {1.0 / 4.0, 1.0 / 6.0, 1.0 / 12.0, 1.0 / 6.0, 1.0 / 9.0, 1.0 / 18.0, 1.0 / 12.0, 1.0 / 18.0, 1.0 / 36.0};
+  // translation correction (QDFIPPI.cc:143-158): x0 vs centroid of the
+  // (possibly swapped) pair as ordered for this call
+  double Lc[3];
+  const double *Vc = P.flipped ? P.Vb : P.Va;
+  for (int c = 0; c < 3; c++) Lc[c] = x0[c] - (Vc[c] + Vc[3 + c] + Vc[6 + c]) / 3.0;
+  bool applyL = sqrt(dot3(Lc, Lc)) > 1.0e-8 * sqrt(dot3(A, A));
+  double hDot[9][4], hNab[4], hTim[9][4], hTimRM3[9];
+  // hDot/hTimes for moments RM1, R0, R1, R2 (QDFIPPI.cc:163-251)
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      const double *Qa = P.Qa + 3 * a, *Qb = P.Qb + 3 * b;
+      const double *OQa = P.flipped ? Qb : Qa, *OQb = P.flipped ? Qa : Qb;
+      double V0mQ[3], V0PmQP[3];
+      sub3(OVa[0], OQa, V0mQ); sub3(OVb[0], OQb, V0PmQP);
+      double VdVP = dot3(V0mQ, V0PmQP), VdAP = dot3(V0mQ, AP), VdBP = dot3(V0mQ, BP);
+      double AdVP = dot3(A, V0PmQP), AdAP = dot3(A, AP), AdBP = dot3(A, BP);
+      double BdVP = dot3(B, V0PmQP), BdAP = dot3(B, AP), BdBP = dot3(B, BP);
+      double QamQb[3], QaxQb[3], V0mV0P[3], S[3];
+      sub3(OVa[0], OVb[0], V0mV0P);
+      sub3(Qa, Qb, QamQb);
+      cross3(Qa, Qb, QaxQb);
+      cross3(OVa[0], OVb[0], S); double ht0000 = dot3(QaxQb, V0mV0P) + dot3(QamQb, S);
+      cross3(A, OVb[0], S);      double ht1000 = dot3(QaxQb, A) + dot3(QamQb, S);
+      cross3(B, OVb[0], S);      double ht0100 = dot3(QaxQb, B) + dot3(QamQb, S);
+      cross3(OVa[0], AP, S);     double ht0010 = -dot3(QaxQb, AP) + dot3(QamQb, S);
+      cross3(OVa[0], BP, S);     double ht0001 = -dot3(QaxQb, BP) + dot3(QamQb, S);
+      cross3(A, AP, S);          double ht1010 = dot3(QamQb, S);
+      cross3(A, BP, S);          double ht1001 = dot3(QamQb, S);
+      cross3(B, AP, S);          double ht0110 = dot3(QamQb, S);
+      cross3(B, BP, S);          double ht0101 = dot3(QamQb, S);
+      double rm3 = dot3(QaxQb, xMxp) + dot3(QamQb, xXxp);
+      if (applyL) { cross3(Lc, QamQb, S); rm3 -= dot3(S, xMxp); }
+      const double *Ms[4] = {RM1, r0m, R1, R2};
+      for (int q = 0; q < 4; q++) {
+        const double *M = Ms[q];
+        hDot[3 * a + b][q] = M[0] * VdVP + M[1] * VdAP + M[2] * VdBP + M[3] * AdVP + M[4] * AdAP
+                             + M[5] * AdBP + M[6] * BdVP + M[7] * BdAP + M[8] * BdBP;
+        hTim[3 * a + b][q] = M[0] * ht0000 + M[3] * ht1000 + M[6] * ht0100 + M[1] * ht0010
+                             + M[2] * ht0001 + M[4] * ht1010 + M[5] * ht1001 + M[7] * ht0110
+                             + M[8] * ht0101;
+      }
+      hTimRM3[3 * a + b] = rm3;
+      if (P.flipped) {
+        hTimRM3[3 * a + b] *= -1.0;
+        for (int q = 0; q < 3; q++) hTim[3 * a + b][q] *= -1.0;
+      }
+    }
+  hNab[0] = 4.0 * RM1[0]; hNab[1] = 1.0; hNab[2] = 4.0 * R1[0]; hNab[3] = 4.0 * R2[0];
+  int stride = needH1 ? 4 : 2;
+  double *out = HN + (long long)t * (nreg * 9 * stride);
+  for (int r = 0; r < nreg; r++) {
+    cplx k = regs[r].k;
+    Moments m;
+    pair_moments<1>(xi, eta, D, k, needH1, m);
+    cplx H0[9], H1[9];
+    combos_from_moments(m, Qal, Qbl, k, needH1, H0, H1);
+    cplx ik = IMUL(k), OOIK2 = 1.0 / (ik * ik), PF[5];
+    PF[0] = mk(SCUFFX_INV4PI, 0);
+    for (int q = 1; q < 5; q++) PF[q] = ik * PF[q - 1];
+    for (int q = 0; q < 9; q++) {
+      cplx h0 = H0[q];
+      h0 += PF[0] * (mk(hDot[q][0], 0) + OOIK2 * hNab[0])
+            + PF[1] * (mk(hDot[q][1], 0) + OOIK2 * hNab[1])
+            + PF[2] * 0.5 * (mk(hDot[q][2], 0) + OOIK2 * hNab[2])
+            + PF[3] * (1.0 / 6.0) * (mk(hDot[q][3], 0) + OOIK2 * hNab[3]);
+      double *o = out + (r * 9 + q) * stride;
+      o[0] = h0.r; o[1] = h0.i;
+      if (needH1) {
+        cplx h1 = H1[q];
+        h1 += PF[0] * (-1.0 * hTimRM3[q]) + PF[2] * (0.5 * hTim[q][0])
+              + PF[3] * ((1.0 / 3.0) * hTim[q][1]) + PF[4] * ((1.0 / 6.0) * hTim[q][2]);
+        o[2] = h1.r; o[3] = h1.i;
+      }
+    }
+  }
+}
+
+hipError_t launch_near(hipStream_t stream, const NearPair *np, int nnear, const double *records,
+                       int nreg, const RegionK *regs, int needH1, double *HN)
+{
+  if (nnear <= 0 || nreg <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_near, dim3((nnear + 127) / 128), dim3(128), 0, stream, np, nnear, records,
+                     nreg, regs, needH1, HN);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// K2: edge pairs.  GC0 = LaLb(HPP-HPM-HMP+HMM)[0],
+// GC1 = LaLb/(ik)(...)[1] (EdgeEdgeInteractions.cc:154-188); EFIE entry
+// i w mu s GC0, PMCHWT 2x2 block [[i w mu G, -ik C], [-ik C, -i w eps G]]
+// summed over the common regions (SurfaceSurfaceInteractions.cc:162-327).
+// Threads run down a column of the block (coalesced column-major
+// writes); same-surface blocks fill only the upper triangle.
+// ---------------------------------------------------------------------
+__device__ inline void fetch_h(int pa, int qa, int pb, int qb, int r, int nreg, int stride,
+                               int nPa, int nPb, int same,
+                               const double *HP, const double *HN, const double *HS,
+                               const int *nearRowPtr, const int *nearCol, const int *nearIdx,
+                               cplx *h0, cplx *h1)
+{
+  const double *rec;
+  if (same && pa == pb) {
+    rec = HS + ((long long)pa * nreg + r) * 9 * 2 + (3 * qa + qb) * 2;
+    *h0 = mk(rec[0], rec[1]);
+    *h1 = mk(0, 0);
+    return;
+  }
+  int ni = near_find(nearRowPtr, nearCol, pa, pb);
+  if (ni >= 0) {
+    rec = HN + ((long long)nearIdx[ni] * nreg + r) * 9 * stride + (3 * qa + qb) * stride;
+  } else if (same && pb < pa) {
+    rec = HP + pair_index(pb, pa, nPa, nPb, same) * (nreg * 9 * stride) + (r * 9 + 3 * qb + qa) * stride;
+  } else {
+    rec = HP + pair_index(pa, pb, nPa, nPb, same) * (nreg * 9 * stride) + (r * 9 + 3 * qa + qb) * stride;
+  }
+  *h0 = mk(rec[0], rec[1]);
+  *h1 = stride == 4 ? mk(rec[2], rec[3]) : mk(0, 0);
+}
+
+__global__ void __launch_bounds__(256)
+k_stamp(int nEa, const int *EIa, const double *ELa, int nPa, int nEb, const int *EIb, const double *ELb,
+        int nPb, int same, int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
+        const double *HP, const double *HN, const double *HS,
+        const int *nearRowPtr, const int *nearCol, const int *nearIdx,
+        double2 *M, long ldm, int rowOff, int colOff)
+{
+  int ea = blockIdx.x * blockDim.x + threadIdx.x;
+  int eb = blockIdx.y;
+  if (ea >= nEa || eb >= nEb) return;
+  if (same && ea > eb) return;
+  int stride = needH1 ? 4 : 2;
+  int pA[2] = {EIa[4 * ea], EIa[4 * ea + 1]}, qA[2] = {EIa[4 * ea + 2], EIa[4 * ea + 3]};
+  int pB[2] = {EIb[4 * eb], EIb[4 * eb + 1]}, qB[2] = {EIb[4 * eb + 2], EIb[4 * eb + 3]};
+  double LL = ELa[ea] * ELb[eb];
+  cplx m00 = mk(0, 0), m01 = mk(0, 0), m11 = mk(0, 0);
+  for (int r = 0; r < nreg; r++) {
+    cplx G[4], C[4];
+    for (int s = 0; s < 2; s++)
+      for (int t = 0; t < 2; t++) {
+        G[2 * s + t] = mk(0, 0); C[2 * s + t] = mk(0, 0);
+        if (pA[s] < 0 || pB[t] < 0) continue;
+        fetch_h(pA[s], qA[s], pB[t], qB[t], r, nreg, stride, nPa, nPb, same, HP, HN, HS,
+                nearRowPtr, nearCol, nearIdx, &G[2 * s + t], &C[2 * s + t]);
+      }
+    cplx k = regs[r].k;
+    cplx GC0 = LL * (G[0] - G[1] - G[2] + G[3]);
+    cplx GC1 = (mk(LL, 0) / IMUL(k)) * (C[0] - C[1] - C[2] + C[3]);
+    if (k.r == 0.0 && k.i == 0.0) { GC0 = mk(0, 0); GC1 = mk(0, 0); }
+    m00 += regs[r].pf1 * GC0;
+    m01 += regs[r].pf2 * GC1;
+    m11 += regs[r].pf3 * GC0;
+  }
+  long R = rowOff, Cc = colOff;
+  if (pecA && pecB) {
+    M[(R + ea) + (Cc + eb) * ldm] = make_double2(m00.r, m00.i);
+  } else if (pecA) {
+    M[(R + ea) + (Cc + 2 * eb) * ldm] = make_double2(m00.r, m00.i);
+    M[(R + ea) + (Cc + 2 * eb + 1) * ldm] = make_double2(m01.r, m01.i);
+  } else if (pecB) {
+    M[(R + 2 * ea) + (Cc + eb) * ldm] = make_double2(m00.r, m00.i);
+    M[(R + 2 * ea + 1) + (Cc + eb) * ldm] = make_double2(m01.r, m01.i);
+  } else {
+    long x = R + 2 * ea, y = Cc + 2 * eb;
+    M[x + y * ldm] = make_double2(m00.r, m00.i);
+    if (!same || ea != eb) M[(x + 1) + y * ldm] = make_double2(m01.r, m01.i);
+    M[x + (y + 1) * ldm] = make_double2(m01.r, m01.i);
+    M[(x + 1) + (y + 1) * ldm] = make_double2(m11.r, m11.i);
+  }
+}
+
+hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const double *ELa, int nPa,
+                        int nEb, const int *EIb, const double *ELb, int nPb, int same,
+                        int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
+                        const double *HP, const double *HN, const double *HS,
+                        const int *nearRowPtr, const int *nearCol, const int *nearIdx,
+                        double2 *M, long ldm, int rowOff, int colOff)
+{
+  if (nEa <= 0 || nEb <= 0) return hipSuccess;
+  dim3 grid((nEa + 255) / 256, nEb);
+  hipLaunchKernelGGL(k_stamp, grid, dim3(256), 0, stream, nEa, EIa, ELa, nPa, nEb, EIb, ELb, nPb,
+                     same, pecA, pecB, nreg, regs, needH1, HP, HN, HS, nearRowPtr, nearCol, nearIdx,
+                     M, ldm, rowOff, colOff);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// lower triangle <- transpose of the upper triangle
+// (AssembleBEMMatrix.cc:753-758), 32x32 tiles staged through LDS
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mirror(double2 *M, long ldm, int n)
+{
+  __shared__ double2 tile[32][33];
+  int bi = blockIdx.y, bj = blockIdx.x;   // destination tile (rows bi, cols bj), bi >= bj
+  if (bj > bi) return;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  // read source tile: rows bj*32.., cols bi*32.. (upper part)
+  for (int yy = ty; yy < 32; yy += 8) {
+    int r = bj * 32 + tx, c = bi * 32 + yy;
+    if (r < n && c < n) tile[yy][tx] = M[r + (long)c * ldm];
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    int r = bi * 32 + tx, c = bj * 32 + yy;
+    if (r < n && c < n && r > c) M[r + (long)c * ldm] = tile[tx][yy];
+  }
+}
+
+hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n)
+{
+  if (n <= 1) return hipSuccess;
+  int nt = (n + 31) / 32;
+  hipLaunchKernelGGL(k_mirror, dim3(nt, nt), dim3(256), 0, stream, M, ldm, n);
+  return hipGetLastError();
+}
+
+}  // namespace scuffx

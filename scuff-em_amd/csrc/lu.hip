@@ -1,0 +1,531 @@
+// lu.hip -- blocked complex LU with partial pivoting, triangular solves
+// and the refinement helpers, complex128 and complex64.
+//
+// Right-looking blocked algorithm (outer block nb, inner block 32):
+//  * a 32-column sub-panel is factored column by column by one
+//    cooperative launch: every workgroup owns a slice of rows, finds its
+//    largest |re|+|im| entry, a grid barrier picks the global pivot (first
+//    maximal row, as izamax), the two rows are exchanged through a small
+//    global buffer, and each workgroup scales and rank-1 updates its rows;
+//  * row interchanges are applied to the other columns (laswp),
+//  * U12 = L11^{-1} A12 (unit-lower triangular solve),
+//  * A22 -= A21 U12 on the matrix cores: v_mfma_f64_16x16x4f64 for
+//    complex128 and v_mfma_f32_16x16x4f32 for complex64, four real
+//    products per complex product.
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+#include "lu.h"
+#include "prof.h"
+
+namespace cg = cooperative_groups;
+
+namespace scuffx {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct CT;
+template <> struct CT<double2> {
+  typedef double R;
+  typedef d4 V4;
+  __device__ static inline V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static inline V4 zero() { return V4{0, 0, 0, 0}; }
+};
+template <> struct CT<float2> {
+  typedef float R;
+  typedef f4 V4;
+  __device__ static inline V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static inline V4 zero() { return V4{0, 0, 0, 0}; }
+};
+
+template <typename T> __device__ inline T cmul(T a, T b) { T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r; }
+template <typename T> __device__ inline T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename T> __device__ inline T crecip(T a)
+{
+  // Smith's algorithm for 1/a
+  T r;
+  typedef decltype(a.x) R;
+  if (fabs(a.x) >= fabs(a.y)) { R t = a.y / a.x, d = a.x + a.y * t; r.x = 1 / d; r.y = -t / d; }
+  else { R t = a.x / a.y, d = a.x * t + a.y; r.x = t / d; r.y = -1 / d; }
+  return r;
+}
+template <typename T> __device__ inline T cdiv(T a, T b)
+{
+  typedef decltype(a.x) R;
+  T r;
+  if (fabs(b.x) >= fabs(b.y)) { R t = b.y / b.x, d = b.x + b.y * t; r.x = (a.x + a.y * t) / d; r.y = (a.y - a.x * t) / d; }
+  else { R t = b.x / b.y, d = b.x * t + b.y; r.x = (a.x * t + a.y) / d; r.y = (a.y * t - a.x) / d; }
+  return r;
+}
+
+// ---------------------------------------------------------------------
+// C[m x n] -= A[m x k] B[k x n]; 64x64 block tile, 4 waves of 32x32,
+// K staged 16 at a time through LDS as separate real/imag planes.
+// ---------------------------------------------------------------------
+#define GBM 64
+#define GBN 64
+#define GBK 16
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc)
+{
+  typedef typename CT<T>::R R;
+  typedef typename CT<T>::V4 V4;
+  __shared__ R sAr[GBK][GBM + 4], sAi[GBK][GBM + 4], sBr[GBK][GBN + 4], sBi[GBK][GBN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const long row0 = (long)blockIdx.x * GBM, col0 = (long)blockIdx.y * GBN;
+  V4 accr[2][2], acci[2][2];
+  for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); }
+  for (long k0 = 0; k0 < k; k0 += GBK) {
+    for (int s = 0; s < 4; s++) {
+      int i = tid & 63, kk = (tid >> 6) + 4 * s;
+      long gi = row0 + i, gk = k0 + kk;
+      T v;
+      if (gi < m && gk < k) v = A[gi + gk * lda]; else { v.x = 0; v.y = 0; }
+      sAr[kk][i] = v.x; sAi[kk][i] = v.y;
+      int kb = tid & 15, j = (tid >> 4) + 16 * s;
+      long gkb = k0 + kb, gj = col0 + j;
+      T u;
+      if (gkb < k && gj < n) u = B[gkb + gj * ldb]; else { u.x = 0; u.y = 0; }
+      sBr[kb][j] = u.x; sBi[kb][j] = u.y;
+    }
+    __syncthreads();
+    for (int kk0 = 0; kk0 < GBK; kk0 += 4) {
+      int kk = kk0 + (lane >> 4);
+      R ar[2], ai[2], br[2], bi[2];
+      for (int t = 0; t < 2; t++) {
+        ar[t] = sAr[kk][wr * 32 + t * 16 + (lane & 15)];
+        ai[t] = sAi[kk][wr * 32 + t * 16 + (lane & 15)];
+        br[t] = sBr[kk][wc * 32 + t * 16 + (lane & 15)];
+        bi[t] = sBi[kk][wc * 32 + t * 16 + (lane & 15)];
+      }
+      for (int ti = 0; ti < 2; ti++)
+        for (int tj = 0; tj < 2; tj++) {
+          accr[ti][tj] = CT<T>::mfma(-ar[ti], br[tj], accr[ti][tj]);
+          accr[ti][tj] = CT<T>::mfma(ai[ti], bi[tj], accr[ti][tj]);
+          acci[ti][tj] = CT<T>::mfma(-ar[ti], bi[tj], acci[ti][tj]);
+          acci[ti][tj] = CT<T>::mfma(-ai[ti], br[tj], acci[ti][tj]);
+        }
+    }
+    __syncthreads();
+  }
+  for (int ti = 0; ti < 2; ti++)
+    for (int tj = 0; tj < 2; tj++)
+      for (int r = 0; r < 4; r++) {
+        long gi = row0 + wr * 32 + ti * 16 + 4 * (lane >> 4) + r;
+        long gj = col0 + wc * 32 + tj * 16 + (lane & 15);
+        if (gi < m && gj < n) {
+          T c = C[gi + gj * ldc];
+          c.x += accr[ti][tj][r];
+          c.y += acci[ti][tj][r];
+          C[gi + gj * ldc] = c;
+        }
+      }
+}
+
+template <typename T>
+static void gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
+                     hipStream_t s)
+{
+  if (m <= 0 || n <= 0 || k <= 0) return;
+  dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
+  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
+  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
+}
+
+// ---------------------------------------------------------------------
+// cooperative panel factorization: rows [r0, r0+m), columns [r0, r0+w)
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, float *candv, int *candi,
+        T *prow, T *jrow, long chunk)
+{
+  typedef decltype(T().x) R;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ R sv[256];
+  __shared__ long si[256];
+  __shared__ T spiv[32];
+  __shared__ long sp;
+  const int tid = threadIdx.x, G = gridDim.x, g = blockIdx.x;
+  const long lo = r0 + (long)g * chunk, hi = std::min(r0 + m, lo + chunk);
+  for (int j = 0; j < w; j++) {
+    const long c = r0 + j;
+    // local maximum of |re|+|im| over own rows >= r0+j; smallest row on ties
+    R best = -1; long bi = -1;
+    for (long i = std::max(lo, r0 + j) + tid; i < hi; i += 256) {
+      T v = A[i + c * lda];
+      R a = fabs(v.x) + fabs(v.y);
+      if (a > best) { best = a; bi = i; }
+    }
+    sv[tid] = best; si[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        R ov = sv[tid + s]; long oi = si[tid + s];
+        if (ov > sv[tid] || (ov == sv[tid] && oi >= 0 && (si[tid] < 0 || oi < si[tid]))) { sv[tid] = ov; si[tid] = oi; }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      ((R *)(candv))[g] = sv[0];
+      candi[g] = (int)si[0];
+    }
+    grid.sync();
+    if (tid == 0) {
+      R b = -1; long p = -1;
+      for (int q = 0; q < G; q++) {
+        R v = ((R *)candv)[q]; long ii = candi[q];
+        if (ii < 0) continue;
+        if (v > b || (v == b && ii < p)) { b = v; p = ii; }
+      }
+      if (p < 0) p = c;
+      sp = p;
+      if (g == 0) {
+        ipiv[c] = (int)(p + 1);
+        if (b == 0 && *info == 0) *info = (int)(c + 1);
+      }
+    }
+    __syncthreads();
+    const long p = sp;
+    // owners publish the two rows (panel columns only)
+    if (p >= lo && p < hi) for (int q = tid; q < w; q += 256) prow[q] = A[p + (r0 + q) * lda];
+    if (c >= lo && c < hi) for (int q = tid; q < w; q += 256) jrow[q] = A[c + (r0 + q) * lda];
+    grid.sync();
+    if (p != c) {
+      if (c >= lo && c < hi) for (int q = tid; q < w; q += 256) A[c + (r0 + q) * lda] = prow[q];
+      if (p >= lo && p < hi) for (int q = tid; q < w; q += 256) A[p + (r0 + q) * lda] = jrow[q];
+    }
+    for (int q = tid; q < w; q += 256) spiv[q] = prow[q];
+    __syncthreads();
+    T piv = spiv[j];
+    bool nz = (piv.x != 0 || piv.y != 0);
+    T rp = nz ? crecip(piv) : piv;
+    for (long i = std::max(lo, c + 1) + tid; i < hi; i += 256) {
+      T l = A[i + c * lda];
+      if (nz) { l = cmul(l, rp); A[i + c * lda] = l; }
+      for (int q = j + 1; q < w; q++) {
+        T a = A[i + (r0 + q) * lda];
+        A[i + (r0 + q) * lda] = csub(a, cmul(l, spiv[q]));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// row interchanges rows [k1,k2) (absolute, ipiv 1-based) on columns [c0,c1)
+template <typename T>
+__global__ void k_laswp(T *A, long lda, long c0, long c1, long k1, long k2, const int *ipiv)
+{
+  long c = c0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  for (long s = k1; s < k2; s++) {
+    long p = ipiv[s] - 1;
+    if (p != s) { T t = A[s + c * lda]; A[s + c * lda] = A[p + c * lda]; A[p + c * lda] = t; }
+  }
+}
+
+template <typename T>
+static void laswp(T *A, long lda, long c0, long c1, long k1, long k2, const int *ipiv, hipStream_t s)
+{
+  if (c1 <= c0 || k2 <= k1) return;
+  hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, s, A, lda, c0, c1, k1, k2, ipiv);
+}
+
+// B[w x ncol] <- L^{-1} B, L unit lower w x w (A block at (r0,r0)); one
+// thread per column, L read through the cache
+template <typename T>
+__global__ void k_trsm_llu(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol)
+{
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  T *b = B + c * ldb;
+  for (int i = 0; i < w; i++) {
+    T x = b[i];
+    if (x.x == 0 && x.y == 0) continue;
+    const T *Lc = A + (r0 + i) * lda + r0;
+    for (int r = i + 1; r < w; r++) b[r] = csub(b[r], cmul(Lc[r], x));
+  }
+}
+
+template <typename T>
+static void trsm_llu(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol, hipStream_t s)
+{
+  if (ncol <= 0 || w <= 0) return;
+  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
+  hipLaunchKernelGGL(k_trsm_llu<T>, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, s, A, lda, r0, w, B, ldb, ncol);
+  prof_end(s, tok, 4.0 * (double)w * (double)w * (double)ncol);
+}
+
+template <typename T>
+int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb)
+{
+  if (n < 0 || lda < std::max(1L, n)) return -1;
+  if (n == 0) return 0;
+  const int ib = 32;
+  int dev = 0, ncu = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  int perCU = 0;
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_panel<T>, 256, 0);
+  long maxG = (long)std::max(1, ncu) * std::max(1, std::min(perCU, 2));
+  float *candv = nullptr; int *candi = nullptr, *dinfo = nullptr; T *prow = nullptr, *jrow = nullptr;
+  if (hipMalloc(&candv, sizeof(double) * 1024) != hipSuccess || hipMalloc(&candi, sizeof(int) * 1024) != hipSuccess ||
+      hipMalloc(&prow, sizeof(T) * 64) != hipSuccess || hipMalloc(&jrow, sizeof(T) * 64) != hipSuccess ||
+      hipMalloc(&dinfo, sizeof(int)) != hipSuccess)
+    return -2;
+  hipMemsetAsync(dinfo, 0, sizeof(int), stream);
+  int rc = 0;
+  for (long k0 = 0; k0 < n && rc == 0; k0 += nb) {
+    long jb = std::min<long>(nb, n - k0);
+    for (long j0 = k0; j0 < k0 + jb; j0 += ib) {
+      int w = (int)std::min<long>(ib, k0 + jb - j0);
+      long m = n - j0;
+      long G = std::min<long>(maxG, std::min<long>(1024, (m + 255) / 256));
+      if (G < 1) G = 1;
+      long chunk = (m + G - 1) / G;
+      G = (m + chunk - 1) / chunk;
+      long r0 = j0;
+      void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
+                      (void *)&dinfo, (void *)&candv, (void *)&candi, (void *)&prow, (void *)&jrow, (void *)&chunk};
+      int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
+      hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3((unsigned)G), dim3(256), args, 0, stream);
+      prof_end(stream, tok, 0.0);
+      if (e != hipSuccess) { rc = -3; break; }
+      laswp(A, lda, k0, j0, j0, j0 + w, d_ipiv, stream);
+      laswp(A, lda, j0 + w, k0 + jb, j0, j0 + w, d_ipiv, stream);
+      long nr = k0 + jb - (j0 + w);
+      if (nr > 0) {
+        trsm_llu(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
+        gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
+                 A + (j0 + w) + (j0 + w) * lda, lda, stream);
+      }
+    }
+    laswp(A, lda, 0, k0, k0, k0 + jb, d_ipiv, stream);
+    laswp(A, lda, k0 + jb, n, k0, k0 + jb, d_ipiv, stream);
+    long nr = n - k0 - jb;
+    if (nr > 0) {
+      trsm_llu(A, lda, k0, (int)jb, A + k0 + (k0 + jb) * lda, lda, nr, stream);
+      gemm_sub(nr, nr, jb, A + (k0 + jb) + k0 * lda, lda, A + k0 + (k0 + jb) * lda, lda,
+               A + (k0 + jb) + (k0 + jb) * lda, lda, stream);
+    }
+  }
+  hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream);
+  hipStreamSynchronize(stream);
+  if (prof_enabled()) prof_collect();
+  hipFree(candv); hipFree(candi); hipFree(prow); hipFree(jrow); hipFree(dinfo);
+  if (rc == 0 && hipGetLastError() != hipSuccess) rc = -3;
+  return rc;
+}
+
+// ---------------------------------------------------------------------
+// triangular solves.  Blocked for trans='N': a diagonal block is solved
+// by one workgroup per right-hand side, the rest updated by the GEMM.
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_trsv_block(const T *A, long lda, long r0, int bs, T *B, long ldb, int lower, int unit)
+{
+  __shared__ T x[256];
+  T *b = B + (long)blockIdx.x * ldb + r0;
+  const int tid = threadIdx.x;
+  if (tid < bs) x[tid] = b[tid];
+  __syncthreads();
+  if (lower) {
+    for (int i = 0; i < bs; i++) {
+      if (!unit && tid == 0) x[i] = cdiv(x[i], A[(r0 + i) + (r0 + i) * lda]);
+      __syncthreads();
+      T xi = x[i];
+      if (tid > i && tid < bs) x[tid] = csub(x[tid], cmul(A[(r0 + tid) + (r0 + i) * lda], xi));
+      __syncthreads();
+    }
+  } else {
+    for (int i = bs - 1; i >= 0; i--) {
+      if (!unit && tid == 0) x[i] = cdiv(x[i], A[(r0 + i) + (r0 + i) * lda]);
+      __syncthreads();
+      T xi = x[i];
+      if (tid < i) x[tid] = csub(x[tid], cmul(A[(r0 + tid) + (r0 + i) * lda], xi));
+      __syncthreads();
+    }
+  }
+  if (tid < bs) b[tid] = x[tid];
+}
+
+// transposed / conjugate-transposed solves, one workgroup per column
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_trsv_trans(const T *A, long lda, long n, T *B, long ldb, int upper_first, int conj_)
+{
+  T *b = B + (long)blockIdx.x * ldb;
+  __shared__ T red[256];
+  const int tid = threadIdx.x;
+  // solve U^T y = b (forward), then L^T x = y (backward)
+  for (int pass = 0; pass < 2; pass++) {
+    bool up = (pass == 0);
+    for (long s = 0; s < n; s++) {
+      long i = up ? s : n - 1 - s;
+      T acc; acc.x = 0; acc.y = 0;
+      if (up) { for (long r = tid; r < i; r += 256) { T a = A[r + i * lda]; if (conj_) a.y = -a.y; acc.x += a.x * b[r].x - a.y * b[r].y; acc.y += a.x * b[r].y + a.y * b[r].x; } }
+      else { for (long r = i + 1 + tid; r < n; r += 256) { T a = A[r + i * lda]; if (conj_) a.y = -a.y; acc.x += a.x * b[r].x - a.y * b[r].y; acc.y += a.x * b[r].y + a.y * b[r].x; } }
+      red[tid] = acc;
+      __syncthreads();
+      for (int q = 128; q > 0; q >>= 1) { if (tid < q) { red[tid].x += red[tid + q].x; red[tid].y += red[tid + q].y; } __syncthreads(); }
+      if (tid == 0) {
+        T v = csub(b[i], red[0]);
+        if (up) { T d = A[i + i * lda]; if (conj_) d.y = -d.y; v = cdiv(v, d); }
+        b[i] = v;
+      }
+      __syncthreads();
+    }
+  }
+  (void)upper_first;
+}
+
+template <typename T>
+__global__ void k_laswp_rev(T *B, long ldb, long n, long nrhs, const int *ipiv)
+{
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  for (long s = n - 1; s >= 0; s--) {
+    long p = ipiv[s] - 1;
+    if (p != s) { T t = B[s + c * ldb]; B[s + c * ldb] = B[p + c * ldb]; B[p + c * ldb] = t; }
+  }
+}
+
+template <typename T>
+int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B, long ldb,
+             hipStream_t stream)
+{
+  if (n < 0 || nrhs < 0 || lda < std::max(1L, n) || ldb < std::max(1L, n)) return -1;
+  if (n == 0 || nrhs == 0) return 0;
+  if (trans == 'N' || trans == 'n') {
+    laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);
+    const int bs = 256;
+    for (long kb = 0; kb < n; kb += bs) {
+      int w = (int)std::min<long>(bs, n - kb);
+      hipLaunchKernelGGL(k_trsv_block<T>, dim3((unsigned)nrhs), dim3(256), 0, stream, LU, lda, kb, w, B, ldb, 1, 1);
+      if (kb + w < n) gemm_sub(n - kb - w, nrhs, (long)w, LU + (kb + w) + kb * lda, lda, B + kb, ldb, B + kb + w, ldb, stream);
+    }
+    long last = ((n - 1) / bs) * bs;
+    for (long kb = last; kb >= 0; kb -= bs) {
+      int w = (int)std::min<long>(bs, n - kb);
+      hipLaunchKernelGGL(k_trsv_block<T>, dim3((unsigned)nrhs), dim3(256), 0, stream, LU, lda, kb, w, B, ldb, 0, 0);
+      if (kb > 0) gemm_sub(kb, nrhs, (long)w, LU + kb * lda, lda, B + kb, ldb, B, ldb, stream);
+    }
+  } else if (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c') {
+    int cj = (trans == 'C' || trans == 'c');
+    hipLaunchKernelGGL(k_trsv_trans<T>, dim3((unsigned)nrhs), dim3(256), 0, stream, LU, lda, n, B, ldb, 1, cj);
+    hipLaunchKernelGGL(k_laswp_rev<T>, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, stream, B, ldb, n, nrhs, d_ipiv);
+  } else return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template int lu_factor<double2>(double2 *, long, long, int *, int *, hipStream_t, int);
+template int lu_factor<float2>(float2 *, long, long, int *, int *, hipStream_t, int);
+template int lu_solve<double2>(const double2 *, long, long, const int *, char, long, double2 *, long, hipStream_t);
+template int lu_solve<float2>(const float2 *, long, long, const int *, char, long, float2 *, long, hipStream_t);
+
+// ---------------------------------------------------------------------
+// refinement helpers
+// ---------------------------------------------------------------------
+#define RES_COLCHUNK 512
+__global__ void __launch_bounds__(256)
+k_resid_partial(const double2 *A, long n, long lda, const double2 *x, double2 *work)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  long c0 = (long)blockIdx.y * RES_COLCHUNK, c1 = std::min(n, c0 + RES_COLCHUNK);
+  if (i >= n) return;
+  double sr = 0, si = 0;
+  for (long c = c0; c < c1; c++) {
+    double2 a = A[i + c * lda], v = x[c];
+    sr += a.x * v.x - a.y * v.y;
+    si += a.x * v.y + a.y * v.x;
+  }
+  work[blockIdx.y * n + i] = make_double2(sr, si);
+}
+
+__global__ void k_resid_final(long n, int nch, const double2 *b, const double2 *work, double2 *r)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sr = 0, si = 0;
+  for (int q = 0; q < nch; q++) { sr += work[q * n + i].x; si += work[q * n + i].y; }
+  r[i] = make_double2(b[i].x - sr, b[i].y - si);
+}
+
+int residual_c128(const double2 *A, long n, long lda, const double2 *x, const double2 *b, double2 *r,
+                  double2 *work, hipStream_t stream)
+{
+  int nch = (int)((n + RES_COLCHUNK - 1) / RES_COLCHUNK);
+  hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)((n + 255) / 256), nch), dim3(256), 0, stream, A, n, lda, x, work);
+  hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, nch, b, work, r);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+__global__ void k_c128_to_c64(const double2 *x, float2 *y, long rows, long cols, long ldx, long ldy)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (i >= rows) return;
+  for (; c < cols; c += gridDim.y) {
+    double2 v = x[i + c * ldx];
+    y[i + c * ldy] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+int convert_c128_to_c64(const double2 *x, float2 *y, long rows, long cols, long ldx, long ldy, hipStream_t stream)
+{
+  unsigned gy = (unsigned)std::min<long>(cols, 65535);
+  hipLaunchKernelGGL(k_c128_to_c64, dim3((unsigned)((rows + 255) / 256), gy), dim3(256), 0, stream, x, y, rows, cols, ldx, ldy);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+__global__ void k_c64_to_c128(const float2 *x, double2 *y, long n)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = make_double2(x[i].x, x[i].y);
+}
+int convert_c64_to_c128(const float2 *x, double2 *y, long n, hipStream_t stream)
+{
+  hipLaunchKernelGGL(k_c64_to_c128, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x, y, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+__global__ void k_axpy64(const float2 *d, double2 *x, long n)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { x[i].x += (double)d[i].x; x[i].y += (double)d[i].y; }
+}
+int axpy_c64_into_c128(const float2 *d, double2 *x, long n, hipStream_t stream)
+{
+  hipLaunchKernelGGL(k_axpy64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, d, x, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+__global__ void k_absmax(const double2 *x, long n, double *out)
+{
+  __shared__ double s[256];
+  double m = 0;
+  for (long i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i].x) + fabs(x[i].y));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) { if (threadIdx.x < q) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + q]); __syncthreads(); }
+  if (threadIdx.x == 0) *out = s[0];
+}
+double absmax_c128(const double2 *x, long n, double *work, hipStream_t stream)
+{
+  hipLaunchKernelGGL(k_absmax, dim3(1), dim3(256), 0, stream, x, n, work);
+  double h = 0;
+  hipMemcpyAsync(&h, work, sizeof(double), hipMemcpyDeviceToHost, stream);
+  hipStreamSynchronize(stream);
+  return h;
+}
+
+}  // namespace scuffx

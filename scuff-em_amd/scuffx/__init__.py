@@ -1,0 +1,347 @@
+"""scuffx -- Python front end of the MI355X BEM system path.
+
+Mirrors the names of the reference's SWIG module for this path
+(libs/python/scuff.i:9-53, usage in doc/docs/API/MainFlow.md):
+
+    G  = scuffx.RWGGeometry("Sphere.scuffgeo")        # RWGGeometry.cc:353
+    M  = G.AssembleBEMMatrix(Omega)                    # AssembleBEMMatrix.cc:684
+    KN = G.AssembleRHSVector(Omega, scuffx.PlaneWave([1,0,0],[0,0,1]))
+    M.LUFactorize()                                    # LBWrappers.cc:151
+    M.LUSolve(KN)                                      # LBWrappers.cc:175
+
+Matrices and vectors live in GPU memory (torch tensors used as plain
+device buffers); all arithmetic runs in the HIP library libscuffx.so
+through its C ABI (include/scuffx.h).  There is no CPU fallback: if the
+library or a GPU is missing the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libscuffx.so")
+_LIB = None
+
+ZVAC = 376.73031346177
+
+
+class SCUFFXError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIBPATH):
+            raise ImportError("libscuffx.so not built (run __graft_entry__.build() or make -C scuff-em_amd)")
+        L = C.CDLL(_LIBPATH)
+        vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        sig = {
+            "scuffx_last_error": ([], C.c_char_p),
+            "scuffx_version": ([], C.c_int),
+            "scuffx_set_stream": ([vp], C.c_int),
+            "scuffx_geom_open": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+            "scuffx_geom_new": ([C.POINTER(vp)], C.c_int),
+            "scuffx_geom_add_mesh": ([vp, vp, C.c_int, vp, C.c_int, C.c_char_p], C.c_int),
+            "scuffx_geom_add_msh": ([vp, C.c_char_p, C.c_char_p], C.c_int),
+            "scuffx_geom_free": ([vp], C.c_int),
+            "scuffx_geom_total_bfs": ([vp], C.c_int),
+            "scuffx_geom_num_surfaces": ([vp], C.c_int),
+            "scuffx_geom_counts": ([vp, C.c_int, ip, ip, ip], C.c_int),
+            "scuffx_geom_edges": ([vp, C.c_int, vp, vp], C.c_int),
+            "scuffx_geom_panels": ([vp, C.c_int, vp], C.c_int),
+            "scuffx_icosphere": ([C.c_int, C.c_double, vp, vp, vp], C.c_int),
+            "scuffx_fippi_build": ([vp], C.c_int),
+            "scuffx_fippi_count": ([vp], C.c_int),
+            "scuffx_fippi_records": ([vp, vp], C.c_int),
+            "scuffx_assemble": ([vp, C.c_double, C.c_double, vp, C.c_long], C.c_int),
+            "scuffx_assemble_stats": ([vp, vp, vp], C.c_int),
+            "scuffx_lu_factor": ([vp, C.c_long, C.c_long, vp, C.c_int], C.c_int),
+            "scuffx_lu_solve": ([vp, C.c_long, C.c_long, vp, C.c_char, C.c_long, vp, C.c_long, C.c_int], C.c_int),
+            "scuffx_lu32_from_c128": ([vp, C.c_long, C.c_long, vp, C.c_long, vp], C.c_int),
+            "scuffx_refine": ([vp, C.c_long, C.c_long, vp, C.c_long, vp, vp, vp, C.c_double, C.c_int,
+                               ip, dp], C.c_int),
+            "scuffx_rhs_planewave": ([vp, C.c_double, C.c_double, vp, vp, vp], C.c_int),
+            "scuffx_power": ([vp, C.c_double, C.c_double, vp, vp, dp, dp], C.c_int),
+            "scuffx_profile": ([C.c_int], C.c_int),
+            "scuffx_profile_read": ([C.c_char_p, dp, dp, lp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc < 0:
+        raise SCUFFXError(lib().scuffx_last_error().decode() or ("error %d" % rc))
+    return rc
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise SCUFFXError("scuffx needs a GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _sync_stream():
+    torch = _torch()
+    lib().scuffx_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def icosphere(level, radius=1.0, center=None):
+    """Vertices (nV x 3) and triangles (nT x 3) of a recursively subdivided
+    icosahedron projected on a sphere (benchmark meshes)."""
+    nV, nT = 10 * 4 ** level + 2, 20 * 4 ** level
+    V = np.zeros((nV, 3))
+    T = np.zeros((nT, 3), dtype=np.int32)
+    c = None if center is None else np.ascontiguousarray(center, dtype=np.float64)
+    _check(lib().scuffx_icosphere(level, radius, None if c is None else c.ctypes.data_as(C.c_void_p),
+                                  V.ctypes.data_as(C.c_void_p), T.ctypes.data_as(C.c_void_p)))
+    return V, T
+
+
+class HVector:
+    """Dense complex vector in GPU memory (libs/libhmat/libhmat.h:71-150)."""
+
+    def __init__(self, N, dtype="complex128", data=None):
+        torch = _torch()
+        tdt = torch.complex128 if dtype in ("complex128", torch.complex128) else torch.complex64
+        self.N = N
+        self.t = torch.zeros(N, dtype=tdt, device="cuda") if data is None else data
+
+    def numpy(self):
+        return self.t.cpu().numpy()
+
+    def GetEntry(self, n):
+        return complex(self.t[n].item())
+
+    def SetEntry(self, n, v):
+        self.t[n] = v
+
+
+class HMatrix:
+    """Dense column-major complex matrix in GPU memory with LU factors and
+    pivots (libs/libhmat/libhmat.h:211-394).  Storage is a torch tensor
+    of shape (NC, NR): t[c, r] is entry (r, c)."""
+
+    def __init__(self, NR, NC, dtype="complex128", data=None):
+        torch = _torch()
+        self.NR, self.NC = NR, NC
+        self.prec = 64 if dtype in ("complex128", torch.complex128) else 32
+        tdt = torch.complex128 if self.prec == 64 else torch.complex64
+        self.t = torch.zeros((NC, NR), dtype=tdt, device="cuda") if data is None else data
+        self.ipiv = None
+        self.factored = False
+
+    @classmethod
+    def from_numpy(cls, A, dtype="complex128"):
+        torch = _torch()
+        A = np.asarray(A)
+        M = cls(A.shape[0], A.shape[1], dtype=dtype)
+        M.t.copy_(torch.from_numpy(np.ascontiguousarray(A.T)).to(M.t.dtype))
+        return M
+
+    def numpy(self):
+        return self.t.cpu().numpy().T.copy()
+
+    def GetEntry(self, r, c):
+        return complex(self.t[c, r].item())
+
+    def SetEntry(self, r, c, v):
+        self.t[c, r] = v
+        self.factored = False
+
+    def LUFactorize(self):
+        """In-place LU with partial pivoting; returns LAPACK info."""
+        torch = _torch()
+        if self.NR != self.NC:
+            raise SCUFFXError("LUFactorize needs a square matrix")
+        if self.ipiv is None:
+            self.ipiv = torch.zeros(self.NR, dtype=torch.int32, device="cuda")
+        _sync_stream()
+        info = _check(lib().scuffx_lu_factor(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), self.prec))
+        self.factored = True
+        return info
+
+    def LUSolve(self, X, Trans="N", nrhs=None):
+        """Overwrites X (HVector or HMatrix) with op(A)^{-1} X."""
+        if not self.factored:
+            raise SCUFFXError("LUSolve called on an unfactored matrix")
+        if isinstance(X, HVector):
+            t, n_rhs, ldb = X.t, 1, X.N
+        else:
+            t, n_rhs, ldb = X.t, (X.NC if nrhs is None else nrhs), X.NR
+        if t.dtype != self.t.dtype:
+            raise SCUFFXError("LUSolve: precision of the right-hand side differs from the factors")
+        _sync_stream()
+        _check(lib().scuffx_lu_solve(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), Trans.encode()[:1],
+                                     n_rhs, _ptr(t), ldb, self.prec))
+        return 0
+
+
+class PlaneWave:
+    """Plane wave E = E0 exp(i k nHat.x) in the exterior medium
+    (libs/libIncField/PlaneWave.cc:71-87)."""
+
+    def __init__(self, E0, nHat):
+        self.E0 = np.asarray(E0, dtype=np.complex128)
+        self.nHat = np.asarray(nHat, dtype=np.float64)
+
+
+class RWGGeometry:
+    """Surfaces, RWG edges and regions; BEM assembly on the GPU."""
+
+    def __init__(self, GeoFile=None):
+        self.h = C.c_void_p()
+        if GeoFile is None:
+            _check(lib().scuffx_geom_new(C.byref(self.h)))
+        else:
+            _check(lib().scuffx_geom_open(GeoFile.encode(), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _LIB is not None:
+            _LIB.scuffx_geom_free(self.h)
+            self.h = C.c_void_p()
+
+    def AddMesh(self, V, T, Material="PEC"):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.int32)
+        return _check(lib().scuffx_geom_add_mesh(self.h, V.ctypes.data_as(C.c_void_p), V.size // 3,
+                                                 T.ctypes.data_as(C.c_void_p), T.size // 3,
+                                                 Material.encode()))
+
+    def AddMeshFile(self, path, Material="PEC"):
+        return _check(lib().scuffx_geom_add_msh(self.h, path.encode(), Material.encode()))
+
+    @property
+    def TotalBFs(self):
+        return _check(lib().scuffx_geom_total_bfs(self.h))
+
+    @property
+    def NumSurfaces(self):
+        return _check(lib().scuffx_geom_num_surfaces(self.h))
+
+    def Counts(self, ns=0):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().scuffx_geom_counts(self.h, ns, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def Edges(self, ns=0):
+        _, _, ne = self.Counts(ns)
+        out = np.zeros((ne, 9), dtype=np.int32)
+        lens = np.zeros(ne)
+        _check(lib().scuffx_geom_edges(self.h, ns, out.ctypes.data_as(C.c_void_p), lens.ctypes.data_as(C.c_void_p)))
+        return out, lens
+
+    def Panels(self, ns=0):
+        _, npan, _ = self.Counts(ns)
+        out = np.zeros((npan, 8))
+        _check(lib().scuffx_geom_panels(self.h, ns, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def AllocateBEMMatrix(self):
+        N = self.TotalBFs
+        return HMatrix(N, N)
+
+    def BuildFIPPITable(self):
+        _sync_stream()
+        return _check(lib().scuffx_fippi_build(self.h))
+
+    def FIPPIRecords(self):
+        n = lib().scuffx_fippi_count(self.h)
+        out = np.zeros((n, 36))
+        if n:
+            _check(lib().scuffx_fippi_records(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def AssembleBEMMatrix(self, Omega, M=None):
+        N = self.TotalBFs
+        if M is None:
+            M = HMatrix(N, N)
+        elif M.NR != N or M.NC != N or M.prec != 64:
+            # wrong-size matrix: a new one is returned, as the reference does
+            # (AssembleBEMMatrix.cc:700-705)
+            M = HMatrix(N, N)
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_assemble(self.h, Omega.real, Omega.imag, _ptr(M.t), N))
+        M.factored = False
+        return M
+
+    def AssemblyStats(self):
+        ms = np.zeros(6)
+        cnt = np.zeros(4, dtype=np.int64)
+        _check(lib().scuffx_assemble_stats(self.h, ms.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p)))
+        keys = ["fippi_ms", "self_ms", "far_ms", "near_ms", "stamp_ms", "mirror_ms"]
+        d = dict(zip(keys, ms.tolist()))
+        d.update(dict(zip(["nfar", "nnear", "nself", "nfippi"], cnt.tolist())))
+        return d
+
+    def AssembleRHSVector(self, Omega, IF, RHS=None):
+        N = self.TotalBFs
+        if RHS is None:
+            RHS = HVector(N)
+        E = np.zeros(6)
+        E[0::2] = IF.E0.real
+        E[1::2] = IF.E0.imag
+        nh = np.ascontiguousarray(IF.nHat)
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_rhs_planewave(self.h, Omega.real, Omega.imag, E.ctypes.data_as(C.c_void_p),
+                                          nh.ctypes.data_as(C.c_void_p), _ptr(RHS.t)))
+        return RHS
+
+    def GetPower(self, Omega, KN, RHS):
+        """{'PAbs', 'PExt', 'PScat'} in the reference's power units."""
+        pa, pe = C.c_double(), C.c_double()
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_power(self.h, Omega.real, Omega.imag, _ptr(KN.t), _ptr(RHS.t), C.byref(pa), C.byref(pe)))
+        return {"PAbs": pa.value, "PExt": pe.value, "PScat": pe.value - pa.value}
+
+
+PROFILED_KERNELS = ("gemm_c128", "gemm_c64", "panel_c128", "panel_c64", "trsm_c128", "trsm_c64",
+                    "far", "near", "self", "stamp", "fippi", "resid")
+
+
+def profile(on=True):
+    """Time every launch of the hot kernels with HIP events on their
+    stream (resets the totals)."""
+    _check(lib().scuffx_profile(1 if on else 0))
+
+
+def profile_read():
+    """{kernel: (ms, work, launches)} accumulated since profile(True)."""
+    out = {}
+    for k in PROFILED_KERNELS:
+        ms, w, n = C.c_double(), C.c_double(), C.c_long()
+        _check(lib().scuffx_profile_read(k.encode(), C.byref(ms), C.byref(w), C.byref(n)))
+        out[k] = (ms.value, w.value, n.value)
+    return out
+
+
+def solve_mixed(M, b, tol=1e-13, maxiter=30):
+    """Solve M x = b with a complex64 LU of M and complex128 iterative
+    refinement; returns (x, iterations, final relative residual)."""
+    torch = _torch()
+    N = M.NR
+    LU32 = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ipiv = torch.zeros(N, dtype=torch.int32, device="cuda")
+    _sync_stream()
+    info = _check(lib().scuffx_lu32_from_c128(_ptr(M.t), N, N, _ptr(LU32), N, _ptr(ipiv)))
+    if info > 0:
+        raise SCUFFXError("singular complex64 factor (info=%d)" % info)
+    x = HVector(N)
+    it, rel = C.c_int(), C.c_double()
+    _check(lib().scuffx_refine(_ptr(M.t), N, N, _ptr(LU32), N, _ptr(ipiv), _ptr(b.t), _ptr(x.t),
+                               tol, maxiter, C.byref(it), C.byref(rel)))
+    return x, it.value, rel.value

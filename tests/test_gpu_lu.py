@@ -1,0 +1,126 @@
+"""GPU parity of the dense LU and triangular solves (HMatrix::LUFactorize
+/ LUSolve, libs/libhmat/LBWrappers.cc:151-245) against the oracle's
+LAPACK-contract zgetrf/zgetrs, called through the C ABI."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def rand(n, seed, m=None):
+    rng = np.random.default_rng(seed)
+    m = n if m is None else m
+    return rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+
+
+def gpu_lu(sx, A, dtype="complex128"):
+    M = sx.HMatrix.from_numpy(A, dtype=dtype)
+    info = M.LUFactorize()
+    return M, info
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 31, 32, 33, 64, 100, 255, 257, 300, 513, 700])
+def test_lu64_matches_oracle(sx, O, n):
+    print(f"lu64 n={n}", flush=True)
+    A = rand(n, SEED + n)
+    M, info = gpu_lu(sx, A)
+    LU, ipiv, info_o = O.zgetrf(A)
+    assert info == info_o == 0
+    piv = M.ipiv.cpu().numpy()
+    assert np.array_equal(piv, ipiv), np.nonzero(piv != ipiv)[0][:5]
+    F = M.numpy()
+    err = np.max(np.abs(F - LU)) / np.max(np.abs(LU))
+    assert err <= 1e-12 * max(1.0, n / 100), err
+
+
+@pytest.mark.parametrize("n", [40, 300, 700])
+def test_lu32_backward_error(sx, O, n):
+    print(f"lu32 n={n}", flush=True)
+    A = rand(n, SEED + 7 * n).astype(np.complex64)
+    M, info = gpu_lu(sx, A, dtype="complex64")
+    assert info == 0
+    F = M.numpy().astype(np.complex128)
+    piv = M.ipiv.cpu().numpy()
+    L = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    PA = A.astype(np.complex128).copy()
+    for i in range(n):
+        p = piv[i] - 1
+        if p != i:
+            PA[[i, p], :] = PA[[p, i], :]
+    berr = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+    assert berr <= 2e-6 * np.sqrt(n / 40), berr
+    # pivots agree with the oracle on the same (rounded) input in all but
+    # near-tie positions
+    _, ipiv, _ = O.zgetrf(A.astype(np.complex128))
+    assert np.mean(piv == ipiv) > 0.95
+
+
+def test_lu_singular_info(sx, O):
+    A = rand(70, 5)
+    A[:, 40] = 0
+    M, info = gpu_lu(sx, A)
+    _, _, info_o = O.zgetrf(A)
+    assert info == info_o == 41
+
+
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+@pytest.mark.parametrize("n,nrhs", [(77, 1), (300, 3), (600, 2)])
+def test_lu_solve_matches_oracle(sx, O, trans, n, nrhs):
+    print(f"solve {trans} n={n} nrhs={nrhs}", flush=True)
+    A = rand(n, SEED + 3 * n)
+    B = rand(n, SEED + 5 * n, nrhs)
+    M, _ = gpu_lu(sx, A)
+    X = sx.HMatrix.from_numpy(B)
+    M.LUSolve(X, Trans=trans)
+    Xg = X.numpy()
+    # oracle solve on the GPU's own factors: checks the solve alone
+    Xo = O.zgetrs(M.numpy(), M.ipiv.cpu().numpy(), B, trans)
+    assert np.linalg.norm(Xg - Xo) <= 1e-11 * np.linalg.norm(Xo)
+    op = {"N": A, "T": A.T, "C": A.conj().T}[trans]
+    assert np.linalg.norm(op @ Xg - B) <= 1e-10 * np.linalg.norm(B) * n
+
+
+def test_lu_solve_hvector(sx):
+    n = 129
+    A = rand(n, 9)
+    b = rand(n, 10, 1)[:, 0]
+    M, _ = gpu_lu(sx, A)
+    import torch
+    v = sx.HVector(n, data=torch.from_numpy(b.copy()).cuda())
+    M.LUSolve(v)
+    assert np.linalg.norm(A @ v.numpy() - b) <= 1e-11 * np.linalg.norm(b)
+
+
+def test_lu_solve_errors(sx):
+    n = 8
+    M = sx.HMatrix.from_numpy(rand(n, 1))
+    with pytest.raises(sx.SCUFFXError):
+        M.LUSolve(sx.HMatrix.from_numpy(rand(n, 2, 1)))     # not factored yet
+    M.LUFactorize()
+    with pytest.raises(sx.SCUFFXError):
+        M.LUSolve(sx.HMatrix.from_numpy(rand(n, 2, 1)), Trans="X")
+    with pytest.raises(sx.SCUFFXError):
+        M.LUSolve(sx.HMatrix.from_numpy(rand(n, 2, 1), dtype="complex64"))
+
+
+@pytest.mark.parametrize("n", [200, 1000])
+def test_mixed_refinement(sx, n):
+    print(f"mixed n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rand(n, SEED + n) + n * np.eye(n) * 0.2
+    x_true = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    b = A @ x_true
+    import torch
+    M = sx.HMatrix.from_numpy(A)
+    bv = sx.HVector(n, data=torch.from_numpy(b.copy()).cuda())
+    x, iters, rel = sx.solve_mixed(M, bv, tol=1e-14, maxiter=20)
+    xs = x.numpy()
+    assert iters >= 1
+    assert np.linalg.norm(xs - x_true) <= 1e-12 * np.linalg.norm(x_true)
+    # the complex128 matrix is left untouched
+    assert np.array_equal(M.numpy(), A)

@@ -1,0 +1,160 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point of include/scuffx.h, and the product's mesh topology and
+edge indexing agree bit for bit with the oracle (InitEdgeList.cc:47-371,
+RWGSurface.cc:55-97) on icospheres and on the reference's own test meshes.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "scuffx.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(scuffx_\w+)\s*\(", text, re.M)))
+
+
+def test_abi_exports_every_declared_symbol(sx):
+    import subprocess
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    L = sx.lib()
+    for s in syms:
+        assert hasattr(L, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", sx._LIBPATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (scuffx_\w+)", out))
+    assert set(syms) == exported
+
+
+def test_abi_version_and_errors(sx):
+    L = sx.lib()
+    assert L.scuffx_version() >= 100
+    # argument errors are reported without touching the device
+    assert L.scuffx_geom_total_bfs(None) == -1
+    assert b"null" in L.scuffx_last_error()
+    assert L.scuffx_lu_factor(None, 4, 4, None, 64) == -1
+    assert L.scuffx_lu_solve(None, 4, 4, None, b"N", 1, None, 4, 64) == -1
+    with pytest.raises(sx.SCUFFXError):
+        G = sx.RWGGeometry()
+        G.AddMesh(np.zeros((3, 3)), np.array([[0, 1, 2]]), "NOT_A_MATERIAL")
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 4])
+def test_icosphere_closed_forms(sx, O, level):
+    V, T = sx.icosphere(level)
+    assert V.shape == (10 * 4 ** level + 2, 3) and T.shape == (20 * 4 ** level, 3)
+    assert np.allclose(np.linalg.norm(V, axis=1), 1.0, atol=1e-15)
+    Vo, To = O.icosphere(level)
+    assert np.array_equal(V, Vo) and np.array_equal(T, To)
+    G = sx.RWGGeometry()
+    G.AddMesh(V, T, "PEC")
+    nV, nP, nE = G.Counts(0)
+    assert nE == 30 * 4 ** level and nV - nE + nP == 2
+    assert G.TotalBFs == nE
+    # outward normals
+    P = G.Panels(0)
+    assert np.all(np.einsum("ij,ij->i", P[:, 0:3], P[:, 3:6]) > 0)
+
+
+def _compare_topology(G, Go, ns=0):
+    e, le = G.Edges(ns)
+    eo, leo = Go.edges(ns)
+    assert np.array_equal(e, eo)
+    assert np.array_equal(le, leo)
+    assert np.array_equal(G.Panels(ns), Go.panels(ns))
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_edges_bit_exact_icosphere(sx, O, level):
+    V, T = O.icosphere(level)
+    G = sx.RWGGeometry()
+    G.AddMesh(V, T, "CONST_EPS_4+0.1i")
+    Go = O.Geometry()
+    Go.add_surface(V, T, kind=1, eps=4 + 0.1j)
+    assert G.TotalBFs == Go.N == 2 * 30 * 4 ** level
+    _compare_topology(G, Go)
+
+
+@pytest.mark.parametrize("mesh,nedges", [("UnitTestSphere_R0P75_414.msh", 414), ("SSphere_255.msh", 357)])
+def test_edges_bit_exact_reference_meshes(sx, O, mesh, nedges):
+    path = os.path.join(GOLDEN, mesh)
+    G = sx.RWGGeometry()
+    G.AddMeshFile(path, "PEC")
+    Go = O.Geometry()
+    Go.add_msh(path)
+    nV, nP, nE = G.Counts(0)
+    assert nE == nedges and nV - nE + nP == 2
+    _compare_topology(G, Go)
+
+
+def test_gmsh_roundtrip(sx, O, tmp_path):
+    V, T = O.icosphere(2, 1.0)
+    fn = str(tmp_path / "s.msh")
+    O.write_gmsh(fn, V, T)
+    G = sx.RWGGeometry()
+    G.AddMeshFile(fn, "PEC")
+    Go = O.Geometry()
+    Go.add_surface(V, T, kind=0)
+    _compare_topology(G, Go)
+
+
+def test_scuffgeo_objects_and_displacement(sx, O, tmp_path):
+    mesh = os.path.join(GOLDEN, "SSphere_255.msh")
+    geo = tmp_path / "two.scuffgeo"
+    geo.write_text(
+        "# two spheres\n"
+        f"OBJECT Lower\n  MESHFILE {mesh}\nENDOBJECT\n\n"
+        f"OBJECT Upper\n  MESHFILE {mesh}\n  DISPLACED 0 0 3\n  MATERIAL CONST_EPS_4+0.1i\nENDOBJECT\n")
+    G = sx.RWGGeometry(str(geo))
+    assert G.NumSurfaces == 2
+    assert G.TotalBFs == 357 + 2 * 357
+    # the oracle sees the displaced vertices directly
+    Go = O.Geometry()
+    Go.add_msh(mesh)
+    P0 = Go.panels(0)
+    P1 = G.Panels(1)
+    assert np.array_equal(P1[:, 3:], P0[:, 3:]) or np.allclose(P1[:, 3:], P0[:, 3:], rtol=0, atol=1e-15)
+    assert np.allclose(P1[:, 0:3] - P0[:, 0:3], [0, 0, 3], rtol=0, atol=1e-14)
+
+
+def test_scuffgeo_rotation_composes(sx, tmp_path):
+    mesh = os.path.join(GOLDEN, "SSphere_255.msh")
+    geo = tmp_path / "rot.scuffgeo"
+    geo.write_text(f"OBJECT A\n MESHFILE {mesh}\n ROTATED 90 ABOUT 0 0 1\n DISPLACED 1 0 0\nENDOBJECT\n")
+    G = sx.RWGGeometry(str(geo))
+    G0 = sx.RWGGeometry()
+    G0.AddMeshFile(mesh)
+    c0 = G0.Panels(0)[:, 0:3]
+    c1 = G.Panels(0)[:, 0:3]
+    want = np.stack([-c0[:, 1] + 1, c0[:, 0], c0[:, 2]], axis=1)
+    assert np.allclose(c1, want, atol=1e-14)
+
+
+def test_scuffgeo_errors(sx, tmp_path):
+    bad = tmp_path / "bad.scuffgeo"
+    bad.write_text("OBJECT A\n MESHFILE nothere.msh\nENDOBJECT\n")
+    with pytest.raises(sx.SCUFFXError):
+        sx.RWGGeometry(str(bad))
+    bad.write_text("OBJECT A\n MESHFILE x.msh\n DISPLACED 1 2\nENDOBJECT\n")
+    with pytest.raises(sx.SCUFFXError, match="DISPLACED"):
+        sx.RWGGeometry(str(bad))
+
+
+@pytest.mark.parametrize("name,ok", [("PEC", True), ("VACUUM", True), ("CONST_EPS_12+0.01i", True),
+                                     ("CONST_EPS_2.5_MU_1.5", True), ("DRUDE_10_0.1", True),
+                                     ("CONST_EPS_2.5_SIGMA_1e3", True), ("GOLD", False),
+                                     ("CONST_EPS_x", False)])
+def test_material_names(sx, name, ok):
+    V, T = sx.icosphere(0)
+    G = sx.RWGGeometry()
+    if ok:
+        G.AddMesh(V, T, name)
+        assert G.TotalBFs == (30 if name == "PEC" else 60)
+    else:
+        with pytest.raises(sx.SCUFFXError):
+            G.AddMesh(V, T, name)
