@@ -244,6 +244,8 @@ def zgetrs(LU, ipiv, B, trans="N"):
     B = np.asfortranarray(B, dtype=np.complex128).copy(order="F")
     if B.ndim == 1:
         B = B.reshape(-1, 1, order="F")
+    LU = np.asfortranarray(LU, dtype=np.complex128)
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
     n = LU.shape[0]
     lib().o_api_zgetrs(trans.encode(), n, B.shape[1], LU, n, ipiv, B, n)
     return B

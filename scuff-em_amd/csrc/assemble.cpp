@@ -277,7 +277,11 @@ void Assembler::build_near()
           }
         }
       }
+      // ordered near pairs that no edge pair of the block reaches (for
+      // example (pb, pa) when every edge of pb precedes every edge of pa
+      // on one surface) get no record: idx stays -1
       for (size_t q = 0; q < BB.col.size(); q++) {
+        if (first[q] == ~0ULL) continue;
         int pa = (int)(std::upper_bound(BB.rowPtr.begin(), BB.rowPtr.end(), (int)q) - BB.rowPtr.begin()) - 1;
         occ.push_back({first[q], bi, pa, (int)q});
       }

@@ -325,7 +325,12 @@ __device__ inline void fetch_h(int pa, int qa, int pb, int qb, int r, int nreg, 
   }
   int ni = near_find(nearRowPtr, nearCol, pa, pb);
   if (ni >= 0) {
-    rec = HN + ((long long)nearIdx[ni] * nreg + r) * 9 * stride + (3 * qa + qb) * stride;
+    int id = nearIdx[ni];
+    if (id < 0) {   // a near pair outside the upper-triangle walk: never requested
+      *h0 = mk(0, 0); *h1 = mk(0, 0);
+      return;
+    }
+    rec = HN + ((long long)id * nreg + r) * 9 * stride + (3 * qa + qb) * stride;
   } else if (same && pb < pa) {
     rec = HP + pair_index(pb, pa, nPa, nPb, same) * (nreg * 9 * stride) + (r * 9 + 3 * qb + qa) * stride;
   } else {

@@ -30,17 +30,23 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct CT;
+// Both 16x16x4 instructions take A[i][k] from lane i + 16k and B[k][j]
+// from lane j + 16k; result element r of a lane sits in column lane % 16
+// and row 4*(lane/16) + r for f32 but row lane/16 + 4r for f64 (measured
+// on gfx950 with a one-wave probe).
 template <> struct CT<double2> {
   typedef double R;
   typedef d4 V4;
   __device__ static inline V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
   __device__ static inline V4 zero() { return V4{0, 0, 0, 0}; }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct CT<float2> {
   typedef float R;
   typedef f4 V4;
   __device__ static inline V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   __device__ static inline V4 zero() { return V4{0, 0, 0, 0}; }
+  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <typename T> __device__ inline T cmul(T a, T b) { T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r; }
@@ -119,7 +125,7 @@ k_gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T
   for (int ti = 0; ti < 2; ti++)
     for (int tj = 0; tj < 2; tj++)
       for (int r = 0; r < 4; r++) {
-        long gi = row0 + wr * 32 + ti * 16 + 4 * (lane >> 4) + r;
+        long gi = row0 + wr * 32 + ti * 16 + CT<T>::row(lane, r);
         long gj = col0 + wc * 32 + tj * 16 + (lane & 15);
         if (gi < m && gj < n) {
           T c = C[gi + gj * ldc];
