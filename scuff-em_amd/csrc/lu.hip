@@ -70,16 +70,17 @@ template <typename T> __device__ inline T cdiv(T a, T b)
 }
 
 // ---------------------------------------------------------------------
-// C[m x n] -= A[m x k] B[k x n]; 64x64 block tile, 4 waves of 32x32,
-// K staged 16 at a time through LDS as separate real/imag planes.
+// C[m x n] -= A[m x k] B[k x n]  (ACC)  or  C = A B  (!ACC);
+// 64x64 block tile, 4 waves of 32x32, K staged 16 at a time through LDS
+// as separate real/imag planes, four real MFMAs per complex product.
 // ---------------------------------------------------------------------
 #define GBM 64
 #define GBN 64
 #define GBK 16
 
-template <typename T>
+template <typename T, bool ACC>
 __global__ void __launch_bounds__(256)
-k_gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc)
+k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc)
 {
   typedef typename CT<T>::R R;
   typedef typename CT<T>::V4 V4;
@@ -114,10 +115,12 @@ k_gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T
       }
       for (int ti = 0; ti < 2; ti++)
         for (int tj = 0; tj < 2; tj++) {
-          accr[ti][tj] = CT<T>::mfma(-ar[ti], br[tj], accr[ti][tj]);
-          accr[ti][tj] = CT<T>::mfma(ai[ti], bi[tj], accr[ti][tj]);
-          acci[ti][tj] = CT<T>::mfma(-ar[ti], bi[tj], acci[ti][tj]);
-          acci[ti][tj] = CT<T>::mfma(-ai[ti], br[tj], acci[ti][tj]);
+          // acc = Re/Im of -(A B) when accumulating into C, +(A B) otherwise
+          const R sg = ACC ? (R)-1 : (R)1;
+          accr[ti][tj] = CT<T>::mfma(sg * ar[ti], br[tj], accr[ti][tj]);
+          accr[ti][tj] = CT<T>::mfma(-sg * ai[ti], bi[tj], accr[ti][tj]);
+          acci[ti][tj] = CT<T>::mfma(sg * ar[ti], bi[tj], acci[ti][tj]);
+          acci[ti][tj] = CT<T>::mfma(sg * ai[ti], br[tj], acci[ti][tj]);
         }
     }
     __syncthreads();
@@ -128,103 +131,146 @@ k_gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T
         long gi = row0 + wr * 32 + ti * 16 + CT<T>::row(lane, r);
         long gj = col0 + wc * 32 + tj * 16 + (lane & 15);
         if (gi < m && gj < n) {
-          T c = C[gi + gj * ldc];
-          c.x += accr[ti][tj][r];
-          c.y += acci[ti][tj][r];
+          T c;
+          if (ACC) { c = C[gi + gj * ldc]; c.x += accr[ti][tj][r]; c.y += acci[ti][tj][r]; }
+          else { c.x = accr[ti][tj][r]; c.y = acci[ti][tj][r]; }
           C[gi + gj * ldc] = c;
         }
       }
+}
+
+template <typename T, bool ACC>
+static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
+                 hipStream_t s)
+{
+  if (m <= 0 || n <= 0 || k <= 0) return;
+  dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
+  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
+  hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
 }
 
 template <typename T>
 static void gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
                      hipStream_t s)
 {
-  if (m <= 0 || n <= 0 || k <= 0) return;
-  dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
-  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
-  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
-  prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
+  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s);
 }
 
 // ---------------------------------------------------------------------
-// cooperative panel factorization: rows [r0, r0+m), columns [r0, r0+w)
+// cooperative panel factorization: rows [r0, r0+m), columns [r0, r0+w),
+// w <= PW.  Workgroup g keeps rows [r0 + g*rows, ...) of the panel in
+// LDS for the whole panel.  Per column one grid barrier: before it every
+// workgroup publishes its best |re|+|im| candidate together with that
+// candidate's row, and the owner of the diagonal row publishes that row;
+// after it every workgroup picks the global pivot (largest, first row on
+// ties, as izamax) and has both rows of the interchange at hand.  The
+// published slots alternate between two buffers so that a workgroup
+// running ahead never overwrites a slot another one still reads.
 // ---------------------------------------------------------------------
+#define PW 32
+
+template <typename T> struct PanelRows;
+template <> struct PanelRows<double2> { static const int n = 128; };
+template <> struct PanelRows<float2> { static const int n = 256; };
+
+struct PanelScratch {
+  double *candv;   // [2][Gmax]
+  int *candi;      // [2][Gmax]
+  void *candrow;   // [2][Gmax][PW] T
+  void *rowc;      // [2][PW] T
+};
+
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, float *candv, int *candi,
-        T *prow, T *jrow, long chunk)
+__global__ void __launch_bounds__(PanelRows<T>::n)
+k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax)
 {
   typedef decltype(T().x) R;
+  constexpr int NR = PanelRows<T>::n;
   cg::grid_group grid = cg::this_grid();
-  __shared__ R sv[256];
-  __shared__ long si[256];
-  __shared__ T spiv[32];
-  __shared__ long sp;
-  const int tid = threadIdx.x, G = gridDim.x, g = blockIdx.x;
-  const long lo = r0 + (long)g * chunk, hi = std::min(r0 + m, lo + chunk);
+  __shared__ T slab[PW][NR];           // column-major: slab[q][t] = A(lo + t, r0 + q)
+  __shared__ R wv[NR / 64];
+  __shared__ int wi[NR / 64];
+  __shared__ T spiv[PW], sold[PW];
+  __shared__ int s_best, s_p;
+  __shared__ R s_bv;
+  const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
+  const long lo = r0 + (long)g * NR;
+  const int nrow = (int)std::min<long>(NR, r0 + m - lo);
+  const long i = lo + t;                 // this thread's row
+  T *candrow = (T *)S.candrow, *rowc = (T *)S.rowc;
+  for (int q = 0; q < w; q++)
+    if (t < nrow) slab[q][t] = A[i + (r0 + q) * lda];
+  __syncthreads();
   for (int j = 0; j < w; j++) {
     const long c = r0 + j;
-    // local maximum of |re|+|im| over own rows >= r0+j; smallest row on ties
-    R best = -1; long bi = -1;
-    for (long i = std::max(lo, r0 + j) + tid; i < hi; i += 256) {
-      T v = A[i + c * lda];
-      R a = fabs(v.x) + fabs(v.y);
-      if (a > best) { best = a; bi = i; }
+    const int buf = j & 1;
+    // local candidate: largest |re|+|im| among own rows >= c, first on ties
+    R best = -1; int bi = -1;
+    if (t < nrow && i >= c) { T v = slab[j][t]; best = fabs(v.x) + fabs(v.y); bi = t; }
+    for (int off = 32; off > 0; off >>= 1) {
+      R ob = __shfl_xor(best, off);
+      int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi >= 0 && (bi < 0 || oi < bi))) { best = ob; bi = oi; }
     }
-    sv[tid] = best; si[tid] = bi;
+    if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) {
-        R ov = sv[tid + s]; long oi = si[tid + s];
-        if (ov > sv[tid] || (ov == sv[tid] && oi >= 0 && (si[tid] < 0 || oi < si[tid]))) { sv[tid] = ov; si[tid] = oi; }
-      }
-      __syncthreads();
+    if (t == 0) {
+      R b = wv[0]; int bb = wi[0];
+      for (int q = 1; q < NR / 64; q++)
+        if (wv[q] > b || (wv[q] == b && wi[q] >= 0 && (bb < 0 || wi[q] < bb))) { b = wv[q]; bb = wi[q]; }
+      s_best = bb;
+      S.candv[buf * Gmax + g] = (double)b;
+      S.candi[buf * Gmax + g] = bb >= 0 ? (int)(lo + bb) : -1;
     }
-    if (tid == 0) {
-      ((R *)(candv))[g] = sv[0];
-      candi[g] = (int)si[0];
+    __syncthreads();
+    if (t < w) {
+      if (s_best >= 0) candrow[((long)buf * Gmax + g) * PW + t] = slab[t][s_best];
+      if (c >= lo && c < lo + nrow) rowc[buf * PW + t] = slab[t][c - lo];
     }
     grid.sync();
-    if (tid == 0) {
-      R b = -1; long p = -1;
-      for (int q = 0; q < G; q++) {
-        R v = ((R *)candv)[q]; long ii = candi[q];
+    // global pivot
+    if (t < 64) {
+      R b = -1; int p = -1, gw = -1;
+      for (int q = t; q < G; q += 64) {
+        R v = (R)S.candv[buf * Gmax + q]; int ii = S.candi[buf * Gmax + q];
         if (ii < 0) continue;
-        if (v > b || (v == b && ii < p)) { b = v; p = ii; }
+        if (v > b || (v == b && ii < p)) { b = v; p = ii; gw = q; }
       }
-      if (p < 0) p = c;
-      sp = p;
-      if (g == 0) {
-        ipiv[c] = (int)(p + 1);
-        if (b == 0 && *info == 0) *info = (int)(c + 1);
+      for (int off = 32; off > 0; off >>= 1) {
+        R ob = __shfl_xor(b, off);
+        int op = __shfl_xor(p, off), og = __shfl_xor(gw, off);
+        if (op >= 0 && (ob > b || (ob == b && (p < 0 || op < p)))) { b = ob; p = op; gw = og; }
       }
+      if (t == 0) { s_p = p < 0 ? (int)c : p; s_best = gw; s_bv = b; }
     }
     __syncthreads();
-    const long p = sp;
-    // owners publish the two rows (panel columns only)
-    if (p >= lo && p < hi) for (int q = tid; q < w; q += 256) prow[q] = A[p + (r0 + q) * lda];
-    if (c >= lo && c < hi) for (int q = tid; q < w; q += 256) jrow[q] = A[c + (r0 + q) * lda];
-    grid.sync();
-    if (p != c) {
-      if (c >= lo && c < hi) for (int q = tid; q < w; q += 256) A[c + (r0 + q) * lda] = prow[q];
-      if (p >= lo && p < hi) for (int q = tid; q < w; q += 256) A[p + (r0 + q) * lda] = jrow[q];
+    const long p = s_p;
+    if (t < w) {
+      sold[t] = rowc[buf * PW + t];
+      spiv[t] = s_best >= 0 ? candrow[((long)buf * Gmax + s_best) * PW + t] : sold[t];
     }
-    for (int q = tid; q < w; q += 256) spiv[q] = prow[q];
+    if (g == 0 && t == 0) {
+      ipiv[c] = (int)(p + 1);
+      if (s_bv == 0 && *info == 0) *info = (int)(c + 1);
+    }
+    __syncthreads();
+    if (t < w) {
+      if (c >= lo && c < lo + nrow) slab[t][c - lo] = spiv[t];
+      if (p != c && p >= lo && p < lo + nrow) slab[t][p - lo] = sold[t];
+    }
     __syncthreads();
     T piv = spiv[j];
-    bool nz = (piv.x != 0 || piv.y != 0);
-    T rp = nz ? crecip(piv) : piv;
-    for (long i = std::max(lo, c + 1) + tid; i < hi; i += 256) {
-      T l = A[i + c * lda];
-      if (nz) { l = cmul(l, rp); A[i + c * lda] = l; }
-      for (int q = j + 1; q < w; q++) {
-        T a = A[i + (r0 + q) * lda];
-        A[i + (r0 + q) * lda] = csub(a, cmul(l, spiv[q]));
-      }
+    const bool nz = (piv.x != 0 || piv.y != 0);
+    if (t < nrow && i > c) {
+      T l = slab[j][t];
+      if (nz) { l = cmul(l, crecip(piv)); slab[j][t] = l; }
+      for (int q = j + 1; q < w; q++) slab[q][t] = csub(slab[q][t], cmul(l, spiv[q]));
     }
     __syncthreads();
   }
+  for (int q = 0; q < w; q++)
+    if (t < nrow) A[i + (r0 + q) * lda] = slab[q][t];
 }
 
 // row interchanges rows [k1,k2) (absolute, ipiv 1-based) on columns [c0,c1)
@@ -246,29 +292,58 @@ static void laswp(T *A, long lda, long c0, long c1, long k1, long k2, const int 
   hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, s, A, lda, c0, c1, k1, k2, ipiv);
 }
 
-// B[w x ncol] <- L^{-1} B, L unit lower w x w (A block at (r0,r0)); one
-// thread per column, L read through the cache
+// B[w x ncol] <- L^{-1} B for the unit-lower w x w block of the panel
+// (w <= PW): L staged in LDS, each thread carries one column in registers
 template <typename T>
-__global__ void k_trsm_llu(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol)
+__global__ void __launch_bounds__(128)
+k_trsm_small(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol)
 {
+  __shared__ T L[PW][PW + 1];
+  for (int e = threadIdx.x; e < PW * PW; e += blockDim.x) {
+    int r = e % PW, q = e / PW;
+    if (r < w && q < w) L[r][q] = A[(r0 + r) + (r0 + q) * lda];
+  }
+  __syncthreads();
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncol) return;
   T *b = B + c * ldb;
-  for (int i = 0; i < w; i++) {
-    T x = b[i];
-    if (x.x == 0 && x.y == 0) continue;
-    const T *Lc = A + (r0 + i) * lda + r0;
-    for (int r = i + 1; r < w; r++) b[r] = csub(b[r], cmul(Lc[r], x));
+  T x[PW];
+#pragma unroll
+  for (int r = 0; r < PW; r++) if (r < w) x[r] = b[r];
+#pragma unroll
+  for (int q = 0; q < PW; q++) {
+    if (q < w) {
+#pragma unroll
+      for (int r = q + 1; r < PW; r++)
+        if (r < w) x[r] = csub(x[r], cmul(L[r][q], x[q]));
+    }
   }
+#pragma unroll
+  for (int r = 0; r < PW; r++) if (r < w) b[r] = x[r];
 }
 
 template <typename T>
-static void trsm_llu(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol, hipStream_t s)
+static void trsm_small(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol, hipStream_t s)
 {
   if (ncol <= 0 || w <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
-  hipLaunchKernelGGL(k_trsm_llu<T>, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, s, A, lda, r0, w, B, ldb, ncol);
+  hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, s, A, lda, r0, w, B, ldb, ncol);
   prof_end(s, tok, 4.0 * (double)w * (double)w * (double)ncol);
+}
+
+// B[nb x ncol] <- L^{-1} B with the nb x nb unit-lower block at A(r0,r0),
+// by PW-row steps: substitution on the diagonal step (trsm_small) and a
+// GEMM update of the rows below it, as a blocked LAPACK trsm does
+template <typename T>
+static void trsm_block(const T *A, long lda, long r0, int nb, T *B, long ldb, long ncol, hipStream_t s)
+{
+  for (int i0 = 0; i0 < nb; i0 += PW) {
+    int w = std::min(PW, nb - i0);
+    trsm_small(A, lda, r0 + i0, w, B + i0, ldb, ncol, s);
+    if (i0 + w < nb)
+      gemm_sub((long)(nb - i0 - w), ncol, (long)w, A + (r0 + i0 + w) + (r0 + i0) * lda, lda, B + i0, ldb,
+               B + i0 + w, ldb, s);
+  }
 }
 
 template <typename T>
@@ -276,41 +351,43 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
 {
   if (n < 0 || lda < std::max(1L, n)) return -1;
   if (n == 0) return 0;
-  const int ib = 32;
-  int dev = 0, ncu = 0;
+  if (nb > 256) nb = 256;
+  const int ib = PW;
+  constexpr int NR = PanelRows<T>::n;
+  const long Gmax = (n + NR - 1) / NR;
+  int dev = 0, ncu = 0, perCU = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  int perCU = 0;
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_panel<T>, 256, 0);
-  long maxG = (long)std::max(1, ncu) * std::max(1, std::min(perCU, 2));
-  float *candv = nullptr; int *candi = nullptr, *dinfo = nullptr; T *prow = nullptr, *jrow = nullptr;
-  if (hipMalloc(&candv, sizeof(double) * 1024) != hipSuccess || hipMalloc(&candi, sizeof(int) * 1024) != hipSuccess ||
-      hipMalloc(&prow, sizeof(T) * 64) != hipSuccess || hipMalloc(&jrow, sizeof(T) * 64) != hipSuccess ||
-      hipMalloc(&dinfo, sizeof(int)) != hipSuccess)
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_panel<T>, NR, 0);
+  if ((long)std::max(1, ncu) * std::max(1, perCU) < Gmax) return -5;   // panel would not be co-resident
+  PanelScratch S;
+  int *dinfo = nullptr;
+  if (hipMalloc(&S.candv, sizeof(double) * 2 * Gmax) != hipSuccess ||
+      hipMalloc(&S.candi, sizeof(int) * 2 * Gmax) != hipSuccess ||
+      hipMalloc(&S.candrow, sizeof(T) * 2 * Gmax * PW) != hipSuccess ||
+      hipMalloc(&S.rowc, sizeof(T) * 2 * PW) != hipSuccess || hipMalloc(&dinfo, sizeof(int)) != hipSuccess)
     return -2;
   hipMemsetAsync(dinfo, 0, sizeof(int), stream);
   int rc = 0;
+  int gmaxi = (int)Gmax;
   for (long k0 = 0; k0 < n && rc == 0; k0 += nb) {
     long jb = std::min<long>(nb, n - k0);
     for (long j0 = k0; j0 < k0 + jb; j0 += ib) {
       int w = (int)std::min<long>(ib, k0 + jb - j0);
       long m = n - j0;
-      long G = std::min<long>(maxG, std::min<long>(1024, (m + 255) / 256));
-      if (G < 1) G = 1;
-      long chunk = (m + G - 1) / G;
-      G = (m + chunk - 1) / chunk;
+      long G = (m + NR - 1) / NR;
       long r0 = j0;
       void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
-                      (void *)&dinfo, (void *)&candv, (void *)&candi, (void *)&prow, (void *)&jrow, (void *)&chunk};
+                      (void *)&dinfo, (void *)&S, (void *)&gmaxi};
       int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
-      hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3((unsigned)G), dim3(256), args, 0, stream);
+      hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3((unsigned)G), dim3(NR), args, 0, stream);
       prof_end(stream, tok, 0.0);
       if (e != hipSuccess) { rc = -3; break; }
       laswp(A, lda, k0, j0, j0, j0 + w, d_ipiv, stream);
       laswp(A, lda, j0 + w, k0 + jb, j0, j0 + w, d_ipiv, stream);
       long nr = k0 + jb - (j0 + w);
       if (nr > 0) {
-        trsm_llu(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
+        trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
         gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
                  A + (j0 + w) + (j0 + w) * lda, lda, stream);
       }
@@ -319,7 +396,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
     laswp(A, lda, k0 + jb, n, k0, k0 + jb, d_ipiv, stream);
     long nr = n - k0 - jb;
     if (nr > 0) {
-      trsm_llu(A, lda, k0, (int)jb, A + k0 + (k0 + jb) * lda, lda, nr, stream);
+      trsm_block(A, lda, k0, (int)jb, A + k0 + (k0 + jb) * lda, lda, nr, stream);
       gemm_sub(nr, nr, jb, A + (k0 + jb) + k0 * lda, lda, A + k0 + (k0 + jb) * lda, lda,
                A + (k0 + jb) + (k0 + jb) * lda, lda, stream);
     }
@@ -327,11 +404,10 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream);
   hipStreamSynchronize(stream);
   if (prof_enabled()) prof_collect();
-  hipFree(candv); hipFree(candi); hipFree(prow); hipFree(jrow); hipFree(dinfo);
+  hipFree(S.candv); hipFree(S.candi); hipFree(S.candrow); hipFree(S.rowc); hipFree(dinfo);
   if (rc == 0 && hipGetLastError() != hipSuccess) rc = -3;
   return rc;
 }
-
 // ---------------------------------------------------------------------
 // triangular solves.  Blocked for trans='N': a diagonal block is solved
 // by one workgroup per right-hand side, the rest updated by the GEMM.
@@ -406,12 +482,95 @@ __global__ void k_laswp_rev(T *B, long ldb, long n, long nrhs, const int *ipiv)
   }
 }
 
+// One 64-row block step of a triangular solve with few right-hand sides
+// (blockIdx.y = right-hand side).  Every workgroup stages the diagonal
+// block in LDS and solves it redundantly with one wave (shuffles, no
+// barriers), then updates its own 256 rows outside the block with the
+// block's solution (coalesced column reads).  The solution segment is
+// written to Xout, never to the vector read in the same launch.
+//   LOWER: unit-lower L, rows below the block are updated in Bin.
+//  !LOWER: upper U with its diagonal, rows above the block are updated.
+#define TB 64
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256)
+k_trsv_step(const T *LU, long lda, long n, long kb, int bs, T *Bin, long ldb, T *Xout, long ldx)
+{
+  __shared__ T D[TB][TB + 1];
+  __shared__ T xs[TB];
+  const int t = threadIdx.x;
+  T *b = Bin + (long)blockIdx.y * ldb;
+  T *xo = Xout + (long)blockIdx.y * ldx;
+  for (int e = t; e < TB * TB; e += 256) {
+    int r = e % TB, q = e / TB;
+    if (r < bs && q < bs) D[r][q] = LU[(kb + r) + (kb + q) * lda];
+  }
+  if (t < bs) xs[t] = b[kb + t];
+  __syncthreads();
+  if (t < 64) {
+    T x;
+    x.x = 0; x.y = 0;
+    if (t < bs) x = xs[t];
+    if (LOWER) {
+      for (int i = 0; i < bs; i++) {
+        T xi;
+        xi.x = __shfl(x.x, i); xi.y = __shfl(x.y, i);
+        if (t > i && t < bs) x = csub(x, cmul(D[t][i], xi));
+      }
+    } else {
+      for (int i = bs - 1; i >= 0; i--) {
+        if (t == i) x = cdiv(x, D[i][i]);
+        T xi;
+        xi.x = __shfl(x.x, i); xi.y = __shfl(x.y, i);
+        if (t < i) x = csub(x, cmul(D[t][i], xi));
+      }
+    }
+    if (t < bs) xs[t] = x;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && t < bs) xo[kb + t] = xs[t];
+  const long row = LOWER ? kb + bs + (long)blockIdx.x * 256 + t : (long)blockIdx.x * 256 + t;
+  const long rend = LOWER ? n : kb;
+  if (row < rend) {
+    T acc;
+    acc.x = 0; acc.y = 0;
+    for (int i = 0; i < bs; i++) {
+      T a = LU[row + (kb + i) * lda], v = xs[i];
+      acc.x += a.x * v.x - a.y * v.y;
+      acc.y += a.x * v.y + a.y * v.x;
+    }
+    b[row] = csub(b[row], acc);
+  }
+}
+
 template <typename T>
 int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B, long ldb,
              hipStream_t stream)
 {
   if (n < 0 || nrhs < 0 || lda < std::max(1L, n) || ldb < std::max(1L, n)) return -1;
   if (n == 0 || nrhs == 0) return 0;
+  if ((trans == 'N' || trans == 'n') && nrhs <= 16) {
+    T *W = nullptr;
+    if (hipMalloc(&W, sizeof(T) * n * nrhs) != hipSuccess) return -2;
+    laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);
+    // L y = P b: y accumulates in W, B holds the running right-hand side
+    for (long kb = 0; kb < n; kb += TB) {
+      int bs = (int)std::min<long>(TB, n - kb);
+      long rows = n - kb - bs;
+      dim3 grid((unsigned)std::max<long>(1, (rows + 255) / 256), (unsigned)nrhs);
+      hipLaunchKernelGGL((k_trsv_step<T, true>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs, B, ldb, W, n);
+    }
+    // U x = y: W is updated in place, the solution lands in B
+    long last = ((n - 1) / TB) * TB;
+    for (long kb = last; kb >= 0; kb -= TB) {
+      int bs = (int)std::min<long>(TB, n - kb);
+      dim3 grid((unsigned)std::max<long>(1, (kb + 255) / 256), (unsigned)nrhs);
+      hipLaunchKernelGGL((k_trsv_step<T, false>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs, W, n, B, ldb);
+    }
+    hipError_t e = hipGetLastError();
+    hipStreamSynchronize(stream);
+    hipFree(W);
+    return e == hipSuccess ? 0 : -3;
+  }
   if (trans == 'N' || trans == 'n') {
     laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);
     const int bs = 256;
