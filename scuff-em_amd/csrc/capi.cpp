@@ -14,6 +14,7 @@
 #include "lu.h"
 #include "prof.h"
 #include "rhs.h"
+#include "ws.h"
 
 using namespace scuffx;
 
@@ -198,7 +199,10 @@ int scuffx_lu_factor(void *A, long n, long lda, int *ipiv, int prec)
   if (prec == 64) rc = lu_factor<double2>((double2 *)A, n, lda, ipiv, &info, g_stream);
   else if (prec == 32) rc = lu_factor<float2>((float2 *)A, n, lda, ipiv, &info, g_stream);
   else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
-  if (rc) return fail(rc, "LU factorization failed on the device");
+  if (rc == -5) return fail(SCUFFX_ERR_UNSUPPORTED, "matrix too large for a co-resident panel grid");
+  if (rc == -6) return fail(SCUFFX_ERR_HIP, "panel grid barrier timed out");
+  if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "LU workspace allocation failed");
+  if (rc) return fail(SCUFFX_ERR_HIP, "LU factorization failed on the device");
   return info;
 }
 
@@ -210,7 +214,8 @@ int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char tran
   else if (prec == 32) rc = lu_solve<float2>((const float2 *)LU, n, lda, ipiv, trans, nrhs, (float2 *)B, ldb, g_stream);
   else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
   if (rc == -1) return fail(SCUFFX_ERR_ARG, "bad LU solve arguments");
-  if (rc) return fail(rc, "LU solve failed on the device");
+  if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "solve workspace allocation failed");
+  if (rc) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
   if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
   return 0;
 }
@@ -227,12 +232,13 @@ int scuffx_refine(const void *M, long n, long ldm, const void *LU32, long ld32, 
 {
   if (!M || !LU32 || !ipiv || !b || !x || n <= 0) return fail(SCUFFX_ERR_ARG, "bad arguments");
   int nch = (int)((n + 511) / 512);
-  double2 *r = nullptr, *work = nullptr;
-  float2 *r32 = nullptr;
-  double *scal = nullptr;
-  if (hipMalloc(&r, sizeof(double2) * n) != hipSuccess || hipMalloc(&work, sizeof(double2) * n * nch) != hipSuccess ||
-      hipMalloc(&r32, sizeof(float2) * n) != hipSuccess || hipMalloc(&scal, sizeof(double)) != hipSuccess)
-    return fail(SCUFFX_ERR_ALLOC, "workspace allocation failed");
+  size_t nr = 256 * ((sizeof(double2) * n + 255) / 256), nw = 256 * ((sizeof(double2) * n * nch + 255) / 256);
+  size_t n32 = 256 * ((sizeof(float2) * n + 255) / 256);
+  char *wsb = (char *)ws_get(WS_REFINE, nr + nw + n32 + 256);
+  if (!wsb) return fail(SCUFFX_ERR_ALLOC, "workspace allocation failed");
+  double2 *r = (double2 *)wsb, *work = (double2 *)(wsb + nr);
+  float2 *r32 = (float2 *)(wsb + nr + nw);
+  double *scal = (double *)(wsb + nr + nw + n32);
   hipStream_t s = g_stream;
   const double2 *bb = (const double2 *)b;
   double2 *xx = (double2 *)x;
@@ -257,7 +263,6 @@ int scuffx_refine(const void *M, long n, long ldm, const void *LU32, long ld32, 
   }
   hipStreamSynchronize(s);
   if (prof_enabled()) prof_collect();
-  hipFree(r); hipFree(work); hipFree(r32); hipFree(scal);
   if (iters) *iters = it;
   if (relres) *relres = rel;
   return 0;

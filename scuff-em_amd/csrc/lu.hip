@@ -21,6 +21,7 @@
 
 #include "lu.h"
 #include "prof.h"
+#include "ws.h"
 
 namespace cg = cooperative_groups;
 
@@ -91,17 +92,31 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
   V4 accr[2][2], acci[2][2];
   for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); }
   for (long k0 = 0; k0 < k; k0 += GBK) {
+    // all eight global loads in flight before any LDS store
+    T v[4], u[4];
+    if (row0 + GBM <= m && col0 + GBN <= n && k0 + GBK <= k) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        v[s] = A[(row0 + (tid & 63)) + (k0 + (tid >> 6) + 4 * s) * lda];
+        u[s] = B[(k0 + (tid & 15)) + (col0 + (tid >> 4) + 16 * s) * ldb];
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        int i = tid & 63, kk = (tid >> 6) + 4 * s;
+        long gi = row0 + i, gk = k0 + kk;
+        if (gi < m && gk < k) v[s] = A[gi + gk * lda]; else { v[s].x = 0; v[s].y = 0; }
+        int kb = tid & 15, j = (tid >> 4) + 16 * s;
+        long gkb = k0 + kb, gj = col0 + j;
+        if (gkb < k && gj < n) u[s] = B[gkb + gj * ldb]; else { u[s].x = 0; u[s].y = 0; }
+      }
+    }
+#pragma unroll
     for (int s = 0; s < 4; s++) {
       int i = tid & 63, kk = (tid >> 6) + 4 * s;
-      long gi = row0 + i, gk = k0 + kk;
-      T v;
-      if (gi < m && gk < k) v = A[gi + gk * lda]; else { v.x = 0; v.y = 0; }
-      sAr[kk][i] = v.x; sAi[kk][i] = v.y;
+      sAr[kk][i] = v[s].x; sAi[kk][i] = v[s].y;
       int kb = tid & 15, j = (tid >> 4) + 16 * s;
-      long gkb = k0 + kb, gj = col0 + j;
-      T u;
-      if (gkb < k && gj < n) u = B[gkb + gj * ldb]; else { u.x = 0; u.y = 0; }
-      sBr[kb][j] = u.x; sBi[kb][j] = u.y;
+      sBr[kb][j] = u[s].x; sBi[kb][j] = u[s].y;
     }
     __syncthreads();
     for (int kk0 = 0; kk0 < GBK; kk0 += 4) {
@@ -139,14 +154,144 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       }
 }
 
+// Large-tile variant: 128 x (32 NT) per workgroup, each wave 64 x (16 NT)
+// as 4 x NT MFMA tiles, the next K slab prefetched into registers while
+// the current one is consumed from LDS.  NT = 4 for complex64 (128x128
+// tiles); NT = 2 for complex128, whose accumulators are twice as wide.
+#define HBM_ 128
+#define HBK_ 16
+static const bool kUseBigGemm = false;
+template <typename T> struct BigNT;
+template <> struct BigNT<double2> { static const int v = 2; };
+template <> struct BigNT<float2> { static const int v = 4; };
+
+template <typename T, bool ACC>
+__global__ void __launch_bounds__(256)
+k_gemm_big(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc)
+{
+  typedef typename CT<T>::R R;
+  typedef typename CT<T>::V4 V4;
+  constexpr int NT = BigNT<T>::v, BN = 32 * NT, NB = BN / 16;   // NB: B elements per thread
+  __shared__ R sAr[HBK_][HBM_ + 4], sAi[HBK_][HBM_ + 4], sBr[HBK_][BN + 4], sBi[HBK_][BN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const long row0 = (long)blockIdx.x * HBM_, col0 = (long)blockIdx.y * BN;
+  V4 accr[4][NT], acci[4][NT];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < NT; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); }
+  // per thread: 8 A elements (row tid%128, k = tid/128 + 2s) and
+  // NB B elements (k = tid%16, col tid/16 + 16s)
+  T pa[8], pb[NB];
+  const bool interior = row0 + HBM_ <= m && col0 + BN <= n;
+  auto gload = [&](long k0) {
+    if (interior && k0 + HBK_ <= k) {
+#pragma unroll
+      for (int s2 = 0; s2 < 8; s2++) pa[s2] = A[(row0 + (tid & 127)) + (k0 + (tid >> 7) + 2 * s2) * lda];
+#pragma unroll
+      for (int s2 = 0; s2 < NB; s2++) pb[s2] = B[(k0 + (tid & 15)) + (col0 + (tid >> 4) + 16 * s2) * ldb];
+      return;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < 8; s2++) {
+      int i = tid & 127, kk = (tid >> 7) + 2 * s2;
+      long gi = row0 + i, gk = k0 + kk;
+      if (gi < m && gk < k) pa[s2] = A[gi + gk * lda]; else { pa[s2].x = 0; pa[s2].y = 0; }
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < NB; s2++) {
+      int kb = tid & 15, j = (tid >> 4) + 16 * s2;
+      long gkb = k0 + kb, gj = col0 + j;
+      if (gkb < k && gj < n) pb[s2] = B[gkb + gj * ldb]; else { pb[s2].x = 0; pb[s2].y = 0; }
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int s2 = 0; s2 < 8; s2++) {
+      int i = tid & 127, kk = (tid >> 7) + 2 * s2;
+      sAr[kk][i] = pa[s2].x; sAi[kk][i] = pa[s2].y;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < NB; s2++) {
+      int kb = tid & 15, j = (tid >> 4) + 16 * s2;
+      sBr[kb][j] = pb[s2].x; sBi[kb][j] = pb[s2].y;
+    }
+  };
+  gload(0);
+  sstore();
+  __syncthreads();
+  for (long k0 = 0; k0 < k; k0 += HBK_) {
+    const bool more = k0 + HBK_ < k;
+    if (more) gload(k0 + HBK_);
+#pragma unroll
+    for (int kk0 = 0; kk0 < HBK_; kk0 += 4) {
+      const int kk = kk0 + (lane >> 4);
+      R ar[4], ai[4], br[NT], bi[NT];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        ar[t] = sAr[kk][wr * 64 + t * 16 + (lane & 15)];
+        ai[t] = sAi[kk][wr * 64 + t * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        br[t] = sBr[kk][wc * 16 * NT + t * 16 + (lane & 15)];
+        bi[t] = sBi[kk][wc * 16 * NT + t * 16 + (lane & 15)];
+      }
+      const R sg = ACC ? (R)-1 : (R)1;
+#pragma unroll
+      for (int ti = 0; ti < 4; ti++) {
+        const R a_r = sg * ar[ti], a_i = sg * ai[ti], na_i = -a_i;
+#pragma unroll
+        for (int tj = 0; tj < NT; tj++) {
+          accr[ti][tj] = CT<T>::mfma(a_r, br[tj], accr[ti][tj]);
+          accr[ti][tj] = CT<T>::mfma(na_i, bi[tj], accr[ti][tj]);
+          acci[ti][tj] = CT<T>::mfma(a_r, bi[tj], acci[ti][tj]);
+          acci[ti][tj] = CT<T>::mfma(a_i, br[tj], acci[ti][tj]);
+        }
+      }
+    }
+    __syncthreads();
+    if (more) {
+      sstore();
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+    for (int tj = 0; tj < NT; tj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        long gi = row0 + wr * 64 + ti * 16 + CT<T>::row(lane, r);
+        long gj = col0 + wc * 16 * NT + tj * 16 + (lane & 15);
+        if (gi < m && gj < n) {
+          T c;
+          if (ACC) { c = C[gi + gj * ldc]; c.x += accr[ti][tj][r]; c.y += acci[ti][tj][r]; }
+          else { c.x = accr[ti][tj][r]; c.y = acci[ti][tj][r]; }
+          C[gi + gj * ldc] = c;
+        }
+      }
+}
+
 template <typename T, bool ACC>
 static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
                  hipStream_t s)
 {
   if (m <= 0 || n <= 0 || k <= 0) return;
-  dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
-  hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  // large tiles once there are enough of them to fill the chip
+  constexpr long BN = 32 * BigNT<T>::v;
+  long big_tiles = ((m + HBM_ - 1) / HBM_) * ((n + BN - 1) / BN);
+  // (measured slower than the 64x64 kernel on gfx950 at one wave per
+  // SIMD; kept for tuning, off by default)
+  if (kUseBigGemm && big_tiles >= 256 && k >= 64) {
+    dim3 grid((unsigned)((m + HBM_ - 1) / HBM_), (unsigned)((n + BN - 1) / BN));
+    hipLaunchKernelGGL((k_gemm_big<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  } else {
+    dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
+    hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  }
   prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
 }
 
@@ -175,11 +320,47 @@ template <> struct PanelRows<double2> { static const int n = 128; };
 template <> struct PanelRows<float2> { static const int n = 256; };
 
 struct PanelScratch {
+  unsigned *bar;   // grid barrier: [0] arrivals, [32] generation, [64] timeout flag (own cache lines)
   double *candv;   // [2][Gmax]
   int *candi;      // [2][Gmax]
   void *candrow;   // [2][Gmax][PW] T
   void *rowc;      // [2][PW] T
+  int *plan;       // [2*PW+1]: dst rows, src rows, count of the panel's net row permutation
 };
+
+// Grid barrier among the co-resident workgroups of a cooperative launch:
+// release fence, one relaxed arrival counter, relaxed polling of a
+// generation word, acquire fence.  (Polling with acquire loads would
+// invalidate the L2 on every poll; this costs ~4 us at 120 workgroups
+// where the generic grid barrier costs ~17 us, measured on gfx950.)
+// Every wait is bounded: past GBAR_SPIN polls the timeout flag is set,
+// and once it is set no workgroup waits any more, so the kernel always
+// finishes; the host turns the flag into an error.
+#define GBAR_SPIN (1u << 20)
+__device__ inline void gbar(unsigned *bar, int G, unsigned &gen)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned *cnt = bar, *gw = bar + 32, *err = bar + 64;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      unsigned o = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (o == (unsigned)G - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      unsigned spins = 0;
+      while (__hip_atomic_load(gw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (++spins > GBAR_SPIN) { __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+  }
+  gen++;
+  __syncthreads();
+}
 
 template <typename T>
 __global__ void __launch_bounds__(PanelRows<T>::n)
@@ -187,11 +368,11 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
 {
   typedef decltype(T().x) R;
   constexpr int NR = PanelRows<T>::n;
-  cg::grid_group grid = cg::this_grid();
   __shared__ T slab[PW][NR];           // column-major: slab[q][t] = A(lo + t, r0 + q)
   __shared__ R wv[NR / 64];
   __shared__ int wi[NR / 64];
   __shared__ T spiv[PW], sold[PW];
+  __shared__ long spv[PW];
   __shared__ int s_best, s_p;
   __shared__ R s_bv;
   const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
@@ -199,8 +380,19 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
   const int nrow = (int)std::min<long>(NR, r0 + m - lo);
   const long i = lo + t;                 // this thread's row
   T *candrow = (T *)S.candrow, *rowc = (T *)S.rowc;
-  for (int q = 0; q < w; q++)
-    if (t < nrow) slab[q][t] = A[i + (r0 + q) * lda];
+  // every workgroup reads the generation before any can complete a barrier
+  unsigned gen = __hip_atomic_load(S.bar + 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t < nrow) {
+    T v[PW];
+    if (w == PW) {
+#pragma unroll
+      for (int q = 0; q < PW; q++) v[q] = A[i + (r0 + q) * lda];
+#pragma unroll
+      for (int q = 0; q < PW; q++) slab[q][t] = v[q];
+    } else {
+      for (int q = 0; q < w; q++) slab[q][t] = A[i + (r0 + q) * lda];
+    }
+  }
   __syncthreads();
   for (int j = 0; j < w; j++) {
     const long c = r0 + j;
@@ -228,7 +420,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
       if (s_best >= 0) candrow[((long)buf * Gmax + g) * PW + t] = slab[t][s_best];
       if (c >= lo && c < lo + nrow) rowc[buf * PW + t] = slab[t][c - lo];
     }
-    grid.sync();
+    gbar(S.bar, G, gen);
     // global pivot
     if (t < 64) {
       R b = -1; int p = -1, gw = -1;
@@ -250,6 +442,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
       sold[t] = rowc[buf * PW + t];
       spiv[t] = s_best >= 0 ? candrow[((long)buf * Gmax + s_best) * PW + t] : sold[t];
     }
+    if (t == 0) spv[j] = p;
     if (g == 0 && t == 0) {
       ipiv[c] = (int)(p + 1);
       if (s_bv == 0 && *info == 0) *info = (int)(c + 1);
@@ -271,6 +464,79 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
   }
   for (int q = 0; q < w; q++)
     if (t < nrow) A[i + (r0 + q) * lda] = slab[q][t];
+  // net permutation of the panel's w interchanges on the rows they touch,
+  // one row per lane of wave 0: lanes 0..w-1 are rows r0.., pivot rows
+  // from below are appended.  After the interchanges row dst[q] holds
+  // what row src[q] held before.
+  if (g == 0 && t < 64) {
+    long row = (t < w) ? r0 + t : -1;
+    int content = t, cnt = w;
+    for (int j = 0; j < w; j++) {
+      const long pj = spv[j];
+      if (pj == r0 + j) continue;
+      int ip;
+      if (pj < r0 + w) ip = (int)(pj - r0);
+      else {
+        unsigned long long hit = __ballot(row == pj);
+        if (hit) ip = __ffsll(hit) - 1;
+        else { ip = cnt; if (t == cnt) { row = pj; content = cnt; } cnt++; }
+      }
+      int cj = __shfl(content, j), cp = __shfl(content, ip);
+      if (t == j) content = cp;
+      if (t == ip) content = cj;
+    }
+    long srow = __shfl(row, content);
+    if (t < cnt) { S.plan[t] = (int)row; S.plan[PW * 2 + t] = (int)srow; }
+    if (t == 0) S.plan[PW * 4] = cnt;
+  }
+}
+
+// applies nplans consecutive panel row permutations (each: dst <- src on
+// at most 2 PW rows, stride 4 PW + 1 ints) to the columns [c0, c1) except
+// [s0, s1); one wave per column, four columns in flight per wave, plans in
+// order within a column
+#define PLAN_LEN (4 * PW + 1)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_laswp_plans(T *A, long lda, long c0, long c1, long s0, long s1, const int *plans, int nplans)
+{
+  const int lane = threadIdx.x & 63;
+  const long skip = s1 - s0;
+  const long ncol = (c1 - c0) - skip;
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  for (long c4 = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 4; c4 < ncol; c4 += nw * 4) {
+    long cc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      long c = c0 + c4 + u;
+      cc[u] = c < s0 ? c : c + skip;
+    }
+    for (int pl = 0; pl < nplans; pl++) {
+      const int *plan = plans + (long)pl * PLAN_LEN;
+      const int cnt = plan[PW * 4];
+      const long dst = lane < cnt ? plan[lane] : 0, src = lane < cnt ? plan[PW * 2 + lane] : 0;
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (c4 + u < ncol && lane < cnt) v[u] = A[src + cc[u] * lda];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (c4 + u < ncol && lane < cnt) A[dst + cc[u] * lda] = v[u];
+      // the next plan's loads see these stores
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+  }
+}
+
+template <typename T>
+static void laswp_plans(T *A, long lda, long c0, long c1, long s0, long s1, const int *plans, int nplans,
+                        hipStream_t st)
+{
+  long ncol = (c1 - c0) - (s1 - s0);
+  if (ncol <= 0 || nplans <= 0) return;
+  long nwaves = (ncol + 3) / 4;
+  unsigned blocks = (unsigned)std::min<long>(4096, (nwaves + 3) / 4);
+  hipLaunchKernelGGL(k_laswp_plans<T>, dim3(blocks), dim3(256), 0, st, A, lda, c0, c1, s0, s1, plans, nplans);
 }
 
 // row interchanges rows [k1,k2) (absolute, ipiv 1-based) on columns [c0,c1)
@@ -293,23 +559,58 @@ static void laswp(T *A, long lda, long c0, long c1, long k1, long k2, const int 
 }
 
 // B[w x ncol] <- L^{-1} B for the unit-lower w x w block of the panel
-// (w <= PW): L staged in LDS, each thread carries one column in registers
+// (w <= PW): L and a 128-column tile of B staged in LDS with coalesced
+// loads; each thread then carries one column in registers
+template <typename T> struct TsmCols;
+template <> struct TsmCols<double2> { static const int v = 64; };
+template <> struct TsmCols<float2> { static const int v = 128; };
+
 template <typename T>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(TsmCols<T>::v)
 k_trsm_small(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol)
 {
+  constexpr int TSC = TsmCols<T>::v;
   __shared__ T L[PW][PW + 1];
-  for (int e = threadIdx.x; e < PW * PW; e += blockDim.x) {
-    int r = e % PW, q = e / PW;
-    if (r < w && q < w) L[r][q] = A[(r0 + r) + (r0 + q) * lda];
+  __shared__ T Bt[PW][TSC + 1];
+  const int t = threadIdx.x;
+  const long c0 = (long)blockIdx.x * TSC;
+  {
+    // L and the B tile staged through registers in batches of 8 loads
+    constexpr int NL = PW * PW / TSC;
+    const bool full = (w == PW && c0 + TSC <= ncol);
+#pragma unroll
+    for (int u0 = 0; u0 < NL; u0 += 8) {
+      T lv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        int e = t + (u0 + u) * TSC, r = e % PW, q = e / PW;
+        if (full || (r < w && q < w)) lv[u] = A[(r0 + r) + (r0 + q) * lda];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        int e = t + (u0 + u) * TSC, r = e % PW, q = e / PW;
+        if (full || (r < w && q < w)) L[r][q] = lv[u];
+      }
+    }
+#pragma unroll
+    for (int u0 = 0; u0 < PW; u0 += 8) {
+      T bv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        int e = t + (u0 + u) * TSC, r = e % PW, cc = e / PW;
+        if (full || (r < w && c0 + cc < ncol)) bv[u] = B[r + (c0 + cc) * ldb];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        int e = t + (u0 + u) * TSC, r = e % PW, cc = e / PW;
+        if (full || (r < w && c0 + cc < ncol)) Bt[r][cc] = bv[u];
+      }
+    }
   }
   __syncthreads();
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  T *b = B + c * ldb;
   T x[PW];
 #pragma unroll
-  for (int r = 0; r < PW; r++) if (r < w) x[r] = b[r];
+  for (int r = 0; r < PW; r++) if (r < w) x[r] = Bt[r][t];
 #pragma unroll
   for (int q = 0; q < PW; q++) {
     if (q < w) {
@@ -319,7 +620,12 @@ k_trsm_small(const T *A, long lda, long r0, int w, T *B, long ldb, long ncol)
     }
   }
 #pragma unroll
-  for (int r = 0; r < PW; r++) if (r < w) b[r] = x[r];
+  for (int r = 0; r < PW; r++) if (r < w) Bt[r][t] = x[r];
+  __syncthreads();
+  for (int e = t; e < PW * TSC; e += TSC) {
+    int r = e % PW, cc = e / PW;
+    if (r < w && c0 + cc < ncol) B[r + (c0 + cc) * ldb] = Bt[r][cc];
+  }
 }
 
 template <typename T>
@@ -327,7 +633,8 @@ static void trsm_small(const T *A, long lda, long r0, int w, T *B, long ldb, lon
 {
   if (ncol <= 0 || w <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
-  hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, s, A, lda, r0, w, B, ldb, ncol);
+  constexpr int TSC = TsmCols<T>::v;
+  hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((ncol + TSC - 1) / TSC)), dim3(TSC), 0, s, A, lda, r0, w, B, ldb, ncol);
   prof_end(s, tok, 4.0 * (double)w * (double)w * (double)ncol);
 }
 
@@ -346,12 +653,33 @@ static void trsm_block(const T *A, long lda, long r0, int nb, T *B, long ldb, lo
   }
 }
 
+// second stream of the lookahead (default priority, non-blocking), one per
+// device, created on first use
+static hipStream_t side_stream()
+{
+  static hipStream_t st[64] = {nullptr};
+  int d = 0;
+  hipGetDevice(&d);
+  if (!st[d & 63]) hipStreamCreateWithFlags(&st[d & 63], hipStreamNonBlocking);
+  return st[d & 63];
+}
+
+// Right-looking blocked LU with one block of lookahead.  Block b (nb
+// columns from K) is factored in 32-column panels on the caller's stream
+// (panel, its row interchanges inside the block, small TRSM and GEMM for
+// the block's remaining columns); the next block's columns are then
+// updated on the same stream, while the rest of the trailing matrix is
+// updated on the side stream.  The next block's panels therefore run
+// while the big GEMM of this block does.  Row interchanges on columns
+// left of a block are applied at the end (nothing reads those rows in
+// between), so the side stream's GEMM never sees its L21 rows move.
 template <typename T>
 int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb)
 {
   if (n < 0 || lda < std::max(1L, n)) return -1;
   if (n == 0) return 0;
   if (nb > 256) nb = 256;
+  if (nb % PW) nb = 256;
   const int ib = PW;
   constexpr int NR = PanelRows<T>::n;
   const long Gmax = (n + NR - 1) / NR;
@@ -360,54 +688,105 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_panel<T>, NR, 0);
   if ((long)std::max(1, ncu) * std::max(1, perCU) < Gmax) return -5;   // panel would not be co-resident
+  const long npanels = (n + PW - 1) / PW;
+  // panel scratch, carved from one workspace: barrier words (3 cache
+  // lines), info, all panel plans, candidate values / indices / rows,
+  // diagonal rows
   PanelScratch S;
-  int *dinfo = nullptr;
-  if (hipMalloc(&S.candv, sizeof(double) * 2 * Gmax) != hipSuccess ||
-      hipMalloc(&S.candi, sizeof(int) * 2 * Gmax) != hipSuccess ||
-      hipMalloc(&S.candrow, sizeof(T) * 2 * Gmax * PW) != hipSuccess ||
-      hipMalloc(&S.rowc, sizeof(T) * 2 * PW) != hipSuccess || hipMalloc(&dinfo, sizeof(int)) != hipSuccess)
-    return -2;
+  size_t off_info = 96 * sizeof(unsigned), off_plan = off_info + 64;
+  size_t off_cv = off_plan + 256 * ((sizeof(int) * PLAN_LEN * npanels + 255) / 256);
+  size_t off_ci = off_cv + 256 * ((sizeof(double) * 2 * Gmax + 255) / 256);
+  size_t off_cr = off_ci + 256 * ((sizeof(int) * 2 * Gmax + 255) / 256);
+  size_t off_rc = off_cr + 256 * ((sizeof(T) * 2 * Gmax * PW + 255) / 256);
+  size_t total = off_rc + sizeof(T) * 2 * PW;
+  char *wsb = (char *)ws_get(WS_PANEL, total);
+  if (!wsb) return -2;
+  S.bar = (unsigned *)wsb;
+  int *dinfo = (int *)(wsb + off_info);
+  int *plans = (int *)(wsb + off_plan);
+  S.candv = (double *)(wsb + off_cv);
+  S.candi = (int *)(wsb + off_ci);
+  S.candrow = (void *)(wsb + off_cr);
+  S.rowc = (void *)(wsb + off_rc);
+  hipStream_t side = side_stream();
+  hipEvent_t evP = nullptr, evR = nullptr;
+  hipEventCreateWithFlags(&evP, hipEventDisableTiming);
+  hipEventCreateWithFlags(&evR, hipEventDisableTiming);
   hipMemsetAsync(dinfo, 0, sizeof(int), stream);
+  hipMemsetAsync(S.bar, 0, sizeof(unsigned) * 96, stream);
+  // the side stream starts after the caller's stream has the matrix ready
+  hipEventRecord(evR, stream);
+  hipStreamWaitEvent(side, evR, 0);
   int rc = 0;
   int gmaxi = (int)Gmax;
-  for (long k0 = 0; k0 < n && rc == 0; k0 += nb) {
-    long jb = std::min<long>(nb, n - k0);
-    for (long j0 = k0; j0 < k0 + jb; j0 += ib) {
-      int w = (int)std::min<long>(ib, k0 + jb - j0);
+  for (long K = 0; K < n && rc == 0; K += nb) {
+    const long jb = std::min<long>(nb, n - K);
+    const long p0 = K / PW;                 // first panel of the block
+    for (long j0 = K; j0 < K + jb; j0 += ib) {
+      int w = (int)std::min<long>(ib, K + jb - j0);
       long m = n - j0;
       long G = (m + NR - 1) / NR;
       long r0 = j0;
+      S.plan = plans + (j0 / PW) * PLAN_LEN;
       void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
                       (void *)&dinfo, (void *)&S, (void *)&gmaxi};
       int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
       hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3((unsigned)G), dim3(NR), args, 0, stream);
       prof_end(stream, tok, 0.0);
       if (e != hipSuccess) { rc = -3; break; }
-      laswp(A, lda, k0, j0, j0, j0 + w, d_ipiv, stream);
-      laswp(A, lda, j0 + w, k0 + jb, j0, j0 + w, d_ipiv, stream);
-      long nr = k0 + jb - (j0 + w);
+      // this panel's interchanges on the block's other columns
+      laswp_plans(A, lda, K, K + jb, j0, j0 + w, S.plan, 1, stream);
+      long nr = K + jb - (j0 + w);
       if (nr > 0) {
         trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
         gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
                  A + (j0 + w) + (j0 + w) * lda, lda, stream);
       }
     }
-    laswp(A, lda, 0, k0, k0, k0 + jb, d_ipiv, stream);
-    laswp(A, lda, k0 + jb, n, k0, k0 + jb, d_ipiv, stream);
-    long nr = n - k0 - jb;
-    if (nr > 0) {
-      trsm_block(A, lda, k0, (int)jb, A + k0 + (k0 + jb) * lda, lda, nr, stream);
-      gemm_sub(nr, nr, jb, A + (k0 + jb) + k0 * lda, lda, A + k0 + (k0 + jb) * lda, lda,
-               A + (k0 + jb) + (k0 + jb) * lda, lda, stream);
+    if (rc) break;
+    const int bplans = (int)((jb + PW - 1) / PW);
+    const long c1 = K + jb;                               // first column right of the block
+    const long nnext = std::min<long>(nb, n - c1), c2 = c1 + nnext;
+    hipEventRecord(evP, stream);
+    // the next block's columns (caller's stream), after the side stream
+    // has finished the previous block's update of them
+    if (nnext > 0) {
+      hipStreamWaitEvent(stream, evR, 0);
+      laswp_plans(A, lda, c1, c2, 0, 0, plans + p0 * PLAN_LEN, bplans, stream);
+      trsm_block(A, lda, K, (int)jb, A + K + c1 * lda, lda, nnext, stream);
+      gemm_sub(n - c1, nnext, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, stream);
+    }
+    // the rest of the trailing matrix (side stream)
+    if (c2 < n) {
+      hipStreamWaitEvent(side, evP, 0);
+      laswp_plans(A, lda, c2, n, 0, 0, plans + p0 * PLAN_LEN, bplans, side);
+      trsm_block(A, lda, K, (int)jb, A + K + c2 * lda, lda, n - c2, side);
+      gemm_sub(n - c1, n - c2, jb, A + c1 + K * lda, lda, A + K + c2 * lda, lda, A + c1 + c2 * lda, lda, side);
+      hipEventRecord(evR, side);
     }
   }
+  // join the side stream, then the deferred interchanges on the columns
+  // left of each block
+  hipEventRecord(evR, side);
+  hipStreamWaitEvent(stream, evR, 0);
+  if (rc == 0)
+    for (long K = nb; K < n; K += nb) {
+      const long jb = std::min<long>(nb, n - K);
+      laswp_plans(A, lda, 0, K, 0, 0, plans + (K / PW) * PLAN_LEN, (int)((jb + PW - 1) / PW), stream);
+    }
+  unsigned tmo = 0;
   hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream);
+  hipMemcpyAsync(&tmo, S.bar + 64, sizeof(unsigned), hipMemcpyDeviceToHost, stream);
   hipStreamSynchronize(stream);
+  hipStreamSynchronize(side);
+  hipEventDestroy(evP);
+  hipEventDestroy(evR);
   if (prof_enabled()) prof_collect();
-  hipFree(S.candv); hipFree(S.candi); hipFree(S.candrow); hipFree(S.rowc); hipFree(dinfo);
+  if (tmo && rc == 0) rc = -6;
   if (rc == 0 && hipGetLastError() != hipSuccess) rc = -3;
   return rc;
 }
+
 // ---------------------------------------------------------------------
 // triangular solves.  Blocked for trans='N': a diagonal block is solved
 // by one workgroup per right-hand side, the rest updated by the GEMM.
@@ -482,49 +861,80 @@ __global__ void k_laswp_rev(T *B, long ldb, long n, long nrhs, const int *ipiv)
   }
 }
 
-// One 64-row block step of a triangular solve with few right-hand sides
-// (blockIdx.y = right-hand side).  Every workgroup stages the diagonal
-// block in LDS and solves it redundantly with one wave (shuffles, no
-// barriers), then updates its own 256 rows outside the block with the
-// block's solution (coalesced column reads).  The solution segment is
-// written to Xout, never to the vector read in the same launch.
-//   LOWER: unit-lower L, rows below the block are updated in Bin.
-//  !LOWER: upper U with its diagonal, rows above the block are updated.
+// Triangular solves with few right-hand sides, in 64-row block steps.
+// The diagonal 64x64 blocks of L (unit lower) and U (upper) are inverted
+// first, all blocks at once (k_diag_inv), so every block step is a small
+// matrix-vector product instead of a 64-step serial substitution.
 #define TB 64
 template <typename T, bool LOWER>
-__global__ void __launch_bounds__(256)
-k_trsv_step(const T *LU, long lda, long n, long kb, int bs, T *Bin, long ldb, T *Xout, long ldx)
+__global__ void __launch_bounds__(TB)
+k_diag_inv(const T *LU, long lda, long n, T *inv)
 {
   __shared__ T D[TB][TB + 1];
-  __shared__ T xs[TB];
+  __shared__ T X[TB][TB + 1];       // X[r][j]: column j of the inverse
+  const int t = threadIdx.x;
+  const long kb = (long)blockIdx.x * TB;
+  const int bs = (int)std::min<long>(TB, n - kb);
+  for (int q = 0; q < TB; q++) {
+    T v;
+    if (t < bs && q < bs) v = LU[(kb + t) + (kb + q) * lda];
+    else { v.x = (t == q) ? 1 : 0; v.y = 0; }
+    D[t][q] = v;
+  }
+  __syncthreads();
+  const int j = t;
+  if (LOWER) {
+    for (int r = 0; r < j; r++) { X[r][j].x = 0; X[r][j].y = 0; }
+    X[j][j].x = 1; X[j][j].y = 0;
+    for (int r = j + 1; r < TB; r++) {
+      T acc; acc.x = 0; acc.y = 0;
+      for (int k = j; k < r; k++) acc = csub(acc, cmul(D[r][k], X[k][j]));
+      X[r][j] = acc;
+    }
+  } else {
+    for (int r = j + 1; r < TB; r++) { X[r][j].x = 0; X[r][j].y = 0; }
+    X[j][j] = crecip(D[j][j]);
+    for (int r = j - 1; r >= 0; r--) {
+      T acc; acc.x = 0; acc.y = 0;
+      for (int k = r + 1; k <= j; k++) acc = csub(acc, cmul(D[r][k], X[k][j]));
+      X[r][j] = cdiv(acc, D[r][r]);
+    }
+  }
+  __syncthreads();
+  T *o = inv + (long)blockIdx.x * TB * TB;
+  for (int q = 0; q < TB; q++) o[t + q * TB] = X[t][q];
+}
+
+// one block step: x_b = inv(D_b) r_b (every workgroup, from L2), the
+// solution written to Xout by workgroup 0, and the rows outside the block
+// updated with x_b (LOWER: rows below, else rows above); blockIdx.y = rhs
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256)
+k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin, long ldb, T *Xout, long ldx)
+{
+  __shared__ T rs[TB], xs[TB];
   const int t = threadIdx.x;
   T *b = Bin + (long)blockIdx.y * ldb;
   T *xo = Xout + (long)blockIdx.y * ldx;
-  for (int e = t; e < TB * TB; e += 256) {
-    int r = e % TB, q = e / TB;
-    if (r < bs && q < bs) D[r][q] = LU[(kb + r) + (kb + q) * lda];
+  if (t < TB) {
+    if (t < bs) rs[t] = b[kb + t]; else { rs[t].x = 0; rs[t].y = 0; }
   }
-  if (t < bs) xs[t] = b[kb + t];
   __syncthreads();
-  if (t < 64) {
-    T x;
-    x.x = 0; x.y = 0;
-    if (t < bs) x = xs[t];
-    if (LOWER) {
-      for (int i = 0; i < bs; i++) {
-        T xi;
-        xi.x = __shfl(x.x, i); xi.y = __shfl(x.y, i);
-        if (t > i && t < bs) x = csub(x, cmul(D[t][i], xi));
-      }
-    } else {
-      for (int i = bs - 1; i >= 0; i--) {
-        if (t == i) x = cdiv(x, D[i][i]);
-        T xi;
-        xi.x = __shfl(x.x, i); xi.y = __shfl(x.y, i);
-        if (t < i) x = csub(x, cmul(D[t][i], xi));
+  if (t < TB) {
+    T acc; acc.x = 0; acc.y = 0;
+#pragma unroll
+    for (int k0 = 0; k0 < TB; k0 += 16) {
+      T a[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) a[u] = invb[t + (k0 + u) * TB];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        T v = rs[k0 + u];
+        acc.x += a[u].x * v.x - a[u].y * v.y;
+        acc.y += a[u].x * v.y + a[u].y * v.x;
       }
     }
-    if (t < bs) xs[t] = x;
+    xs[t] = acc;
   }
   __syncthreads();
   if (blockIdx.x == 0 && t < bs) xo[kb + t] = xs[t];
@@ -533,12 +943,30 @@ k_trsv_step(const T *LU, long lda, long n, long kb, int bs, T *Bin, long ldb, T 
   if (row < rend) {
     T acc;
     acc.x = 0; acc.y = 0;
-    for (int i = 0; i < bs; i++) {
-      T a = LU[row + (kb + i) * lda], v = xs[i];
-      acc.x += a.x * v.x - a.y * v.y;
-      acc.y += a.x * v.y + a.y * v.x;
+    const T *Lr = LU + row + kb * lda;
+    T bo = b[row];
+    if (bs == TB) {
+      // full block: 16 independent loads in flight per batch
+#pragma unroll
+      for (int i0 = 0; i0 < TB; i0 += 16) {
+        T a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = Lr[(long)(i0 + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          T v = xs[i0 + u];
+          acc.x += a[u].x * v.x - a[u].y * v.y;
+          acc.y += a[u].x * v.y + a[u].y * v.x;
+        }
+      }
+    } else {
+      for (int i = 0; i < bs; i++) {
+        T a = Lr[(long)i * lda], v = xs[i];
+        acc.x += a.x * v.x - a.y * v.y;
+        acc.y += a.x * v.y + a.y * v.x;
+      }
     }
-    b[row] = csub(b[row], acc);
+    b[row] = csub(bo, acc);
   }
 }
 
@@ -549,27 +977,31 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
   if (n < 0 || nrhs < 0 || lda < std::max(1L, n) || ldb < std::max(1L, n)) return -1;
   if (n == 0 || nrhs == 0) return 0;
   if ((trans == 'N' || trans == 'n') && nrhs <= 16) {
-    T *W = nullptr;
-    if (hipMalloc(&W, sizeof(T) * n * nrhs) != hipSuccess) return -2;
+    const long nblk = (n + TB - 1) / TB;
+    T *W = (T *)ws_get(WS_SOLVE_W, sizeof(T) * n * nrhs);
+    T *invL = (T *)ws_get(WS_SOLVE_INVL, sizeof(T) * nblk * TB * TB);
+    T *invU = (T *)ws_get(WS_SOLVE_INVU, sizeof(T) * nblk * TB * TB);
+    if (!W || !invL || !invU) return -2;
+    hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL);
+    hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU);
     laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);
     // L y = P b: y accumulates in W, B holds the running right-hand side
     for (long kb = 0; kb < n; kb += TB) {
       int bs = (int)std::min<long>(TB, n - kb);
       long rows = n - kb - bs;
       dim3 grid((unsigned)std::max<long>(1, (rows + 255) / 256), (unsigned)nrhs);
-      hipLaunchKernelGGL((k_trsv_step<T, true>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs, B, ldb, W, n);
+      hipLaunchKernelGGL((k_trsv_inv<T, true>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs,
+                         invL + (kb / TB) * TB * TB, B, ldb, W, n);
     }
     // U x = y: W is updated in place, the solution lands in B
     long last = ((n - 1) / TB) * TB;
     for (long kb = last; kb >= 0; kb -= TB) {
       int bs = (int)std::min<long>(TB, n - kb);
       dim3 grid((unsigned)std::max<long>(1, (kb + 255) / 256), (unsigned)nrhs);
-      hipLaunchKernelGGL((k_trsv_step<T, false>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs, W, n, B, ldb);
+      hipLaunchKernelGGL((k_trsv_inv<T, false>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs,
+                         invU + (kb / TB) * TB * TB, W, n, B, ldb);
     }
-    hipError_t e = hipGetLastError();
-    hipStreamSynchronize(stream);
-    hipFree(W);
-    return e == hipSuccess ? 0 : -3;
+    return hipGetLastError() == hipSuccess ? 0 : -3;
   }
   if (trans == 'N' || trans == 'n') {
     laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);

@@ -60,6 +60,37 @@ def test_lu32_backward_error(sx, O, n):
     assert np.mean(piv == ipiv) > 0.95
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_lu_large_uses_big_tiles(sx, dtype):
+    """n = 2600: the trailing updates run on the large-tile GEMM; factors
+    and pivots against LAPACK zgetrf (scipy) for complex128, backward error
+    for complex64."""
+    import scipy.linalg.lapack as lapack
+    n = 2600
+    print(f"lu large {dtype}", flush=True)
+    A = rand(n, SEED + 11)
+    if dtype == "complex64":
+        A = A.astype(np.complex64)
+    M, info = gpu_lu(sx, A, dtype=dtype)
+    assert info == 0
+    F = M.numpy().astype(np.complex128)
+    piv = M.ipiv.cpu().numpy()
+    if dtype == "complex128":
+        lu2, piv2, _ = lapack.zgetrf(A)
+        assert np.array_equal(piv, piv2 + 1)
+        assert np.max(np.abs(F - lu2)) <= 1e-11 * np.max(np.abs(lu2))
+    else:
+        L = np.tril(F, -1) + np.eye(n)
+        U = np.triu(F)
+        PA = A.astype(np.complex128)[:, :]
+        perm = np.arange(n)
+        for i in range(n):
+            p = piv[i] - 1
+            perm[[i, p]] = perm[[p, i]]
+        berr = np.linalg.norm(PA[perm] - L @ U) / np.linalg.norm(A)
+        assert berr <= 3e-5, berr
+
+
 def test_lu_singular_info(sx, O):
     A = rand(70, 5)
     A[:, 40] = 0
