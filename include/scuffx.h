@@ -57,6 +57,9 @@ int scuffx_geom_free(scuffx_geom *G);
 int scuffx_geom_total_bfs(const scuffx_geom *G);
 int scuffx_geom_num_surfaces(const scuffx_geom *G);
 int scuffx_geom_counts(const scuffx_geom *G, int ns, int *nV, int *nP, int *nE);
+/* first matrix row of surface ns and its number of basis functions
+   (RWGGeometry::BFIndexOffset, RWGSurface::NumBFs, RWGGeometry.cc:708-724) */
+int scuffx_geom_bf_range(const scuffx_geom *G, int ns, int *offset, int *nbf);
 /* per edge: iV1 iV2 iQP iQM iPPanel iMPanel PIndex MIndex Index (libscuff.h:120-146) */
 int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out9, double *lengths);
 /* per panel: centroid[3] zhat[3] radius area (libscuff.h:100-112) */
@@ -74,6 +77,20 @@ int scuffx_fippi_records(scuffx_geom *G, double *out);
 /* ---- RWGGeometry::AssembleBEMMatrix(Omega, M) (AssembleBEMMatrix.cc:684) ----
    fills all N x N entries of the complex128 matrix at M_dev */
 int scuffx_assemble(scuffx_geom *G, double omega_re, double omega_im, void *M_dev, long ldm);
+/* ---- RWGGeometry::AssembleBEMMatrixBlock(nsa, nsb, Omega, 0, M, GradM)
+        (libscuff.h:500-505, AssembleBEMMatrix.cc:349-391) ----
+   block of two distinct surfaces: B (NBF_a x NBF_b, column-major, leading
+   dimension ldb) and, if GradB_dev is not NULL, its derivatives with respect
+   to a rigid displacement of surface nsa along x, y, z: three matrices at
+   GradB_dev + m*gstride (complex entries), leading dimension ldg; with
+   ntorque = 1..3 also the derivatives with respect to a rotation of surface
+   nsa about each axis (dMdT; gamma = 9*ntorque doubles, the rotation
+   generators Gamma[i + 3 j] of CreateGammaMatrix, EdgeEdgeInteractions.cc:241)
+   at dBdT_dev + t*gstride (SurfaceSurfaceInteractions.cc:212-327,
+   PanelPanelInteractions.cc:132-199) */
+int scuffx_assemble_block(scuffx_geom *G, int nsa, int nsb, double omega_re, double omega_im,
+                          void *B_dev, long ldb, void *GradB_dev, long ldg, long gstride,
+                          int ntorque, const double *gamma, void *dBdT_dev);
 /* timings of the last assembly in ms: fippi self far near stamp mirror;
    counts: far pairs, near pairs, self jobs, fippi records */
 int scuffx_assemble_stats(const scuffx_geom *G, double *ms6, long *counts4);
@@ -110,7 +127,7 @@ int scuffx_power(scuffx_geom *G, double omega_re, double omega_im, const void *K
 /* ---- measurement (no counterpart in the reference) ----
    on != 0: every launch of the hot kernels is bracketed by HIP events on
    its stream; totals are reset.  Kernel names: gemm_c128 gemm_c64
-   panel_c128 panel_c64 trsm_c128 trsm_c64 far near self stamp fippi resid.
+   panel_c128 panel_c64 trsm_c128 trsm_c64 far near self stamp fippi resid grad.
    work = algorithmic flops (gemm, trsm, far, near) or bytes (stamp, resid). */
 int scuffx_profile(int on);
 int scuffx_profile_read(const char *kernel, double *ms, double *work, long *launches);

@@ -207,3 +207,14 @@ int o_api_tcr(int order, double *out)
   if (out) memcpy(out, r, 3*sizeof(double)*n);
   return n;
 }
+
+/* block of two distinct surfaces and its displacement gradients */
+int o_api_assemble_block_grad(void *p, double wr, double wi, int nsa, int nsb, double *B, int ldb, double *GradB,
+                              int ntorque, const double *Gamma, double *dBdT)
+{
+  oh *h = (oh *)p;
+  return o_assemble_block_grad(h->G, wr + I*wi, h->FC, nsa, nsb, (zdouble *)B, ldb, (zdouble *)GradB,
+                               ntorque, Gamma, (zdouble *)dBdT);
+}
+
+int o_api_num_bfs(void *p, int ns) { return ((oh *)p)->G->Surfaces[ns]->NumBFs; }

@@ -17,33 +17,61 @@
 
 #include "oracle.h"
 
-void o_get_eeis(OSurface *Sa, int nea, OSurface *Sb, int neb, zdouble k,
-                OFIPPICache *FC, zdouble GC[2])
+/* GetEdgeEdgeInteractions (EdgeEdgeInteractions.cc:50-217): the four
+   panel-pair integrals of two RWG functions, optionally with the three
+   displacement gradients (:183-198) */
+void o_get_eeis_grad(OSurface *Sa, int nea, OSurface *Sb, int neb, zdouble k,
+                     OFIPPICache *FC, zdouble GC[2], zdouble *GradGC,
+                     int ntorque, const double *Gamma, zdouble *dGCdT)
 {
-  if (creal(k) == 0.0 && cimag(k) == 0.0) { GC[0] = GC[1] = 0.0; return; }
+  if (!dGCdT || !Gamma) ntorque = 0;
+  if (creal(k) == 0.0 && cimag(k) == 0.0) {
+    GC[0] = GC[1] = 0.0;
+    if (GradGC) for (int m = 0; m < 6; m++) GradGC[m] = 0.0;
+    for (int m = 0; m < 2*ntorque; m++) dGCdT[m] = 0.0;
+    return;
+  }
   OEdge *Ea = Sa->Edges[nea], *Eb = Sb->Edges[neb];
-  zdouble HPP[2] = {0, 0}, HPM[2] = {0, 0}, HMP[2] = {0, 0}, HMM[2] = {0, 0};
+  zdouble H[4][2], GH[4][6], TH[4][6];
+  memset(H, 0, sizeof(H));
+  memset(GH, 0, sizeof(GH));
+  memset(TH, 0, sizeof(TH));
   OPPIArgs A;
   o_ppi_init(&A);
   A.Sa = Sa; A.Sb = Sb; A.k = k; A.FC = FC;
-  A.npa = Ea->iPPanel; A.iQa = Ea->PIndex; A.npb = Eb->iPPanel; A.iQb = Eb->PIndex;
-  o_get_ppis(&A); HPP[0] = A.H[0]; HPP[1] = A.H[1];
-  if (Eb->iMPanel != -1) {
-    A.npa = Ea->iPPanel; A.iQa = Ea->PIndex; A.npb = Eb->iMPanel; A.iQb = Eb->MIndex;
-    o_get_ppis(&A); HPM[0] = A.H[0]; HPM[1] = A.H[1];
-  }
-  if (Ea->iMPanel != -1) {
-    A.npa = Ea->iMPanel; A.iQa = Ea->MIndex; A.npb = Eb->iPPanel; A.iQb = Eb->PIndex;
-    o_get_ppis(&A); HMP[0] = A.H[0]; HMP[1] = A.H[1];
-  }
-  if (Ea->iMPanel != -1 && Eb->iMPanel != -1) {
-    A.npa = Ea->iMPanel; A.iQa = Ea->MIndex; A.npb = Eb->iMPanel; A.iQb = Eb->MIndex;
-    o_get_ppis(&A); HMM[0] = A.H[0]; HMM[1] = A.H[1];
-  }
+  A.NumGradientComponents = GradGC ? 3 : 0;
+  A.NumTorqueAxes = ntorque;
+  A.GammaMatrix = Gamma;
+  int pa[2] = {Ea->iPPanel, Ea->iMPanel}, qa[2] = {Ea->PIndex, Ea->MIndex};
+  int pb[2] = {Eb->iPPanel, Eb->iMPanel}, qb[2] = {Eb->PIndex, Eb->MIndex};
+  for (int s = 0; s < 2; s++)
+    for (int t = 0; t < 2; t++) {
+      if (pa[s] == -1 || pb[t] == -1) continue;
+      A.npa = pa[s]; A.iQa = qa[s]; A.npb = pb[t]; A.iQb = qb[t];
+      o_get_ppis(&A);
+      H[2*s+t][0] = A.H[0]; H[2*s+t][1] = A.H[1];
+      if (GradGC) memcpy(GH[2*s+t], A.GradH, sizeof(GH[0]));
+      if (ntorque) memcpy(TH[2*s+t], A.dHdT, sizeof(TH[0]));
+    }
   double GPreFac = Ea->Length*Eb->Length;
   zdouble CPreFac = Ea->Length*Eb->Length/(I*k);
-  GC[0] = GPreFac*(HPP[0] - HPM[0] - HMP[0] + HMM[0]);
-  GC[1] = CPreFac*(HPP[1] - HPM[1] - HMP[1] + HMM[1]);
+  GC[0] = GPreFac*(H[0][0] - H[1][0] - H[2][0] + H[3][0]);
+  GC[1] = CPreFac*(H[0][1] - H[1][1] - H[2][1] + H[3][1]);
+  if (GradGC)
+    for (int m = 0; m < 3; m++) {
+      GradGC[2*m+0] = GPreFac*(GH[0][2*m+0] - GH[1][2*m+0] - GH[2][2*m+0] + GH[3][2*m+0]);
+      GradGC[2*m+1] = CPreFac*(GH[0][2*m+1] - GH[1][2*m+1] - GH[2][2*m+1] + GH[3][2*m+1]);
+    }
+  for (int m = 0; m < ntorque; m++) {
+    dGCdT[2*m+0] = GPreFac*(TH[0][2*m+0] - TH[1][2*m+0] - TH[2][2*m+0] + TH[3][2*m+0]);
+    dGCdT[2*m+1] = CPreFac*(TH[0][2*m+1] - TH[1][2*m+1] - TH[2][2*m+1] + TH[3][2*m+1]);
+  }
+}
+
+void o_get_eeis(OSurface *Sa, int nea, OSurface *Sb, int neb, zdouble k,
+                OFIPPICache *FC, zdouble GC[2])
+{
+  o_get_eeis_grad(Sa, nea, Sb, neb, k, FC, GC, NULL, 0, NULL, NULL);
 }
 
 static int count_common_regions(OSurface *Sa, OSurface *Sb, int CR[2], double Signs[2])
@@ -139,5 +167,54 @@ int o_assemble_bem_rows(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M, in
 {
   o_update_epsmu(G, Omega);
   ssi_block(G, 0, 0, Omega, FC, M, ldm, e0, e1, 0, NULL);
+  return 0;
+}
+
+/* AssembleBEMMatrixBlock(nsa, nsb, Omega, 0, B, GradB) for two distinct
+   surfaces (SurfaceSurfaceInteractions.cc:195-327 with GradB): the block
+   B (NBF_a x NBF_b, column-major, ld ldb, overwritten) and its three
+   displacement derivatives GradB + m*ldb*NBF_b, m = x, y, z */
+int o_assemble_block_grad(OGeom *G, zdouble Omega, OFIPPICache *FC, int nsa, int nsb,
+                          zdouble *B, int ldb, zdouble *GradB,
+                          int ntorque, const double *Gamma, zdouble *dBdT)
+{
+  if (!Gamma || !dBdT) ntorque = 0;
+  if (nsa == nsb || nsa < 0 || nsb < 0 || nsa >= G->NumSurfaces || nsb >= G->NumSurfaces) return -1;
+  o_update_epsmu(G, Omega);
+  OSurface *Sa = G->Surfaces[nsa], *Sb = G->Surfaces[nsb];
+  if (ldb < Sa->NumBFs) return -1;
+  size_t bsz = (size_t)ldb*Sb->NumBFs;
+  memset(B, 0, sizeof(zdouble)*bsz);
+  memset(GradB, 0, 3*sizeof(zdouble)*bsz);
+  if (ntorque) memset(dBdT, 0, ntorque*sizeof(zdouble)*bsz);
+  int CR[2]; double Signs[2];
+  int ncr = count_common_regions(Sa, Sb, CR, Signs);
+  for (int r = 0; r < ncr; r++) {
+    zdouble Eps = G->EpsTF[CR[r]], Mu = G->MuTF[CR[r]];
+    if (Eps == 0.0) continue;
+    zdouble k = csqrt2(Eps*Mu)*Omega;
+    zdouble PF1 = Signs[r]*I*Mu*Omega, PF2 = -Signs[r]*I*k, PF3 = -Signs[r]*I*Eps*Omega;
+    for (int nea = 0; nea < Sa->NumEdges; nea++)
+      for (int neb = 0; neb < Sb->NumEdges; neb++) {
+        zdouble GC[2], GG[6], GT[6];
+        o_get_eeis_grad(Sa, nea, Sb, neb, k, FC, GC, GG, ntorque, Gamma, GT);
+        int pecA = Sa->IsPEC, pecB = Sb->IsPEC;
+        int X = pecA ? nea : 2*nea, Y = pecB ? neb : 2*neb;
+#define BADD(M, x, y, v) (M)[(size_t)(x) + (size_t)(y)*ldb] += (v)
+        for (int m = -1; m < 3 + ntorque; m++) {
+          zdouble *T = (m < 0) ? B : (m < 3) ? GradB + (size_t)m*bsz : dBdT + (size_t)(m-3)*bsz;
+          zdouble g0 = (m < 0) ? GC[0] : (m < 3) ? GG[2*m] : GT[2*(m-3)];
+          zdouble g1 = (m < 0) ? GC[1] : (m < 3) ? GG[2*m+1] : GT[2*(m-3)+1];
+          if (pecA && pecB) BADD(T, X, Y, PF1*g0);
+          else if (pecA) { BADD(T, X, Y, PF1*g0); BADD(T, X, Y+1, PF2*g1); }
+          else if (pecB) { BADD(T, X, Y, PF1*g0); BADD(T, X+1, Y, PF2*g1); }
+          else {
+            BADD(T, X, Y, PF1*g0); BADD(T, X, Y+1, PF2*g1);
+            BADD(T, X+1, Y, PF2*g1); BADD(T, X+1, Y+1, PF3*g0);
+          }
+        }
+#undef BADD
+      }
+  }
   return 0;
 }

@@ -185,6 +185,9 @@ typedef struct {
   int WhichAlgorithm;
   zdouble H[2];
   zdouble GradH[6];
+  int NumTorqueAxes;          /* 0..3 */
+  const double *GammaMatrix;  /* 9 per axis, Gamma[i + 3 j] */
+  zdouble dHdT[6];
 } OPPIArgs;
 void o_ppi_init(OPPIArgs *A);
 void o_get_ppis(OPPIArgs *A);
@@ -208,6 +211,12 @@ int o_assemble_bem(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M,
 /* one edge-edge interaction (for tests) */
 void o_get_eeis(OSurface *Sa, int nea, OSurface *Sb, int neb, zdouble k,
                 OFIPPICache *FC, zdouble GC[2]);
+void o_get_eeis_grad(OSurface *Sa, int nea, OSurface *Sb, int neb, zdouble k,
+                     OFIPPICache *FC, zdouble GC[2], zdouble *GradGC,
+                     int ntorque, const double *Gamma, zdouble *dGCdT);
+int o_assemble_block_grad(OGeom *G, zdouble Omega, OFIPPICache *FC, int nsa, int nsb,
+                          zdouble *B, int ldb, zdouble *GradB,
+                          int ntorque, const double *Gamma, zdouble *dBdT);
 /* rows [r0,r1) of the upper triangle only (bounded CPU timing sample) */
 int o_assemble_bem_rows(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M,
                         int ldm, int e0, int e1);

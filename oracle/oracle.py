@@ -64,6 +64,10 @@ def lib():
     L.o_api_icosphere.argtypes = [C.c_int, C.c_double, C.c_void_p, _dp, _ip]
     L.o_api_write_gmsh.argtypes = [C.c_char_p, _dp, C.c_int, _ip, C.c_int]
     L.o_api_tcr.argtypes = [C.c_int, C.c_void_p]
+    L.o_api_assemble_block_grad.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                            C.c_void_p]
+    L.o_api_num_bfs.argtypes = [C.c_void_p, C.c_int]
     _CB = C.CFUNCTYPE(None, C.c_uint, C.POINTER(C.c_double), C.c_void_p, C.c_uint,
                       C.POINTER(C.c_double))
     L._integrand_type = _CB
@@ -210,6 +214,33 @@ class Geometry:
         omega = complex(omega)
         self.L.o_api_assemble_rows(self.h, omega.real, omega.imag, M, N, e0, e1)
         return M
+
+    def num_bfs(self, ns):
+        return self.L.o_api_num_bfs(self.h, ns)
+
+    def assemble_block_grad(self, omega, nsa, nsb, gammas=None):
+        """Block (nsa, nsb) of the BEM matrix for two distinct surfaces, its
+        derivatives with respect to a rigid displacement (x, y, z) of surface
+        nsa and, for each 3x3 rotation generator in `gammas`, the derivative
+        with respect to a rotation of surface nsa about that axis:
+        returns (B, [dB/dx, dB/dy, dB/dz], [dB/dtheta_1, ...])."""
+        na, nb = self.num_bfs(nsa), self.num_bfs(nsb)
+        B = np.zeros((na, nb), dtype=np.complex128, order="F")
+        GB = np.zeros((3, nb, na), dtype=np.complex128)   # each GB[m].T is column-major na x nb
+        nt = 0 if gammas is None else len(gammas)
+        TB = np.zeros((max(nt, 1), nb, na), dtype=np.complex128)
+        gam = np.zeros(9 * max(nt, 1))
+        for t in range(nt):
+            gam[9 * t:9 * t + 9] = np.asarray(gammas[t], dtype=np.float64).ravel(order="F")
+        omega = complex(omega)
+        r = self.L.o_api_assemble_block_grad(self.h, omega.real, omega.imag, nsa, nsb,
+                                             B.ctypes.data_as(C.c_void_p), na,
+                                             GB.ctypes.data_as(C.c_void_p), nt,
+                                             gam.ctypes.data_as(C.c_void_p),
+                                             TB.ctypes.data_as(C.c_void_p))
+        if r != 0:
+            raise RuntimeError("oracle: bad block")
+        return B, [GB[m].T.copy() for m in range(3)], [TB[t].T.copy() for t in range(nt)]
 
     def fippi_count(self):
         return self.L.o_api_fippi_count(self.h)

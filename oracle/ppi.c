@@ -39,9 +39,12 @@ static zdouble exprel(zdouble x, int n)
   return Sum;
 }
 
-/* AssembleInnerPPIIntegrand, non-periodic branch (:108-201) */
-static void inner_integrand(double wp, const double *R, const double *F, const double *FP,
-                            zdouble k, int DeSing, zdouble *HI, zdouble *GHI)
+/* AssembleInnerPPIIntegrand, non-periodic branch (:108-201), with the
+   displacement gradients (:176-180) and the angular derivatives about the
+   axes with rotation generators Gamma (:182-199) */
+static void inner_integrand(double wp, const double *R, const double *X, const double *F, const double *FP,
+                            zdouble k, int DeSing, zdouble *HI, zdouble *GHI,
+                            int ntq, const double *Gamma, zdouble *DTI)
 {
   zdouble ik = I*k, ik2 = ik*ik;
   zdouble hPlus = dot3(F, FP) + 4.0/ik2;
@@ -63,6 +66,18 @@ static void inner_integrand(double wp, const double *R, const double *F, const d
       GHI[2*m+0] += R[m]*hPlus*Psi;
       GHI[2*m+1] += R[m]*hTimes*Zeta + FxFP[m]*Psi;
     }
+  for (int t = 0; t < ntq; t++) {
+    double dX[3] = {0, 0, 0}, dF[3] = {0, 0, 0}, dFxFP[3];
+    for (int m = 0; m < 3; m++)
+      for (int n = 0; n < 3; n++) {
+        dX[m] += Gamma[9*t + m + 3*n]*X[n];
+        dF[m] += Gamma[9*t + m + 3*n]*F[n];
+      }
+    double Puv = dot3(R, dX);
+    cross3(dF, FP, dFxFP);
+    DTI[2*t+0] += hPlus*Puv*Psi + dot3(dF, FP)*Phi;
+    DTI[2*t+1] += hTimes*Puv*Zeta + (dot3(dFxFP, R) + dot3(FxFP, dX))*Psi;
+  }
 }
 
 /* GetPPIs_Cubature (:211-350) */
@@ -73,14 +88,15 @@ static void ppis_cubature(OPPIArgs *A, int DeSing, int HighOrder,
   sub3(Va[1], Va[0], Av); sub3(Va[2], Va[0], Bv);
   sub3(Vb[1], Vb[0], APv); sub3(Vb[2], Vb[0], BPv);
   int want_grad = A->NumGradientComponents > 0;
+  int ntq = (A->GammaMatrix && A->NumTorqueAxes > 0) ? A->NumTorqueAxes : 0;
   int np;
   const double *TCR = o_tcr(HighOrder ? 20 : 4, &np);
-  zdouble H[2] = {0, 0}, G[6] = {0, 0, 0, 0, 0, 0};
+  zdouble H[2] = {0, 0}, G[6] = {0, 0, 0, 0, 0, 0}, T[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < np; i++) {
     double u = TCR[3*i], v = TCR[3*i+1], w = TCR[3*i+2];
     double X[3], F[3];
     for (int m = 0; m < 3; m++) { X[m] = V0[m] + u*Av[m] + v*Bv[m]; F[m] = X[m] - Qa[m]; }
-    zdouble HI[2] = {0, 0}, GI[6] = {0, 0, 0, 0, 0, 0};
+    zdouble HI[2] = {0, 0}, GI[6] = {0, 0, 0, 0, 0, 0}, TI[6] = {0, 0, 0, 0, 0, 0};
     for (int j = 0; j < np; j++) {
       double up = TCR[3*j], vp = TCR[3*j+1], wp = TCR[3*j+2];
       double XP[3], FP[3], R[3];
@@ -89,14 +105,16 @@ static void ppis_cubature(OPPIArgs *A, int DeSing, int HighOrder,
         FP[m] = XP[m] - Qb[m];
         R[m] = X[m] - XP[m];
       }
-      inner_integrand(wp, R, F, FP, A->k, DeSing, HI, want_grad ? GI : NULL);
+      inner_integrand(wp, R, X, F, FP, A->k, DeSing, HI, want_grad ? GI : NULL, ntq, A->GammaMatrix, TI);
     }
     H[0] += w*HI[0];
     H[1] += w*HI[1];
     if (want_grad) for (int m = 0; m < 6; m++) G[m] += w*GI[m];
+    for (int m = 0; m < 2*ntq; m++) T[m] += w*TI[m];
   }
   A->H[0] = H[0]; A->H[1] = H[1];
   if (want_grad) memcpy(A->GradH, G, sizeof(G));
+  if (ntq) memcpy(A->dHdT, T, sizeof(T));
 }
 
 /* ------------------------------------------------------------------ */
@@ -333,18 +351,23 @@ void o_get_ppis(OPPIArgs *A)
     A->H[0] = Result[1] - 4.0*Result[0]/(k*k);
     A->H[1] = (ncv == 3) ? 0.0 : Result[2];
     memset(A->GradH, 0, sizeof(A->GradH));
+    memset(A->dHdT, 0, sizeof(A->dHdT));
     return;
   }
-  zdouble GradSave[6];
+  zdouble GradSave[6], TorqueSave[6];
   A->WhichAlgorithm = OPPI_DESING;
-  if (A->NumGradientComponents > 0) {
+  int ntq = A->NumTorqueAxes;
+  if (A->NumGradientComponents > 0 || (ntq > 0 && A->GammaMatrix)) {
     ppis_cubature(A, 0, 1, Va, Qa, Vb, Qb);
     memcpy(GradSave, A->GradH, sizeof(GradSave));
+    memcpy(TorqueSave, A->dHdT, sizeof(TorqueSave));
   }
   int ng = A->NumGradientComponents;
   A->NumGradientComponents = 0;
+  A->NumTorqueAxes = 0;
   ppis_cubature(A, 1, 0, Va, Qa, Vb, Qb);
   A->NumGradientComponents = ng;
+  A->NumTorqueAxes = ntq;
   OQDFIPPIData D;
   qdfippi(Va, Qa, Vb, Qb, ncv, A->FC, &D);
   zdouble ik = I*k, OOIK2 = 1.0/(ik*ik), PF[5];
@@ -359,4 +382,5 @@ void o_get_ppis(OPPIArgs *A)
            + PF[3]*(1.0/3.0)*D.hTimesR0
            + PF[4]*(1.0/6.0)*D.hTimesR1;
   if (ng > 0) memcpy(A->GradH, GradSave, sizeof(GradSave));
+  if (ntq > 0 && A->GammaMatrix) memcpy(A->dHdT, TorqueSave, sizeof(TorqueSave));
 }

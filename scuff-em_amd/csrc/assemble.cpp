@@ -182,7 +182,7 @@ Assembler::~Assembler()
   for (auto &D : ds_) { hipFree(D.V); hipFree(D.PVI); hipFree(D.PC); hipFree(D.EI); hipFree(D.EL); }
   for (auto &B : blocks_) { hipFree(B.d_rowPtr); hipFree(B.d_col); hipFree(B.d_idx); hipFree(B.d_pairs); }
   hipFree(d_jobs_); hipFree(d_records_);
-  hipFree(d_HP_); hipFree(d_HN_); hipFree(d_HS_); hipFree(d_scratch_);
+  hipFree(d_HP_); hipFree(d_HN_); hipFree(d_HS_); hipFree(d_scratch_); hipFree(d_stage_);
   hipFree(d_regs_);
 }
 
@@ -438,6 +438,39 @@ static int common_regions(const Surface &Sa, const Surface &Sb, int CR[2], doubl
   return n;
 }
 
+// regions shared by surfaces sa and sb, with wavenumber and the three
+// prefactors of the 2x2 block (SurfaceSurfaceInteractions.cc:62-90,
+// :162-176); -5 for a region in the short-wavelength regime
+int Assembler::regions_for(int sa, int sb, const std::vector<cdouble> &eps, const std::vector<cdouble> &mu,
+                           cdouble omega, RegionK rk[2])
+{
+  const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
+  int CR[2]; double Sg[2];
+  int nreg = common_regions(Sa, Sb, CR, Sg);
+  int used = 0;
+  for (int r = 0; r < nreg; r++) {
+    cdouble E = eps[CR[r]], Mu = mu[CR[r]];
+    if (E == 0.0) continue;
+    cdouble k = csqrt2(E * Mu) * omega, I(0, 1);
+    cdouble pf1 = Sg[r] * I * Mu * omega, pf2 = -Sg[r] * I * k, pf3 = -Sg[r] * I * E * omega;
+    rk[used].k = mk(k.real(), k.imag());
+    rk[used].pf1 = mk(pf1.real(), pf1.imag());
+    rk[used].pf2 = mk(pf2.real(), pf2.imag());
+    rk[used].pf3 = mk(pf3.real(), pf3.imag());
+    // regimes handled on the GPU: every near pair in the
+    // desingularization regime |k| R_max <= pi (PanelPanelInteractions.cc:429-491)
+    double Rmax = 0;
+    for (auto &P : Sa.Panels) Rmax = std::max(Rmax, P.Radius);
+    for (auto &P : Sb.Panels) Rmax = std::max(Rmax, P.Radius);
+    if (std::abs(k) * Rmax > M_PI) {
+      err_ = "short-wavelength regime (|k| R_panel > pi) is not implemented on the GPU path";
+      return -5;
+    }
+    used++;
+  }
+  return used;
+}
+
 int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
 {
   if (!err_.empty()) return -3;
@@ -458,33 +491,12 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
       const DevSurface &Da = ds_[sa], &Db = ds_[sb];
       NearBlock &B = blocks_[bi];
       const int same = (sa == sb);
-      int CR[2]; double Sg[2];
-      int nreg = common_regions(Sa, Sb, CR, Sg);
       RegionK rk[2];
-      int used = 0;
-      for (int r = 0; r < nreg; r++) {
-        cdouble E = eps[CR[r]], Mu = mu[CR[r]];
-        if (E == 0.0) continue;
-        cdouble k = csqrt2(E * Mu) * omega, I(0, 1);
-        cdouble pf1 = Sg[r] * I * Mu * omega, pf2 = -Sg[r] * I * k, pf3 = -Sg[r] * I * E * omega;
-        rk[used].k = mk(k.real(), k.imag());
-        rk[used].pf1 = mk(pf1.real(), pf1.imag());
-        rk[used].pf2 = mk(pf2.real(), pf2.imag());
-        rk[used].pf3 = mk(pf3.real(), pf3.imag());
-        // regimes handled on the GPU: every near pair in the
-        // desingularization regime |k| R_max <= pi, self pairs with
-        // |k| R <= 20 (PanelPanelInteractions.cc:429-491)
-        double Rmax = 0;
-        for (auto &P : Sa.Panels) Rmax = std::max(Rmax, P.Radius);
-        for (auto &P : Sb.Panels) Rmax = std::max(Rmax, P.Radius);
-        if (std::abs(k) * Rmax > M_PI) {
-          err_ = "short-wavelength regime (|k| R_panel > pi) is not implemented on the GPU path";
-          for (auto &e : ev) hipEventDestroy(e);
-          return -5;
-        }
-        used++;
+      int nreg = regions_for(sa, sb, eps, mu, omega, rk);
+      if (nreg < 0) {
+        for (auto &e : ev) hipEventDestroy(e);
+        return nreg;
       }
-      nreg = used;
       const int needH1 = !(Sa.IsPEC && Sb.IsPEC);
       const int stride = needH1 ? 4 : 2;
       HIPCHK(hipMemcpyAsync(d_regs_ + 2 * (bi % 2), rk, sizeof(RegionK) * std::max(nreg, 1), hipMemcpyHostToDevice, stream));
@@ -551,6 +563,107 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
   stats_.mirror_ms = ms;
   stats_.self_ms = t_self; stats_.far_ms = t_far; stats_.near_ms = t_near; stats_.stamp_ms = t_stamp;
   for (auto &e : ev) hipEventDestroy(e);
+  if (prof_enabled()) prof_collect();
+  return 0;
+}
+
+__global__ void k_transpose_into(const double2 *T, long ldt, int rows, int cols, double2 *B, long ldb)
+{
+  __shared__ double2 tile[32][33];
+  int bx = blockIdx.x * 32, by = blockIdx.y * 32;   // tile of T: rows bx.., cols by..
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int yy = ty; yy < 32; yy += 8) {
+    int r = bx + tx, c = by + yy;
+    if (r < rows && c < cols) tile[yy][tx] = T[r + (long)c * ldt];
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    int r = by + tx, c = bx + yy;              // B(r, c) = T(c, r)
+    if (r < cols && c < rows) B[r + (long)c * ldb] = tile[tx][yy];
+  }
+}
+
+// AssembleBEMMatrixBlock for two distinct surfaces (libscuff.h:500-505,
+// AssembleBEMMatrix.cc:349-391): the block B = M(rows of sa, columns of
+// sb) and, if GB is given, its derivatives with respect to a rigid
+// displacement of surface sa (three matrices gstride elements apart)
+int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ldb, double2 *GB, long ldg,
+                              long gstride, int ntorque, const double *gamma, double2 *TB, hipStream_t stream)
+{
+  if (ntorque < 0 || ntorque > 3 || (ntorque > 0 && (!gamma || !TB || !GB))) {
+    err_ = "torque derivatives need 1 to 3 axes, their generators and the gradient output";
+    return -1;
+  }
+  if (!err_.empty()) return -3;
+  const int NS = (int)G_.Surfaces.size();
+  if (sa == sb || sa < 0 || sb < 0 || sa >= NS || sb >= NS) { err_ = "the block needs two distinct surfaces"; return -1; }
+  const Surface &SA = G_.Surfaces[sa], &SB = G_.Surfaces[sb];
+  if (ldb < SA.NumBFs || (GB && ldg < SA.NumBFs) || (GB && gstride < ldg * (long)SB.NumBFs)) {
+    err_ = "leading dimension smaller than the block";
+    return -1;
+  }
+  int rc = build_fippi(stream);
+  if (rc) return rc;
+  std::vector<cdouble> eps(G_.Regions.size()), mu(G_.Regions.size());
+  for (size_t r = 0; r < G_.Regions.size(); r++) G_.Regions[r].eps_mu(omega, &eps[r], &mu[r]);
+  const int lo = std::min(sa, sb), hi = std::max(sa, sb);
+  int bi = 0;
+  for (int i = 0; i < lo; i++) bi += NS - i;
+  bi += hi - lo;
+  const Surface &Sa = G_.Surfaces[lo], &Sb = G_.Surfaces[hi];
+  const DevSurface &Da = ds_[lo], &Db = ds_[hi];
+  NearBlock &NB = blocks_[bi];
+  RegionK rk[2];
+  int nreg = regions_for(lo, hi, eps, mu, omega, rk);
+  if (nreg < 0) return nreg;
+  const int needH1 = !(Sa.IsPEC && Sb.IsPEC);
+  const int stride = needH1 ? 4 : 2;
+  HIPCHK(hipMemcpyAsync(d_regs_, rk, sizeof(RegionK) * std::max(nreg, 1), hipMemcpyHostToDevice, stream));
+  long npairs = (long)Da.nP * Db.nP;
+  if (ensure(&d_HP_, &cap_HP_, (size_t)npairs * nreg * 9 * stride)) return -2;
+  if (ensure(&d_HN_, &cap_HN_, (size_t)NB.pairs.size() * nreg * 9 * stride)) return -2;
+  // the block in (lo, hi) orientation, directly into B or into a
+  // staging copy that is transposed into B
+  double2 *target = B;
+  long ldt = ldb;
+  double *stage = nullptr;
+  if (sa > sb) {
+    if (ensure(&d_stage_, &cap_stage_, (size_t)2 * Sa.NumBFs * Sb.NumBFs)) return -2;
+    stage = d_stage_;
+    target = (double2 *)stage;
+    ldt = Sa.NumBFs;
+  }
+  if (nreg > 0) {
+    HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, Da.nP, Db.V, Db.PVI, Db.PC, Db.nP, 0, NB.d_rowPtr, NB.d_col,
+                      nreg, d_regs_, needH1, d_HP_));
+    if (!NB.pairs.empty())
+      HIPCHK(launch_near(stream, NB.d_pairs, (int)NB.pairs.size(), d_records_, nreg, d_regs_, needH1, d_HN_));
+  }
+  HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, 0, Sa.IsPEC, Sb.IsPEC, nreg,
+                      d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx, target, ldt, 0, 0));
+  if (stage) {
+    dim3 grid((Sa.NumBFs + 31) / 32, (Sb.NumBFs + 31) / 32);
+    hipLaunchKernelGGL(k_transpose_into, grid, dim3(256), 0, stream, (const double2 *)stage, ldt, Sa.NumBFs,
+                       Sb.NumBFs, B, ldb);
+    HIPCHK(hipGetLastError());
+  }
+  if (GB) {
+    // gradient in the caller's orientation; regions and prefactors of (sa, sb)
+    RegionK gk[2];
+    int ng = regions_for(sa, sb, eps, mu, omega, gk);
+    if (ng < 0) return ng;
+    HIPCHK(hipMemcpyAsync(d_regs_ + 2, gk, sizeof(RegionK) * std::max(ng, 1), hipMemcpyHostToDevice, stream));
+    const DevSurface &DA = ds_[sa], &DB = ds_[sb];
+    int tok = prof_begin(stream, PK_GRAD);
+    GammaSet GS;
+    std::memset(&GS, 0, sizeof(GS));
+    GS.n = ntorque;
+    for (int i = 0; i < 9 * ntorque; i++) GS.g[i] = gamma[i];
+    HIPCHK(launch_grad(stream, DA.nE, DA.EI, DA.EL, DA.V, DA.PVI, DA.PC, DB.nE, DB.EI, DB.EL, DB.V, DB.PVI, DB.PC,
+                       SA.IsPEC, SB.IsPEC, ng, d_regs_ + 2, GB, ldg, gstride, GS, TB));
+    prof_end(stream, tok, 0.0);
+  }
+  HIPCHK(hipStreamSynchronize(stream));
   if (prof_enabled()) prof_collect();
   return 0;
 }

@@ -41,6 +41,10 @@ class Assembler {
   // fills the full N x N matrix (column-major complex128, device memory)
   int assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream);
   int build_fippi(hipStream_t stream);
+  // block of two distinct surfaces (sa rows, sb columns) and optionally its
+  // derivatives with respect to a rigid displacement of surface sa
+  int assemble_block(int sa, int sb, cdouble omega, double2 *B, long ldb, double2 *GB, long ldg, long gstride,
+                     int ntorque, const double *gamma, double2 *TB, hipStream_t stream);
   const AsmStats &stats() const { return stats_; }
   const DevSurface &dev_surface(int s) const { return ds_[s]; }
   std::string error() const { return err_; }
@@ -56,12 +60,14 @@ class Assembler {
   FippiJob *d_jobs_ = nullptr;
   double *d_records_ = nullptr;
   bool fippi_ready_ = false;
-  double *d_HP_ = nullptr, *d_HN_ = nullptr, *d_HS_ = nullptr, *d_scratch_ = nullptr;
-  size_t cap_HP_ = 0, cap_HN_ = 0, cap_HS_ = 0, cap_scratch_ = 0;
+  double *d_HP_ = nullptr, *d_HN_ = nullptr, *d_HS_ = nullptr, *d_scratch_ = nullptr, *d_stage_ = nullptr;
+  size_t cap_HP_ = 0, cap_HN_ = 0, cap_HS_ = 0, cap_scratch_ = 0, cap_stage_ = 0;
   RegionK *d_regs_ = nullptr;
   AsmStats stats_;
   std::string err_;
   void build_near();
+  int regions_for(int sa, int sb, const std::vector<cdouble> &eps, const std::vector<cdouble> &mu, cdouble omega,
+                  RegionK rk[2]);
   int ensure(double **p, size_t *cap, size_t need);
 };
 

@@ -71,5 +71,15 @@ hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const doubl
                         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
                         double2 *M, long ldm, int rowOff, int colOff);
 hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n);
+// rotation generators of up to three torque axes, Gamma[9 t + i + 3 j]
+struct GammaSet {
+  int n;
+  double g[27];
+};
+hipError_t launch_grad(hipStream_t stream, int nEa, const int *EIa, const double *ELa, const double *Va,
+                       const int *PVIa, const double *PCa, int nEb, const int *EIb, const double *ELb,
+                       const double *Vb, const int *PVIb, const double *PCb, int pecA, int pecB, int nreg,
+                       const RegionK *regs, double2 *GB, long ldg, long gstride, const GammaSet &GS,
+                       double2 *TB);
 
 }  // namespace scuffx

@@ -111,6 +111,14 @@ int scuffx_geom_counts(const scuffx_geom *G, int ns, int *nV, int *nP, int *nE)
   return 0;
 }
 
+int scuffx_geom_bf_range(const scuffx_geom *G, int ns, int *offset, int *nbf)
+{
+  if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
+  if (offset) *offset = G->G.BFIndexOffset[ns];
+  if (nbf) *nbf = G->G.Surfaces[ns].NumBFs;
+  return 0;
+}
+
 int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out, double *len)
 {
   if (!G || !out || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad arguments");
@@ -179,6 +187,18 @@ int scuffx_assemble(scuffx_geom *G, double wr, double wi, void *M, long ldm)
   if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
   Assembler *A = G->assembler();
   int rc = A->assemble(cdouble(wr, wi), (double2 *)M, ldm, g_stream);
+  if (rc) return fail(rc, A->error());
+  return 0;
+}
+
+int scuffx_assemble_block(scuffx_geom *G, int nsa, int nsb, double wr, double wi, void *B, long ldb, void *GradB,
+                          long ldg, long gstride, int ntorque, const double *gamma, void *dBdT)
+{
+  if (!G || !B || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
+  Assembler *A = G->assembler();
+  int rc = A->assemble_block(nsa, nsb, cdouble(wr, wi), (double2 *)B, ldb, (double2 *)GradB, ldg, gstride, ntorque,
+                             gamma, (double2 *)dBdT, g_stream);
   if (rc) return fail(rc, A->error());
   return 0;
 }

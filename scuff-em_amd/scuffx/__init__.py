@@ -49,6 +49,7 @@ def lib():
             "scuffx_geom_total_bfs": ([vp], C.c_int),
             "scuffx_geom_num_surfaces": ([vp], C.c_int),
             "scuffx_geom_counts": ([vp, C.c_int, ip, ip, ip], C.c_int),
+            "scuffx_geom_bf_range": ([vp, C.c_int, ip, ip], C.c_int),
             "scuffx_geom_edges": ([vp, C.c_int, vp, vp], C.c_int),
             "scuffx_geom_panels": ([vp, C.c_int, vp], C.c_int),
             "scuffx_icosphere": ([C.c_int, C.c_double, vp, vp, vp], C.c_int),
@@ -57,6 +58,8 @@ def lib():
             "scuffx_fippi_records": ([vp, vp], C.c_int),
             "scuffx_assemble": ([vp, C.c_double, C.c_double, vp, C.c_long], C.c_int),
             "scuffx_assemble_stats": ([vp, vp, vp], C.c_int),
+            "scuffx_assemble_block": ([vp, C.c_int, C.c_int, C.c_double, C.c_double, vp, C.c_long, vp, C.c_long,
+                                       C.c_long, C.c_int, vp, vp], C.c_int),
             "scuffx_lu_factor": ([vp, C.c_long, C.c_long, vp, C.c_int], C.c_int),
             "scuffx_lu_solve": ([vp, C.c_long, C.c_long, vp, C.c_char, C.c_long, vp, C.c_long, C.c_int], C.c_int),
             "scuffx_lu32_from_c128": ([vp, C.c_long, C.c_long, vp, C.c_long, vp], C.c_int),
@@ -107,6 +110,15 @@ def icosphere(level, radius=1.0, center=None):
     _check(lib().scuffx_icosphere(level, radius, None if c is None else c.ctypes.data_as(C.c_void_p),
                                   V.ctypes.data_as(C.c_void_p), T.ctypes.data_as(C.c_void_p)))
     return V, T
+
+
+def CreateGammaMatrix(axis):
+    """3x3 generator Gamma of rotations about `axis` (Gamma x = u x x for the
+    unit axis u), the matrix of CreateGammaMatrix
+    (libs/libscuff/EdgeEdgeInteractions.cc:241-270)."""
+    u = np.asarray(axis, dtype=np.float64)
+    u = u / np.linalg.norm(u)
+    return np.array([[0.0, -u[2], u[1]], [u[2], 0.0, -u[0]], [-u[1], u[0], 0.0]])
 
 
 class HVector:
@@ -277,6 +289,47 @@ class RWGGeometry:
         M.factored = False
         return M
 
+    def BFOffset(self, ns):
+        """(first row of surface ns in the BEM matrix, its number of basis
+        functions) -- BFIndexOffset and NumBFs of the reference"""
+        off, nbf = C.c_int(), C.c_int()
+        _check(lib().scuffx_geom_bf_range(self.h, ns, C.byref(off), C.byref(nbf)))
+        return off.value, nbf.value
+
+    def BlockSizes(self, nsa, nsb):
+        return self.BFOffset(nsa)[1], self.BFOffset(nsb)[1]
+
+    def AssembleBEMMatrixBlock(self, nsa, nsb, Omega, GradM=False, TorqueAxes=None):
+        """Block (nsa rows, nsb columns) of the BEM matrix for two distinct
+        surfaces (RWGGeometry::AssembleBEMMatrixBlock, libscuff.h:500-505).
+        GradM=True adds its derivatives with respect to a rigid displacement
+        of surface nsa along x, y, z; TorqueAxes (up to three 3-vectors) adds
+        the derivatives with respect to a rotation of surface nsa about each
+        axis through the origin (dMdT).
+        Returns B, or (B, [dB/dx, dB/dy, dB/dz]), or (B, GradB, [dB/dtheta...])."""
+        torch = _torch()
+        na, nb = self.BlockSizes(nsa, nsb)
+        B = HMatrix(na, nb)
+        nt = 0 if TorqueAxes is None else len(TorqueAxes)
+        want = GradM or nt > 0
+        G3 = torch.zeros((3, nb, na), dtype=torch.complex128, device="cuda") if want else None
+        T3 = torch.zeros((max(nt, 1), nb, na), dtype=torch.complex128, device="cuda") if nt else None
+        gam = np.zeros(9 * max(nt, 1))
+        for t in range(nt):
+            gam[9 * t:9 * t + 9] = CreateGammaMatrix(TorqueAxes[t]).ravel(order="F")
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_assemble_block(self.h, nsa, nsb, Omega.real, Omega.imag, _ptr(B.t), na,
+                                           _ptr(G3) if want else None, na, na * nb, nt,
+                                           gam.ctypes.data_as(C.c_void_p) if nt else None,
+                                           _ptr(T3) if nt else None))
+        if not want:
+            return B
+        grads = [HMatrix(na, nb, data=G3[m]) for m in range(3)]
+        if nt == 0:
+            return B, grads
+        return B, grads, [HMatrix(na, nb, data=T3[t]) for t in range(nt)]
+
     def AssemblyStats(self):
         ms = np.zeros(6)
         cnt = np.zeros(4, dtype=np.int64)
@@ -310,7 +363,7 @@ class RWGGeometry:
 
 
 PROFILED_KERNELS = ("gemm_c128", "gemm_c64", "panel_c128", "panel_c64", "trsm_c128", "trsm_c64",
-                    "far", "near", "self", "stamp", "fippi", "resid")
+                    "far", "near", "self", "stamp", "fippi", "resid", "grad")
 
 
 def profile(on=True):
