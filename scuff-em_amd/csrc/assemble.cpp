@@ -370,8 +370,8 @@ int Assembler::build_fippi(hipStream_t stream)
     HIPCHK(hipMalloc(&d_ovf, sizeof(int) * nj));
     HIPCHK(hipMemsetAsync(d_ovf, 0, sizeof(int) * nj, stream));
     HIPCHK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
-    // pass 1: small heaps for every job
-    int cap1 = 2048, slots1 = 1024;
+    // pass 1: small heaps (4096 regions) for every job, 2048 waves
+    int cap1 = 4096, slots1 = std::min(2048, std::max(1, nj));
     double *hd = nullptr, *hk = nullptr;
     int *hs = nullptr;
     HIPCHK(hipMalloc(&hd, sizeof(double) * (size_t)cap1 * 76 * slots1));
@@ -387,8 +387,10 @@ int Assembler::build_fippi(hipStream_t stream)
     std::vector<int> redo;
     for (int j = 0; j < nj; j++) if (ovf[j]) redo.push_back(j);
     if (!redo.empty()) {
-      // pass 2: the few jobs that outgrew the small heap, full-size heaps
-      int cap2 = 32768, slots2 = std::min<int>(128, (int)redo.size());
+      // pass 2: the jobs that outgrew the small heap, heaps large enough for
+      // the whole evaluation budget (1e6 points / 66 per split), up to 1024
+      // waves (about 20 MB of heap each)
+      int cap2 = 32768, slots2 = std::min<int>(1024, (int)redo.size());
       int *d_list = nullptr;
       HIPCHK(hipMalloc(&d_list, sizeof(int) * redo.size()));
       HIPCHK(hipMemcpyAsync(d_list, redo.data(), sizeof(int) * redo.size(), hipMemcpyHostToDevice, stream));
