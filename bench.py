@@ -33,7 +33,21 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-lu-n", type=int, default=2048, help="oracle LU sample size")
     ap.add_argument("--cpu-rows", type=int, default=6, help="oracle assembly sample (edge rows)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the HBM "
+                         "byte counters of a profiler against a copy of known size) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "copies": 5}))
 
 
 def traffic_from_profiles(kernel):
@@ -87,6 +101,9 @@ def cpu_baseline(level, omega, nrows, nlu, eps):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the sharded design is described in DESIGN.md)", file=sys.stderr)

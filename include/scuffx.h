@@ -64,6 +64,9 @@ int scuffx_geom_bf_range(const scuffx_geom *G, int ns, int *offset, int *nbf);
 int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out9, double *lengths);
 /* per panel: centroid[3] zhat[3] radius area (libscuff.h:100-112) */
 int scuffx_geom_panels(const scuffx_geom *G, int ns, double *out8);
+/* vertex coordinates of surface ns after its birth transform, 3 per vertex
+   (RWGSurface::Vertices, libscuff.h:249) */
+int scuffx_geom_vertices(const scuffx_geom *G, int ns, double *out3);
 /* test-mesh generator: V[3*(10*4^L+2)], T[3*20*4^L] */
 int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T);
 
@@ -79,8 +82,9 @@ int scuffx_fippi_records(scuffx_geom *G, double *out);
 int scuffx_assemble(scuffx_geom *G, double omega_re, double omega_im, void *M_dev, long ldm);
 /* ---- RWGGeometry::AssembleBEMMatrixBlock(nsa, nsb, Omega, 0, M, GradM)
         (libscuff.h:500-505, AssembleBEMMatrix.cc:349-391) ----
-   block of two distinct surfaces: B (NBF_a x NBF_b, column-major, leading
-   dimension ldb) and, if GradB_dev is not NULL, its derivatives with respect
+   block of surfaces nsa (rows) and nsb (columns): B (NBF_a x NBF_b,
+   column-major, leading dimension ldb); nsa == nsb gives the diagonal block
+   (no derivatives).  For nsa != nsb, if GradB_dev is not NULL, its derivatives with respect
    to a rigid displacement of surface nsa along x, y, z: three matrices at
    GradB_dev + m*gstride (complex entries), leading dimension ldg; with
    ntorque = 1..3 also the derivatives with respect to a rotation of surface
@@ -102,6 +106,18 @@ int scuffx_lu_factor(void *A_dev, long n, long lda, int *ipiv_dev, int prec);
 /* ---- HMatrix::LUSolve (LBWrappers.cc:175-245); trans 'N', 'T' or 'C' ---- */
 int scuffx_lu_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev, char trans,
                     long nrhs, void *B_dev, long ldb, int prec);
+
+/* ---- quantities the Casimir caller reads from the factors
+        (applications/scuff-cas3D/CasimirIntegrand.cc:51-146) ----
+   log |det A| and arg det A of an LU-factored matrix (diagonal of U and the
+   parity of the interchanges) */
+int scuffx_lu_logdet(const void *LU_dev, long n, long lda, const int *ipiv_dev, int prec,
+                     double *logabs, double *phase);
+/* sum of the first ncols diagonal entries of A^{-1} D with the factors of A;
+   D (n x ncols, ncols <= n, device) is overwritten by A^{-1} D
+   (GetTraceMInvdM, CasimirIntegrand.cc:100-146, where D is N x N1) */
+int scuffx_trace_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev, void *D_dev, long ldd,
+                       long ncols, int prec, double *tr_re, double *tr_im);
 
 /* ---- mixed precision: complex64 LU of a complex128 matrix plus
         complex128 iterative refinement (no counterpart in the reference;

@@ -585,10 +585,12 @@ __global__ void k_transpose_into(const double2 *T, long ldt, int rows, int cols,
   }
 }
 
-// AssembleBEMMatrixBlock for two distinct surfaces (libscuff.h:500-505,
-// AssembleBEMMatrix.cc:349-391): the block B = M(rows of sa, columns of
-// sb) and, if GB is given, its derivatives with respect to a rigid
-// displacement of surface sa (three matrices gstride elements apart)
+// AssembleBEMMatrixBlock (libscuff.h:500-505, AssembleBEMMatrix.cc:349-391):
+// the block B = M(rows of sa, columns of sb) and, for two distinct
+// surfaces, if GB is given, its derivatives with respect to a rigid
+// displacement of surface sa (three matrices gstride elements apart).
+// sa == sb gives the diagonal block (the T block of the Casimir caller,
+// CasimirIntegrand.cc:276-291) without derivatives.
 int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ldb, double2 *GB, long ldg,
                               long gstride, int ntorque, const double *gamma, double2 *TB, hipStream_t stream)
 {
@@ -598,7 +600,9 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
   }
   if (!err_.empty()) return -3;
   const int NS = (int)G_.Surfaces.size();
-  if (sa == sb || sa < 0 || sb < 0 || sa >= NS || sb >= NS) { err_ = "the block needs two distinct surfaces"; return -1; }
+  if (sa < 0 || sb < 0 || sa >= NS || sb >= NS) { err_ = "surface index out of range"; return -1; }
+  const int same = (sa == sb);
+  if (same && GB) { err_ = "derivatives need two distinct surfaces"; return -1; }
   const Surface &SA = G_.Surfaces[sa], &SB = G_.Surfaces[sb];
   if (ldb < SA.NumBFs || (GB && ldg < SA.NumBFs) || (GB && gstride < ldg * (long)SB.NumBFs)) {
     err_ = "leading dimension smaller than the block";
@@ -621,9 +625,15 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
   const int needH1 = !(Sa.IsPEC && Sb.IsPEC);
   const int stride = needH1 ? 4 : 2;
   HIPCHK(hipMemcpyAsync(d_regs_, rk, sizeof(RegionK) * std::max(nreg, 1), hipMemcpyHostToDevice, stream));
-  long npairs = (long)Da.nP * Db.nP;
+  long npairs = same ? (long)Da.nP * (Da.nP - 1) / 2 : (long)Da.nP * Db.nP;
   if (ensure(&d_HP_, &cap_HP_, (size_t)npairs * nreg * 9 * stride)) return -2;
   if (ensure(&d_HN_, &cap_HN_, (size_t)NB.pairs.size() * nreg * 9 * stride)) return -2;
+  if (same && nreg > 0) {
+    const int chunk = 16384;
+    if (ensure(&d_HS_, &cap_HS_, (size_t)Da.nP * nreg * 18)) return -2;
+    if (ensure(&d_scratch_, &cap_scratch_, (size_t)chunk * 1025 * 4)) return -2;
+    HIPCHK(launch_self(stream, Da.V, Da.PVI, Da.nP, nreg, d_regs_, d_HS_, d_scratch_, chunk));
+  }
   // the block in (lo, hi) orientation, directly into B or into a
   // staging copy that is transposed into B
   double2 *target = B;
@@ -636,13 +646,14 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
     ldt = Sa.NumBFs;
   }
   if (nreg > 0) {
-    HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, Da.nP, Db.V, Db.PVI, Db.PC, Db.nP, 0, NB.d_rowPtr, NB.d_col,
+    HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, Da.nP, Db.V, Db.PVI, Db.PC, Db.nP, same, NB.d_rowPtr, NB.d_col,
                       nreg, d_regs_, needH1, d_HP_));
     if (!NB.pairs.empty())
       HIPCHK(launch_near(stream, NB.d_pairs, (int)NB.pairs.size(), d_records_, nreg, d_regs_, needH1, d_HN_));
   }
-  HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, 0, Sa.IsPEC, Sb.IsPEC, nreg,
-                      d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx, target, ldt, 0, 0));
+  HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same, Sa.IsPEC, Sb.IsPEC,
+                      nreg, d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx, target, ldt, 0, 0));
+  if (same) HIPCHK(launch_mirror(stream, B, ldb, Sa.NumBFs));
   if (stage) {
     dim3 grid((Sa.NumBFs + 31) / 32, (Sb.NumBFs + 31) / 32);
     hipLaunchKernelGGL(k_transpose_into, grid, dim3(256), 0, stream, (const double2 *)stage, ldt, Sa.NumBFs,

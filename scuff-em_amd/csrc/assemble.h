@@ -41,8 +41,8 @@ class Assembler {
   // fills the full N x N matrix (column-major complex128, device memory)
   int assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream);
   int build_fippi(hipStream_t stream);
-  // block of two distinct surfaces (sa rows, sb columns) and optionally its
-  // derivatives with respect to a rigid displacement of surface sa
+  // block of surfaces sa (rows) and sb (columns); for sa != sb optionally
+  // its derivatives with respect to a rigid motion of surface sa
   int assemble_block(int sa, int sb, cdouble omega, double2 *B, long ldb, double2 *GB, long ldg, long gstride,
                      int ntorque, const double *gamma, double2 *TB, hipStream_t stream);
   const AsmStats &stats() const { return stats_; }

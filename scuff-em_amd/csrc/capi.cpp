@@ -146,6 +146,14 @@ int scuffx_geom_panels(const scuffx_geom *G, int ns, double *out)
   return 0;
 }
 
+int scuffx_geom_vertices(const scuffx_geom *G, int ns, double *out)
+{
+  if (!G || !out || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  const Surface &S = G->G.Surfaces[ns];
+  for (size_t i = 0; i < S.V.size(); i++) out[i] = S.V[i];
+  return 0;
+}
+
 int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T)
 {
   if (level < 0 || level > 9 || !V || !T) return fail(SCUFFX_ERR_ARG, "bad icosphere arguments");
@@ -237,6 +245,35 @@ int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char tran
   if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "solve workspace allocation failed");
   if (rc) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
   if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
+  return 0;
+}
+
+int scuffx_lu_logdet(const void *LU, long n, long lda, const int *ipiv, int prec, double *logabs, double *phase)
+{
+  if (!LU || !ipiv || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  double out[2] = {0, 0};
+  int rc;
+  if (prec == 64) rc = lu_logdet<double2>((const double2 *)LU, n, lda, ipiv, out, g_stream);
+  else if (prec == 32) rc = lu_logdet<float2>((const float2 *)LU, n, lda, ipiv, out, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  if (rc) return fail(SCUFFX_ERR_HIP, "log det failed on the device");
+  if (logabs) *logabs = out[0];
+  if (phase) *phase = out[1];
+  return 0;
+}
+
+int scuffx_trace_solve(const void *LU, long n, long lda, const int *ipiv, void *D, long ldd, long ncols, int prec,
+                       double *tr_re, double *tr_im)
+{
+  if (!LU || !ipiv || !D || n <= 0 || ncols <= 0 || ncols > n) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  int rc = scuffx_lu_solve(LU, n, lda, ipiv, 'N', ncols, D, ldd, prec);
+  if (rc) return rc;
+  double out[2] = {0, 0};
+  if (prec == 64) rc = mat_trace<double2>((const double2 *)D, ncols, ldd, out, g_stream);
+  else rc = mat_trace<float2>((const float2 *)D, ncols, ldd, out, g_stream);
+  if (rc) return fail(SCUFFX_ERR_HIP, "trace failed on the device");
+  if (tr_re) *tr_re = out[0];
+  if (tr_im) *tr_im = out[1];
   return 0;
 }
 

@@ -20,6 +20,13 @@ template <typename T>
 int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B,
              long ldb, hipStream_t stream);
 
+// log |det| and arg det of an LU-factored matrix (out[0], out[1]; host)
+template <typename T>
+int lu_logdet(const T *LU, long n, long lda, const int *ipiv, double out[2], hipStream_t stream);
+// complex trace of a square matrix (out[0] + i out[1]; host)
+template <typename T>
+int mat_trace(const T *A, long n, long lda, double out[2], hipStream_t stream);
+
 // r = b - A x (complex128, A column-major n x n); all device pointers
 int residual_c128(const double2 *A, long n, long lda, const double2 *x, const double2 *b,
                   double2 *r, double2 *work, hipStream_t stream);

@@ -52,6 +52,7 @@ def lib():
             "scuffx_geom_bf_range": ([vp, C.c_int, ip, ip], C.c_int),
             "scuffx_geom_edges": ([vp, C.c_int, vp, vp], C.c_int),
             "scuffx_geom_panels": ([vp, C.c_int, vp], C.c_int),
+            "scuffx_geom_vertices": ([vp, C.c_int, vp], C.c_int),
             "scuffx_icosphere": ([C.c_int, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_fippi_build": ([vp], C.c_int),
             "scuffx_fippi_count": ([vp], C.c_int),
@@ -67,6 +68,8 @@ def lib():
                                ip, dp], C.c_int),
             "scuffx_rhs_planewave": ([vp, C.c_double, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_power": ([vp, C.c_double, C.c_double, vp, vp, dp, dp], C.c_int),
+            "scuffx_lu_logdet": ([vp, C.c_long, C.c_long, vp, C.c_int, dp, dp], C.c_int),
+            "scuffx_trace_solve": ([vp, C.c_long, C.c_long, vp, vp, C.c_long, C.c_long, C.c_int, dp, dp], C.c_int),
             "scuffx_profile": ([C.c_int], C.c_int),
             "scuffx_profile_read": ([C.c_char_p, dp, dp, lp], C.c_int),
         }
@@ -200,6 +203,50 @@ class HMatrix:
         return 0
 
 
+    def LogDet(self):
+        """log det of the factored matrix as a complex number
+        (log |det| + i arg det, arg modulo 2 pi)"""
+        if not self.factored:
+            raise SCUFFXError("LogDet needs LUFactorize() first")
+        la, ph = C.c_double(), C.c_double()
+        _sync_stream()
+        _check(lib().scuffx_lu_logdet(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), self.prec,
+                                      C.byref(la), C.byref(ph)))
+        return complex(la.value, ph.value)
+
+    def TraceSolve(self, D):
+        """sum of the first D.NC diagonal entries of A^{-1} D with the
+        factors of A; D (NR x NC with NC <= NR, same precision) is
+        overwritten by A^{-1} D."""
+        if not self.factored:
+            raise SCUFFXError("TraceSolve needs LUFactorize() first")
+        if D.NR != self.NR or D.NC > self.NR or D.prec != self.prec:
+            raise SCUFFXError("TraceSolve: D must have NR rows, at most NR columns and the same precision")
+        tr, ti = C.c_double(), C.c_double()
+        _sync_stream()
+        _check(lib().scuffx_trace_solve(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), _ptr(D.t), D.NR,
+                                        D.NC, self.prec, C.byref(tr), C.byref(ti)))
+        return complex(tr.value, ti.value)
+
+
+def GetLNDetMInvMInf(M, MInf):
+    """energy integrand of the Casimir caller (CasimirIntegrand.cc:51-95,
+    method 1): -log|det(M^{-1} M_inf)| / (2 pi).  M is LU-factored; MInf is
+    the LU-factored block-diagonal matrix or its log|det| as a number."""
+    import math
+    la = MInf if isinstance(MInf, (int, float)) else MInf.LogDet().real
+    v = la - M.LogDet().real
+    return 0.0 if not math.isfinite(v) else -v / (2.0 * math.pi)
+
+
+def GetTraceMInvdM(M, dM):
+    """force integrand of the Casimir caller (CasimirIntegrand.cc:100-146):
+    -2 Re Tr(M^{-1} dM) / (2 pi) with M LU-factored; dM is overwritten."""
+    import math
+    t = 2.0 * M.TraceSolve(dM).real
+    return 0.0 if not math.isfinite(t) else -t / (2.0 * math.pi)
+
+
 class PlaneWave:
     """Plane wave E = E0 exp(i k nHat.x) in the exterior medium
     (libs/libIncField/PlaneWave.cc:71-87)."""
@@ -260,6 +307,13 @@ class RWGGeometry:
         _check(lib().scuffx_geom_panels(self.h, ns, out.ctypes.data_as(C.c_void_p)))
         return out
 
+    def Vertices(self, ns=0):
+        """vertex coordinates of surface ns, shape (NumVertices, 3)"""
+        nv, _, _ = self.Counts(ns)
+        out = np.zeros((nv, 3))
+        _check(lib().scuffx_geom_vertices(self.h, ns, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def AllocateBEMMatrix(self):
         N = self.TotalBFs
         return HMatrix(N, N)
@@ -300,8 +354,9 @@ class RWGGeometry:
         return self.BFOffset(nsa)[1], self.BFOffset(nsb)[1]
 
     def AssembleBEMMatrixBlock(self, nsa, nsb, Omega, GradM=False, TorqueAxes=None):
-        """Block (nsa rows, nsb columns) of the BEM matrix for two distinct
-        surfaces (RWGGeometry::AssembleBEMMatrixBlock, libscuff.h:500-505).
+        """Block (nsa rows, nsb columns) of the BEM matrix
+        (RWGGeometry::AssembleBEMMatrixBlock, libscuff.h:500-505); nsa == nsb
+        gives the diagonal block.  For distinct surfaces,
         GradM=True adds its derivatives with respect to a rigid displacement
         of surface nsa along x, y, z; TorqueAxes (up to three 3-vectors) adds
         the derivatives with respect to a rotation of surface nsa about each

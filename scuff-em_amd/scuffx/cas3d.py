@@ -1,0 +1,238 @@
+"""Casimir caller of the BEM hot path (applications/scuff-cas3D).
+
+At each imaginary frequency Xi the caller assembles the diagonal blocks
+T_s of every surface and the off-diagonal blocks U_{s,s'} of the BEM
+matrix, together with the derivatives dU_{0,s} with respect to a rigid
+displacement / rotation of surface 0, stamps them into M, LU-factorizes M
+and reads from the factors
+
+  energy integrand  -log|det(M^{-1} M_inf)| / (2 pi)    (M_inf = diag(T_s))
+  force / torque    -2 Re Tr(M^{-1} dM) / (2 pi)       (dM = dU^dagger stamped
+                                                        into the rows of s >= 1)
+
+(CasimirIntegrand.cc:51-146, :152-184, :298-473).  The imaginary-frequency
+integral uses the reference's default 'cliff' rule
+(libs/libTriInt/IntegrateCliffFunction.cc:33-320 with the cliff position
+0.7/d_min of SumsIntegrals.cc:606-660) or its fixed trapezoid/Simpson
+rule (SumsIntegrals.cc:532-587).
+
+Every block, factorization, determinant and trace runs on the GPU through
+the C ABI; this module only orchestrates them and moves device blocks
+into place.  Scope: compact (non-periodic) geometries at zero
+temperature, one spatial configuration per geometry object.
+"""
+import math
+
+import numpy as np
+
+from . import HMatrix, SCUFFXError
+
+XIMIN_DEFAULT = 0.001          # scuff-cas3D.cc:80
+RELTOL_DEFAULT = 1.0e-2        # scuff-cas3D.cc:84
+XIMIN_TRAPSIMP = 0.001         # interval of the fixed rule (SumsIntegrals.cc:34-35)
+XIMAX_TRAPSIMP = 10.0
+
+
+class SC3Data:
+    """Workspace of the Casimir caller (CreateSC3Data.cc:45-170):
+    geometry, requested quantities and the block / matrix storage reused
+    at every frequency.
+
+    Quantities are packed in the reference's order: energy, x-, y-,
+    z-force, then one torque per axis in TorqueAxes."""
+
+    def __init__(self, G, Energy=False, XForce=False, YForce=False, ZForce=False, TorqueAxes=None,
+                 XiMin=XIMIN_DEFAULT, AbsTol=0.0, RelTol=RELTOL_DEFAULT):
+        self.G = G
+        self.NS = G.NumSurfaces
+        self.N = G.TotalBFs
+        self.N1 = G.BFOffset(0)[1]
+        self.TorqueAxes = [] if TorqueAxes is None else [np.asarray(a, dtype=np.float64) for a in TorqueAxes]
+        if len(self.TorqueAxes) > 3:
+            raise SCUFFXError("at most three torque axes")
+        self.Mus = [m for m, on in enumerate((XForce, YForce, ZForce)) if on]
+        self.Mus += [3 + t for t in range(len(self.TorqueAxes))]
+        self.Energy = bool(Energy)
+        if not self.Energy and not self.Mus:
+            raise SCUFFXError("no Casimir quantity requested")
+        if self.Mus and self.NS < 2:
+            raise SCUFFXError("forces and torques need at least two surfaces")
+        self.NumQuantities = int(self.Energy) + len(self.Mus)
+        self.XiMin, self.AbsTol, self.RelTol = XiMin, AbsTol, RelTol
+        self.M = HMatrix(self.N, self.N)
+        self.dM = HMatrix(self.N, self.N1) if self.Mus else None
+        self.ByXi = []                       # (Xi, EFT) of every evaluation
+
+    def _offsets(self):
+        return [self.G.BFOffset(ns) for ns in range(self.NS)]
+
+    def GetCasimirIntegrand(self, Xi):
+        """EFT at imaginary frequency Xi (GetCasimirIntegrand,
+        CasimirIntegrand.cc:298-473, for a compact geometry)."""
+        G, M = self.G, self.M
+        Omega = complex(0.0, Xi)
+        offs = self._offsets()
+        # T blocks into M; the LU diagonals of the T blocks make up that of M_inf
+        M.t.zero_()
+        M.factored = False
+        logdet_inf = 0.0
+        for ns, (o, n) in enumerate(offs):
+            T = G.AssembleBEMMatrixBlock(ns, ns, Omega)
+            M.t[o:o + n, o:o + n].copy_(T.t)
+            if self.Energy:
+                T.LUFactorize()
+                logdet_inf += T.LogDet().real
+        # U blocks (upper triangle, adjoint below) and dU_{0,s}
+        dU = {}
+        nt = len(self.TorqueAxes)
+        for ns in range(self.NS):
+            for nsp in range(ns + 1, self.NS):
+                (o, n), (op, nq) = offs[ns], offs[nsp]
+                if ns == 0 and self.Mus:
+                    r = G.AssembleBEMMatrixBlock(0, nsp, Omega, GradM=True,
+                                                 TorqueAxes=self.TorqueAxes if nt else None)
+                    U = r[0]
+                    dU[nsp] = list(r[1]) + (list(r[2]) if nt else [])
+                else:
+                    U = G.AssembleBEMMatrixBlock(ns, nsp, Omega)
+                M.t[op:op + nq, o:o + n].copy_(U.t)                      # InsertBlock
+                M.t[o:o + n, op:op + nq].copy_(U.t.transpose(0, 1).conj())   # InsertBlockAdjoint
+        # Factorize (CasimirIntegrand.cc:152-184)
+        M.LUFactorize()
+        EFT = []
+        if self.Energy:
+            v = logdet_inf - M.LogDet().real          # GetLNDetMInvMInf, method 1
+            EFT.append(0.0 if not math.isfinite(v) else -v / (2.0 * math.pi))
+        for Mu in self.Mus:
+            dM = self.dM
+            dM.t.zero_()
+            for nsp in range(1, self.NS):
+                op, nq = offs[nsp]
+                D = dU[nsp][Mu]
+                dM.t[:, op:op + nq].copy_(D.t.transpose(0, 1).conj())   # InsertBlockAdjoint(dU, off, 0)
+            tr = 2.0 * M.TraceSolve(dM).real
+            EFT.append(0.0 if not math.isfinite(tr) else -tr / (2.0 * math.pi))
+        EFT = np.array(EFT)
+        self.ByXi.append((Xi, EFT.copy()))
+        return EFT
+
+    def MinSurfaceDistance(self):
+        """smallest vertex-vertex distance between distinct surfaces
+        (SumsIntegrals.cc:614-631); 1.0 if it is zero"""
+        V = [self.G.Vertices(ns) for ns in range(self.NS)]
+        d = math.inf
+        for a in range(self.NS):
+            for b in range(a + 1, self.NS):
+                diff = V[a][:, None, :] - V[b][None, :, :]
+                d = min(d, float(np.sqrt((diff * diff).sum(-1)).min()))
+        return 1.0 if d == 0.0 or not math.isfinite(d) else d
+
+    def GetXiIntegral_Cliff(self):
+        """integral over Xi from XiMin to infinity with the cliff rule
+        (SumsIntegrals.cc:606-660); returns (EFT, Error)"""
+        XiCliff = 0.7 / self.MinSurfaceDistance()
+        return IntegrateCliffFunction(lambda xi, skip: self.GetCasimirIntegrand(xi), self.NumQuantities,
+                                      self.XiMin, 0.0, XiCliff, self.AbsTol, self.RelTol)
+
+    def GetXiIntegral_TrapSimp(self, NumIntervals=50):
+        """XiMin * f(XiMin) plus NumIntervals Simpson panels of width
+        (10 - 0.001) / NumIntervals from XiMin on, with trapezoid-difference
+        errors (SumsIntegrals.cc:532-587)"""
+        fL = self.GetCasimirIntegrand(self.XiMin)
+        I = fL * self.XiMin
+        E = np.zeros_like(I)
+        Delta = (XIMAX_TRAPSIMP - XIMIN_TRAPSIMP) / NumIntervals
+        Xi = self.XiMin
+        for _ in range(NumIntervals):
+            fM = self.GetCasimirIntegrand(Xi + 0.5 * Delta)
+            fR = self.GetCasimirIntegrand(Xi + Delta)
+            Xi += Delta
+            simp = (fL + 4.0 * fM + fR) * Delta / 6.0
+            trap = (fL + 2.0 * fM + fR) * Delta / 4.0
+            I = I + simp
+            E = E + np.abs(simp - trap)
+            fL = fR
+        return I, E
+
+
+def IntegrateCliffFunction(f, nFun, xMin, xMax, xCliff, AbsTol, RelTol, MaxEvals=100000):
+    """Integral of a vector function that is flat up to a 'cliff' and then
+    decays (libs/libTriInt/IntegrateCliffFunction.cc:98-320): march from
+    xMin in steps Delta, each step estimated by Simpson (kept) against the
+    two-panel trapezoid (error); a step whose worst component misses both
+    tolerances is halved and retried, one that beats RelTol doubles the
+    next.  A component converges once its local mean |f| has stayed below
+    RelTol times its running mean for two consecutive steps (after the
+    first three); it is then no longer accumulated.  xMax <= xMin means
+    infinity.  f(x, skip) returns nFun values; skip flags converged ones.
+    Returns (Integral, Error)."""
+    if xCliff <= xMin:
+        if xMax > xMin:
+            xCliff = 0.5 * (xMax - xMin)
+        elif xMin > 0.0:
+            xCliff = 1.5 * xMin
+        else:
+            xCliff = 1.0
+    if xMax <= xMin:
+        xMax = 1.234e89
+    conv = np.zeros(nFun, dtype=bool)
+    conv_iters = np.zeros(nFun, dtype=np.int64)
+    I = np.zeros(nFun)
+    E = np.zeros(nFun)
+    AbsI = np.zeros(nFun)
+    fb = [None, None, None]
+    full = [False, False, False]
+    min_conv_iters, min_iters = 2, 3
+    Delta = 0.5 * xCliff
+    iters = evals = 0
+    x = xMin
+    all_conv = False
+    while not all_conv and x < xMax:
+        if evals > MaxEvals:
+            raise SCUFFXError("cliff integration did not converge within %d evaluations" % MaxEvals)
+        if x + Delta > xMax:
+            Delta = xMax - x
+        for i, xi in enumerate((x, x + 0.5 * Delta, x + Delta)):
+            if not full[i]:
+                fb[i] = np.asarray(f(xi, conv.copy()), dtype=np.float64)
+                evals += 1
+        fL, fM, fR = fb
+        simp = Delta * (fL + 4.0 * fM + fR) / 6.0
+        trap = Delta * (fL + 2.0 * fM + fR) / 4.0
+        dI = simp
+        dE = np.abs(simp - trap)
+        dAbs = Delta * (np.abs(fL) + 4.0 * np.abs(fM) + np.abs(fR)) / 6.0
+        act = ~conv
+        max_abs = 0.0
+        max_rel = 0.0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for nf in np.nonzero(act)[0]:
+                max_abs = float(np.fmax(max_abs, dE[nf]))
+                max_rel = float(np.fmax(max_rel, dE[nf] / abs(dI[nf])))   # fmax drops 0/0
+        if Delta > 1.0e-8 and max_rel > 2.0 * RelTol and max_abs > AbsTol:
+            Delta *= 0.5
+            fb[2] = fb[1]
+            full = [True, False, True]
+            continue
+        x += Delta
+        all_conv = True
+        for nf in np.nonzero(act)[0]:
+            I[nf] += dI[nf]
+            E[nf] += dE[nf]
+            AbsI[nf] += dAbs[nf]
+            if iters < min_iters:
+                all_conv = False
+                continue
+            full_slope = AbsI[nf] / (x - xMin)
+            local_slope = dAbs[nf] / Delta
+            conv_iters[nf] = conv_iters[nf] + 1 if local_slope <= RelTol * full_slope else 0
+            if conv_iters[nf] == min_conv_iters:
+                conv[nf] = True
+            else:
+                all_conv = False
+        iters += 1
+        fb[0] = fb[2]
+        full = [True, False, False]
+        if max_rel < RelTol:
+            Delta *= 2.0
+    return I, E

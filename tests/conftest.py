@@ -35,3 +35,32 @@ def sx():
 def rel_fro(A, B):
     """Relative Frobenius distance ||A-B|| / ||B||."""
     return float(np.linalg.norm(np.asarray(A) - np.asarray(B)) / np.linalg.norm(np.asarray(B)))
+
+
+def read_msh(path):
+    """Vertices (file order) and triangles (0-based) of a GMSH 2.x ASCII
+    mesh: every node is kept, every type-2 element is a panel."""
+    lines = open(path).read().split("\n")
+    nodes, tris, i = {}, [], 0
+    while i < len(lines):
+        tag = lines[i].strip()
+        if tag == "$Nodes":
+            n = int(lines[i + 1])
+            for ln in lines[i + 2:i + 2 + n]:
+                a = ln.split()
+                nodes[int(a[0])] = [float(x) for x in a[1:4]]
+            i += n + 2
+        elif tag == "$Elements":
+            n = int(lines[i + 1])
+            for ln in lines[i + 2:i + 2 + n]:
+                a = [int(x) for x in ln.split()]
+                if a[1] == 2:
+                    tris.append(a[3 + a[2]:6 + a[2]])
+            i += n + 2
+        else:
+            i += 1
+    keys = list(nodes)
+    pos = {k: j for j, k in enumerate(keys)}
+    V = np.array([nodes[k] for k in keys])
+    T = np.array([[pos[t] for t in tri] for tri in tris], dtype=np.int32)
+    return V, T

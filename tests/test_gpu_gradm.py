@@ -56,12 +56,16 @@ def test_block_matches_full_matrix(sx, O):
     B10 = G.AssembleBEMMatrixBlock(1, 0, 1.0).numpy()
     assert np.array_equal(B01, M[o0:o0 + n0, o1:o1 + n1])
     assert np.array_equal(B10, M[o1:o1 + n1, o0:o0 + n0])
+    # diagonal (T) blocks, self terms and near pairs included
+    for ns, (o, n) in enumerate([(o0, n0), (o1, n1)]):
+        T = G.AssembleBEMMatrixBlock(ns, ns, 1.0).numpy()
+        assert np.array_equal(T, M[o:o + n, o:o + n])
 
 
-def test_block_rejects_same_surface(sx, O):
+def test_block_rejects_same_surface_gradient(sx, O):
     G, _ = pair(sx, O, 0, (PEC, PEC), (0.0, 0.0, 3.0))
     with pytest.raises(sx.SCUFFXError):
-        G.AssembleBEMMatrixBlock(1, 1, 1.0)
+        G.AssembleBEMMatrixBlock(1, 1, 1.0, GradM=True)
 
 
 @pytest.mark.parametrize("mats", [(PEC, DIEL), (DIEL, DIEL)])
