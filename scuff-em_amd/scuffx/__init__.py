@@ -35,6 +35,12 @@ def lib():
     if _LIB is None:
         if not os.path.exists(_LIBPATH):
             raise ImportError("libscuffx.so not built (run __graft_entry__.build() or make -C scuff-em_amd)")
+        # torch bundles its own libamdhip64.so.7 with the same soname as the
+        # system one this library links: whichever is loaded first serves
+        # the whole process.  torch has to come first so that it runs on
+        # the HIP runtime it was built against; the library is ABI
+        # compatible with either.
+        import torch  # noqa: F401
         L = C.CDLL(_LIBPATH)
         vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
         sig = {

@@ -59,7 +59,11 @@ def test_integrand_matches_oracle(sx, O, tmp_path, kind, material, eps):
     M = Go.assemble(1j * xi)
     _, GB, TB = Go.assemble_block_grad(1j * xi, 0, 1, gammas=[sx.CreateGammaMatrix(a) for a in axes])
     ld = [np.linalg.slogdet(M[o:o + n, o:o + n])[1] for o, n in ((0, n0), (n0, M.shape[0] - n0))]
-    energy = -(sum(ld) - np.linalg.slogdet(M)[1]) / (2 * np.pi)
+    ldm = np.linalg.slogdet(M)[1]
+    energy = -(sum(ld) - ldm) / (2 * np.pi)
+    # the energy is a difference of log determinants of size ~N: its
+    # rounding error scales with their magnitude, not with the difference
+    assert abs(EFT[0] - energy) <= 1e-14 * (abs(ldm) + sum(abs(x) for x in ld)) / (2 * np.pi)
 
     def force(dU):
         dM = np.zeros((M.shape[0], n0), dtype=complex)
@@ -67,9 +71,9 @@ def test_integrand_matches_oracle(sx, O, tmp_path, kind, material, eps):
         X = np.linalg.solve(M, dM)
         return -2.0 * np.trace(X[:n0, :]).real / (2 * np.pi)
 
-    want = np.array([energy, force(GB[0]), force(GB[2]), force(TB[0]), force(TB[1])])
+    want = np.array([force(GB[0]), force(GB[2]), force(TB[0]), force(TB[1])])
     scale = np.abs(want).max()
-    assert np.all(np.abs(EFT - want) <= 1e-10 * scale), (EFT, want)
+    assert np.all(np.abs(EFT[1:] - want) <= 1e-10 * scale), (EFT, want)
 
 
 @pytest.mark.parametrize("name,material,sep", [

@@ -42,8 +42,17 @@ def test_block_and_gradm_match_oracle(sx, O, level, mats, d, omega, order):
     eb = rel_fro(B.numpy(), Bo)
     eg = [rel_fro(GB[m].numpy(), GBo[m]) for m in range(3)]
     print(f"  block {eb:.2e} grad {max(eg):.2e}", flush=True)
-    assert eb <= 1e-12
     assert max(eg) <= 1e-12
+    if nsa < nsb:
+        assert eb <= 1e-12
+    else:
+        # the (1,0) block is the transpose of the (0,1) block, as the
+        # lower blocks of the full matrix are (AssembleBEMMatrix.cc:753-757);
+        # a direct (1,0) evaluation differs from that only by the adaptive
+        # FIPPI tolerance of the near pairs taken in the other order
+        Bt, _, _ = Go.assemble_block_grad(omega, nsb, nsa)
+        assert rel_fro(B.numpy(), Bt.T) <= 1e-12
+        assert eb <= 1e-8
 
 
 def test_block_matches_full_matrix(sx, O):
