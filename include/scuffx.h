@@ -122,6 +122,11 @@ int scuffx_trace_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev
 /* ---- mixed precision: complex64 LU of a complex128 matrix plus
         complex128 iterative refinement (no counterpart in the reference;
         the plain path above is the drop-in) ---- */
+/* LU32 = complex64 copy of M, factored in place with partial pivoting
+   (ipiv 1-based, LAPACK layout); the block-row triangular solves use the
+   explicit inverses of the 32x32 unit-lower panel blocks, so the factors
+   carry slightly larger rounding than scuffx_lu_factor(prec=32).  Returns
+   the first zero pivot or 0. */
 int scuffx_lu32_from_c128(const void *M_dev, long n, long ldm, void *LU32_dev, long ld32,
                           int *ipiv_dev);
 /* x = M^{-1} b refined until |b - M x|_inf <= tol |b|_inf or maxiter steps */

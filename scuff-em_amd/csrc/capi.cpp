@@ -220,6 +220,15 @@ int scuffx_assemble_stats(const scuffx_geom *G, double *ms, long *counts)
   return 0;
 }
 
+static int lu_status(int rc, int info)
+{
+  if (rc == -5) return fail(SCUFFX_ERR_UNSUPPORTED, "matrix too large for a co-resident panel grid");
+  if (rc == -6) return fail(SCUFFX_ERR_HIP, "panel grid barrier timed out");
+  if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "LU workspace allocation failed");
+  if (rc) return fail(SCUFFX_ERR_HIP, "LU factorization failed on the device");
+  return info;
+}
+
 int scuffx_lu_factor(void *A, long n, long lda, int *ipiv, int prec)
 {
   if (!A || !ipiv || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad LU arguments");
@@ -227,11 +236,7 @@ int scuffx_lu_factor(void *A, long n, long lda, int *ipiv, int prec)
   if (prec == 64) rc = lu_factor<double2>((double2 *)A, n, lda, ipiv, &info, g_stream);
   else if (prec == 32) rc = lu_factor<float2>((float2 *)A, n, lda, ipiv, &info, g_stream);
   else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
-  if (rc == -5) return fail(SCUFFX_ERR_UNSUPPORTED, "matrix too large for a co-resident panel grid");
-  if (rc == -6) return fail(SCUFFX_ERR_HIP, "panel grid barrier timed out");
-  if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "LU workspace allocation failed");
-  if (rc) return fail(SCUFFX_ERR_HIP, "LU factorization failed on the device");
-  return info;
+  return lu_status(rc, info);
 }
 
 int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char trans, long nrhs, void *B, long ldb, int prec)
@@ -281,7 +286,9 @@ int scuffx_lu32_from_c128(const void *M, long n, long ldm, void *LU32, long ld32
 {
   if (!M || !LU32 || !ipiv || ldm < n || ld32 < n) return fail(SCUFFX_ERR_ARG, "bad arguments");
   if (convert_c128_to_c64((const double2 *)M, (float2 *)LU32, n, n, ldm, ld32, g_stream)) return fail(SCUFFX_ERR_HIP, "conversion failed");
-  return scuffx_lu_factor(LU32, n, ld32, ipiv, 32);
+  int info = 0;
+  int rc = lu_factor<float2>((float2 *)LU32, n, ld32, ipiv, &info, g_stream, 256, 1);
+  return lu_status(rc, info);
 }
 
 int scuffx_refine(const void *M, long n, long ldm, const void *LU32, long ld32, const int *ipiv,

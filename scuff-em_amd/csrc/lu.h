@@ -11,9 +11,13 @@
 
 namespace scuffx {
 
-// T = double2 (complex128) or float2 (complex64)
+// T = double2 (complex128) or float2 (complex64).  fast = 1 solves the
+// block rows with the explicit inverses of the 32x32 unit-lower panel
+// blocks (products instead of substitution): same pivots, slightly larger
+// rounding; meant for preconditioner factorizations.
 template <typename T>
-int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb = 256);
+int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb = 256,
+              int fast = 0);
 
 // solves op(A) X = B with the factors of lu_factor; trans 'N', 'T', 'C'
 template <typename T>

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -26,6 +27,7 @@
 namespace cg = cooperative_groups;
 
 namespace scuffx {
+
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -52,6 +54,7 @@ template <> struct CT<float2> {
 
 template <typename T> __device__ inline T cmul(T a, T b) { T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r; }
 template <typename T> __device__ inline T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename T> __device__ inline T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
 template <typename T> __device__ inline T crecip(T a)
 {
   // Smith's algorithm for 1/a
@@ -154,144 +157,14 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       }
 }
 
-// Large-tile variant: 128 x (32 NT) per workgroup, each wave 64 x (16 NT)
-// as 4 x NT MFMA tiles, the next K slab prefetched into registers while
-// the current one is consumed from LDS.  NT = 4 for complex64 (128x128
-// tiles); NT = 2 for complex128, whose accumulators are twice as wide.
-#define HBM_ 128
-#define HBK_ 16
-static const bool kUseBigGemm = false;
-template <typename T> struct BigNT;
-template <> struct BigNT<double2> { static const int v = 2; };
-template <> struct BigNT<float2> { static const int v = 4; };
-
-template <typename T, bool ACC>
-__global__ void __launch_bounds__(256)
-k_gemm_big(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc)
-{
-  typedef typename CT<T>::R R;
-  typedef typename CT<T>::V4 V4;
-  constexpr int NT = BigNT<T>::v, BN = 32 * NT, NB = BN / 16;   // NB: B elements per thread
-  __shared__ R sAr[HBK_][HBM_ + 4], sAi[HBK_][HBM_ + 4], sBr[HBK_][BN + 4], sBi[HBK_][BN + 4];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const long row0 = (long)blockIdx.x * HBM_, col0 = (long)blockIdx.y * BN;
-  V4 accr[4][NT], acci[4][NT];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < NT; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); }
-  // per thread: 8 A elements (row tid%128, k = tid/128 + 2s) and
-  // NB B elements (k = tid%16, col tid/16 + 16s)
-  T pa[8], pb[NB];
-  const bool interior = row0 + HBM_ <= m && col0 + BN <= n;
-  auto gload = [&](long k0) {
-    if (interior && k0 + HBK_ <= k) {
-#pragma unroll
-      for (int s2 = 0; s2 < 8; s2++) pa[s2] = A[(row0 + (tid & 127)) + (k0 + (tid >> 7) + 2 * s2) * lda];
-#pragma unroll
-      for (int s2 = 0; s2 < NB; s2++) pb[s2] = B[(k0 + (tid & 15)) + (col0 + (tid >> 4) + 16 * s2) * ldb];
-      return;
-    }
-#pragma unroll
-    for (int s2 = 0; s2 < 8; s2++) {
-      int i = tid & 127, kk = (tid >> 7) + 2 * s2;
-      long gi = row0 + i, gk = k0 + kk;
-      if (gi < m && gk < k) pa[s2] = A[gi + gk * lda]; else { pa[s2].x = 0; pa[s2].y = 0; }
-    }
-#pragma unroll
-    for (int s2 = 0; s2 < NB; s2++) {
-      int kb = tid & 15, j = (tid >> 4) + 16 * s2;
-      long gkb = k0 + kb, gj = col0 + j;
-      if (gkb < k && gj < n) pb[s2] = B[gkb + gj * ldb]; else { pb[s2].x = 0; pb[s2].y = 0; }
-    }
-  };
-  auto sstore = [&]() {
-#pragma unroll
-    for (int s2 = 0; s2 < 8; s2++) {
-      int i = tid & 127, kk = (tid >> 7) + 2 * s2;
-      sAr[kk][i] = pa[s2].x; sAi[kk][i] = pa[s2].y;
-    }
-#pragma unroll
-    for (int s2 = 0; s2 < NB; s2++) {
-      int kb = tid & 15, j = (tid >> 4) + 16 * s2;
-      sBr[kb][j] = pb[s2].x; sBi[kb][j] = pb[s2].y;
-    }
-  };
-  gload(0);
-  sstore();
-  __syncthreads();
-  for (long k0 = 0; k0 < k; k0 += HBK_) {
-    const bool more = k0 + HBK_ < k;
-    if (more) gload(k0 + HBK_);
-#pragma unroll
-    for (int kk0 = 0; kk0 < HBK_; kk0 += 4) {
-      const int kk = kk0 + (lane >> 4);
-      R ar[4], ai[4], br[NT], bi[NT];
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        ar[t] = sAr[kk][wr * 64 + t * 16 + (lane & 15)];
-        ai[t] = sAi[kk][wr * 64 + t * 16 + (lane & 15)];
-      }
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        br[t] = sBr[kk][wc * 16 * NT + t * 16 + (lane & 15)];
-        bi[t] = sBi[kk][wc * 16 * NT + t * 16 + (lane & 15)];
-      }
-      const R sg = ACC ? (R)-1 : (R)1;
-#pragma unroll
-      for (int ti = 0; ti < 4; ti++) {
-        const R a_r = sg * ar[ti], a_i = sg * ai[ti], na_i = -a_i;
-#pragma unroll
-        for (int tj = 0; tj < NT; tj++) {
-          accr[ti][tj] = CT<T>::mfma(a_r, br[tj], accr[ti][tj]);
-          accr[ti][tj] = CT<T>::mfma(na_i, bi[tj], accr[ti][tj]);
-          acci[ti][tj] = CT<T>::mfma(a_r, bi[tj], acci[ti][tj]);
-          acci[ti][tj] = CT<T>::mfma(a_i, br[tj], acci[ti][tj]);
-        }
-      }
-    }
-    __syncthreads();
-    if (more) {
-      sstore();
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int ti = 0; ti < 4; ti++)
-#pragma unroll
-    for (int tj = 0; tj < NT; tj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        long gi = row0 + wr * 64 + ti * 16 + CT<T>::row(lane, r);
-        long gj = col0 + wc * 16 * NT + tj * 16 + (lane & 15);
-        if (gi < m && gj < n) {
-          T c;
-          if (ACC) { c = C[gi + gj * ldc]; c.x += accr[ti][tj][r]; c.y += acci[ti][tj][r]; }
-          else { c.x = accr[ti][tj][r]; c.y = acci[ti][tj][r]; }
-          C[gi + gj * ldc] = c;
-        }
-      }
-}
-
 template <typename T, bool ACC>
 static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
                  hipStream_t s)
 {
   if (m <= 0 || n <= 0 || k <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
-  // large tiles once there are enough of them to fill the chip
-  constexpr long BN = 32 * BigNT<T>::v;
-  long big_tiles = ((m + HBM_ - 1) / HBM_) * ((n + BN - 1) / BN);
-  // (measured slower than the 64x64 kernel on gfx950 at one wave per
-  // SIMD; kept for tuning, off by default)
-  if (kUseBigGemm && big_tiles >= 256 && k >= 64) {
-    dim3 grid((unsigned)((m + HBM_ - 1) / HBM_), (unsigned)((n + BN - 1) / BN));
-    hipLaunchKernelGGL((k_gemm_big<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
-  } else {
-    dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
-    hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
-  }
+  dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
+  hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
   prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
 }
 
@@ -322,49 +195,69 @@ template <> struct PanelRows<float2> { static const int n = 256; };
 struct PanelScratch {
   unsigned *bar;   // grid barrier: [0] arrivals, [32] generation, [64] timeout flag (own cache lines)
   double *candv;   // [2][Gmax]
-  int *candi;      // [2][Gmax]
+  unsigned long long *ckey;   // [2][Gmax]: (local candidate row << 32) | column sequence number
   void *candrow;   // [2][Gmax][PW] T
   void *rowc;      // [2][PW] T
   int *plan;       // [2*PW+1]: dst rows, src rows, count of the panel's net row permutation
+  void *linv;      // [npanels][PW][PW] T or null: inverse of each panel's unit-lower block (column-major)
 };
 
-// Grid barrier among the co-resident workgroups of a cooperative launch:
-// release fence, one relaxed arrival counter, relaxed polling of a
-// generation word, acquire fence.  (Polling with acquire loads would
-// invalidate the L2 on every poll; this costs ~4 us at 120 workgroups
-// where the generic grid barrier costs ~17 us, measured on gfx950.)
-// Every wait is bounded: past GBAR_SPIN polls the timeout flag is set,
-// and once it is set no workgroup waits any more, so the kernel always
-// finishes; the host turns the flag into an error.
-#define GBAR_SPIN (1u << 20)
-__device__ inline void gbar(unsigned *bar, int G, unsigned &gen)
+
+// Cross-workgroup exchange inside the panel without grid barriers: the
+// shared words are read and written only by agent-scope relaxed atomics
+// (coherent across the XCDs' L2s), and a writer waits for its data
+// stores to complete (s_waitcnt) before it stores the sequence-tagged key
+// that readers poll.  No release/acquire fences, hence no L2 write-back or
+// invalidation while the trailing GEMM runs beside the panel.
+__device__ inline void st_coh(unsigned long long *p, unsigned long long v)
 {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned *cnt = bar, *gw = bar + 32, *err = bar + 64;
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      unsigned o = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (o == (unsigned)G - 1) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(gw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      unsigned spins = 0;
-      while (__hip_atomic_load(gw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (++spins > GBAR_SPIN) { __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-  }
-  gen++;
-  __syncthreads();
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long ld_coh(const unsigned long long *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_coh(float2 *p, float2 v)
+{
+  unsigned long long u = ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+  st_coh((unsigned long long *)p, u);
+}
+__device__ inline float2 ld_coh(const float2 *p)
+{
+  unsigned long long u = ld_coh((const unsigned long long *)p);
+  float2 v;
+  v.x = __uint_as_float((unsigned)u);
+  v.y = __uint_as_float((unsigned)(u >> 32));
+  return v;
+}
+__device__ inline void st_coh(double2 *p, double2 v)
+{
+  st_coh((unsigned long long *)p, (unsigned long long)__double_as_longlong(v.x));
+  st_coh((unsigned long long *)p + 1, (unsigned long long)__double_as_longlong(v.y));
+}
+__device__ inline double2 ld_coh(const double2 *p)
+{
+  double2 v;
+  v.x = __longlong_as_double((long long)ld_coh((const unsigned long long *)p));
+  v.y = __longlong_as_double((long long)ld_coh((const unsigned long long *)p + 1));
+  return v;
+}
+// polls before a wait gives up (about a second); the kernel then
+// finishes and the host reports the time-out
+#define GBAR_SPIN (1u << 20)
+
+// every memory operation this thread issued has completed; the signal
+// fences keep the compiler from moving memory operations across it
+__device__ inline void drain()
+{
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(PanelRows<T>::n)
-k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax)
+k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax, unsigned seq0)
 {
   typedef decltype(T().x) R;
   constexpr int NR = PanelRows<T>::n;
@@ -373,15 +266,15 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
   __shared__ int wi[NR / 64];
   __shared__ T spiv[PW], sold[PW];
   __shared__ long spv[PW];
-  __shared__ int s_best, s_p;
+  __shared__ int s_p;
   __shared__ R s_bv;
   const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
   const long lo = r0 + (long)g * NR;
   const int nrow = (int)std::min<long>(NR, r0 + m - lo);
   const long i = lo + t;                 // this thread's row
   T *candrow = (T *)S.candrow, *rowc = (T *)S.rowc;
-  // every workgroup reads the generation before any can complete a barrier
-  unsigned gen = __hip_atomic_load(S.bar + 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned *err = S.bar + 64;
+  __shared__ int s_gw;
   if (t < nrow) {
     T v[PW];
     if (w == PW) {
@@ -397,6 +290,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
   for (int j = 0; j < w; j++) {
     const long c = r0 + j;
     const int buf = j & 1;
+    const unsigned seq = seq0 + (unsigned)j;
     // local candidate: largest |re|+|im| among own rows >= c, first on ties
     R best = -1; int bi = -1;
     if (t < nrow && i >= c) { T v = slab[j][t]; best = fabs(v.x) + fabs(v.y); bi = t; }
@@ -407,40 +301,58 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
     }
     if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
     __syncthreads();
-    if (t == 0) {
-      R b = wv[0]; int bb = wi[0];
-      for (int q = 1; q < NR / 64; q++)
-        if (wv[q] > b || (wv[q] == b && wi[q] >= 0 && (bb < 0 || wi[q] < bb))) { b = wv[q]; bb = wi[q]; }
-      s_best = bb;
-      S.candv[buf * Gmax + g] = (double)b;
-      S.candi[buf * Gmax + g] = bb >= 0 ? (int)(lo + bb) : -1;
-    }
-    __syncthreads();
+    R b0 = wv[0]; int bb = wi[0];
+    for (int q = 1; q < NR / 64; q++)
+      if (wv[q] > b0 || (wv[q] == b0 && wi[q] >= 0 && (bb < 0 || wi[q] < bb))) { b0 = wv[q]; bb = wi[q]; }
+    // publish: candidate row (and row c, by its owner), then value, then key
     if (t < w) {
-      if (s_best >= 0) candrow[((long)buf * Gmax + g) * PW + t] = slab[t][s_best];
-      if (c >= lo && c < lo + nrow) rowc[buf * PW + t] = slab[t][c - lo];
+      if (bb >= 0) st_coh(candrow + ((long)buf * Gmax + g) * PW + t, slab[t][bb]);
+      if (c >= lo && c < lo + nrow) st_coh(rowc + buf * PW + t, slab[t][c - lo]);
     }
-    gbar(S.bar, G, gen);
-    // global pivot
+    if (t == 0)
+      st_coh((unsigned long long *)S.candv + buf * Gmax + g, (unsigned long long)__double_as_longlong((double)b0));
+    drain();
+    __syncthreads();
+    if (t == 0)
+      st_coh(S.ckey + buf * Gmax + g, ((unsigned long long)(unsigned)(bb >= 0 ? bb : -1) << 32) | seq);
+    // global pivot: wave 0 waits for every workgroup's key of this column
     if (t < 64) {
-      R b = -1; int p = -1, gw = -1;
-      for (int q = t; q < G; q += 64) {
-        R v = (R)S.candv[buf * Gmax + q]; int ii = S.candi[buf * Gmax + q];
-        if (ii < 0) continue;
+      // bounded: past GBAR_SPIN polls the error word is set, and once it
+      // is set no workgroup waits again (the panel then finishes with
+      // meaningless values and the host reports the time-out)
+      unsigned spins = 0;
+      for (;;) {
+        bool ok = true;
+        for (int q = lane; q < G; q += 64)
+          if ((unsigned)ld_coh(S.ckey + buf * Gmax + q) != seq) ok = false;
+        if (__ballot(!ok) == 0) break;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (++spins > GBAR_SPIN) { __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      R b = -1; long p = -1; int gw = -1;
+      for (int q = lane; q < G; q += 64) {
+        unsigned long long k = ld_coh(S.ckey + buf * Gmax + q);
+        int li = (int)(unsigned)(k >> 32);
+        if (li < 0) continue;
+        R v = (R)__longlong_as_double((long long)ld_coh((const unsigned long long *)S.candv + buf * Gmax + q));
+        long ii = r0 + (long)q * NR + li;
         if (v > b || (v == b && ii < p)) { b = v; p = ii; gw = q; }
       }
       for (int off = 32; off > 0; off >>= 1) {
         R ob = __shfl_xor(b, off);
-        int op = __shfl_xor(p, off), og = __shfl_xor(gw, off);
+        long op = __shfl_xor(p, off);
+        int og = __shfl_xor(gw, off);
         if (op >= 0 && (ob > b || (ob == b && (p < 0 || op < p)))) { b = ob; p = op; gw = og; }
       }
-      if (t == 0) { s_p = p < 0 ? (int)c : p; s_best = gw; s_bv = b; }
+      if (t == 0) { s_p = p < 0 ? (int)c : (int)p; s_gw = gw; s_bv = b; }
     }
     __syncthreads();
     const long p = s_p;
     if (t < w) {
-      sold[t] = rowc[buf * PW + t];
-      spiv[t] = s_best >= 0 ? candrow[((long)buf * Gmax + s_best) * PW + t] : sold[t];
+      sold[t] = ld_coh(rowc + buf * PW + t);
+      spiv[t] = s_gw >= 0 ? ld_coh(candrow + ((long)buf * Gmax + s_gw) * PW + t) : sold[t];
     }
     if (t == 0) spv[j] = p;
     if (g == 0 && t == 0) {
@@ -464,6 +376,23 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
   }
   for (int q = 0; q < w; q++)
     if (t < nrow) A[i + (r0 + q) * lda] = slab[q][t];
+  // the inverse of the panel's unit-lower w x w block (workgroup 0 holds
+  // its rows), column t by substitution, identity outside w x w
+  if (S.linv && g == 0 && t < PW) {
+    T x[PW];
+#pragma unroll
+    for (int r = 0; r < PW; r++) {
+      T v; v.x = (r == t) ? 1 : 0; v.y = 0;
+      if (r < w && t < w) {
+#pragma unroll
+        for (int k = 0; k < r; k++) v = csub(v, cmul(slab[k][r], x[k]));
+      }
+      x[r] = v;
+    }
+    T *li = (T *)S.linv + (r0 / PW) * PW * PW + (long)t * PW;
+#pragma unroll
+    for (int r = 0; r < PW; r++) li[r] = x[r];
+  }
   // net permutation of the panel's w interchanges on the rows they touch,
   // one row per lane of wave 0: lanes 0..w-1 are rows r0.., pivot rows
   // from below are appended.  After the interchanges row dst[q] holds
@@ -653,8 +582,91 @@ static void trsm_block(const T *A, long lda, long r0, int nb, T *B, long ldb, lo
   }
 }
 
+// B[w x ncol] <- Linv B for a w x w (w <= PW) inverse of a unit-lower
+// panel block, in place: each workgroup stages Linv and a 64-column tile
+// of B in LDS; thread (row, column group) forms one row of 8 columns
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_trmm_strip(const T *Linv, int w, T *B, long ldb, long ncol)
+{
+  __shared__ T Ls[PW][PW + 1];
+  __shared__ T Bt[PW][64 + 1];
+  const int t = threadIdx.x;
+  const long c0 = (long)blockIdx.x * 64;
+  const bool full = (w == PW && c0 + 64 <= ncol);
+  {
+    T lv[4], bv[8];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int e = t + 256 * u, r = e & (PW - 1), k = e / PW;
+      lv[u] = Linv[r + k * PW];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      int e = t + 256 * u, r = e & (PW - 1), c = e / PW;
+      if (full || (r < w && c0 + c < ncol)) bv[u] = B[r + (c0 + c) * ldb];
+      else { bv[u].x = 0; bv[u].y = 0; }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int e = t + 256 * u, r = e & (PW - 1), k = e / PW;
+      Ls[r][k] = lv[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      int e = t + 256 * u, r = e & (PW - 1), c = e / PW;
+      Bt[r][c] = bv[u];
+    }
+  }
+  __syncthreads();
+  const int r = t & (PW - 1), cg = t >> 5;
+  T acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) { acc[u].x = 0; acc[u].y = 0; }
+#pragma unroll 8
+  for (int k = 0; k < PW; k++) {
+    const T a = Ls[r][k];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc[u] = cadd(acc[u], cmul(a, Bt[k][cg * 8 + u]));
+  }
+  if (r < w) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      long c = c0 + cg * 8 + u;
+      if (c < ncol) B[r + c * ldb] = acc[u];
+    }
+  }
+}
+
+template <typename T>
+static void trmm_strip(const T *Linv, int w, T *B, long ldb, long ncol, hipStream_t s)
+{
+  if (ncol <= 0 || w <= 0) return;
+  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
+  hipLaunchKernelGGL(k_trmm_strip<T>, dim3((unsigned)((ncol + 63) / 64)), dim3(256), 0, s, Linv, w, B, ldb, ncol);
+  prof_end(s, tok, 4.0 * (double)w * (double)w * (double)ncol);
+}
+
+// trsm_block with the panels' block inverses: per 32-row step a product
+// with the step's inverse instead of substitution, then the GEMM update
+template <typename T>
+static void trsm_block_inv(const T *A, long lda, long r0, int nb, const T *linv, T *B, long ldb, long ncol,
+                           hipStream_t s)
+{
+  for (int i0 = 0; i0 < nb; i0 += PW) {
+    int w = std::min(PW, nb - i0);
+    trmm_strip(linv + (long)(i0 / PW) * PW * PW, w, B + i0, ldb, ncol, s);
+    if (i0 + w < nb)
+      gemm_sub((long)(nb - i0 - w), ncol, (long)w, A + (r0 + i0 + w) + (r0 + i0) * lda, lda, B + i0, ldb,
+               B + i0 + w, ldb, s);
+  }
+}
+
 // second stream of the lookahead (default priority, non-blocking), one per
 // device, created on first use
+// second stream of the lookahead (default priority, non-blocking), one per
+// device, created on first use.  (Keeping its kernels off a share of the
+// CUs with a CU-masked stream was measured 1.7x slower on gfx950.)
 static hipStream_t side_stream()
 {
   static hipStream_t st[64] = {nullptr};
@@ -674,7 +686,7 @@ static hipStream_t side_stream()
 // left of a block are applied at the end (nothing reads those rows in
 // between), so the side stream's GEMM never sees its L21 rows move.
 template <typename T>
-int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb)
+int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb, int fast)
 {
   if (n < 0 || lda < std::max(1L, n)) return -1;
   if (n == 0) return 0;
@@ -683,10 +695,11 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   const int ib = PW;
   constexpr int NR = PanelRows<T>::n;
   const long Gmax = (n + NR - 1) / NR;
+  const void *panel_fn = (const void *)k_panel<T>;
   int dev = 0, ncu = 0, perCU = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_panel<T>, NR, 0);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, panel_fn, NR, 0);
   if ((long)std::max(1, ncu) * std::max(1, perCU) < Gmax) return -5;   // panel would not be co-resident
   const long npanels = (n + PW - 1) / PW;
   // panel scratch, carved from one workspace: barrier words (3 cache
@@ -696,16 +709,18 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   size_t off_info = 96 * sizeof(unsigned), off_plan = off_info + 64;
   size_t off_cv = off_plan + 256 * ((sizeof(int) * PLAN_LEN * npanels + 255) / 256);
   size_t off_ci = off_cv + 256 * ((sizeof(double) * 2 * Gmax + 255) / 256);
-  size_t off_cr = off_ci + 256 * ((sizeof(int) * 2 * Gmax + 255) / 256);
+  size_t off_cr = off_ci + 256 * ((sizeof(unsigned long long) * 2 * Gmax + 255) / 256);
   size_t off_rc = off_cr + 256 * ((sizeof(T) * 2 * Gmax * PW + 255) / 256);
-  size_t total = off_rc + sizeof(T) * 2 * PW;
+  size_t off_li = off_rc + 256 * ((sizeof(T) * 2 * PW + 255) / 256);
+  size_t total = off_li + (fast ? sizeof(T) * PW * PW * npanels : 0);
   char *wsb = (char *)ws_get(WS_PANEL, total);
   if (!wsb) return -2;
+  S.linv = fast ? (void *)(wsb + off_li) : nullptr;
   S.bar = (unsigned *)wsb;
   int *dinfo = (int *)(wsb + off_info);
   int *plans = (int *)(wsb + off_plan);
   S.candv = (double *)(wsb + off_cv);
-  S.candi = (int *)(wsb + off_ci);
+  S.ckey = (unsigned long long *)(wsb + off_ci);
   S.candrow = (void *)(wsb + off_cr);
   S.rowc = (void *)(wsb + off_rc);
   hipStream_t side = side_stream();
@@ -714,6 +729,8 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   hipEventCreateWithFlags(&evR, hipEventDisableTiming);
   hipMemsetAsync(dinfo, 0, sizeof(int), stream);
   hipMemsetAsync(S.bar, 0, sizeof(unsigned) * 96, stream);
+  // candidate keys start at sequence 0, which no column uses
+  hipMemsetAsync(S.ckey, 0, sizeof(unsigned long long) * 2 * Gmax, stream);
   // the side stream starts after the caller's stream has the matrix ready
   hipEventRecord(evR, stream);
   hipStreamWaitEvent(side, evR, 0);
@@ -728,17 +745,22 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       long G = (m + NR - 1) / NR;
       long r0 = j0;
       S.plan = plans + (j0 / PW) * PLAN_LEN;
+      // column sequence numbers: fresh for every panel of this factorization
+      unsigned seq0 = (unsigned)(j0 + 1);
       void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
-                      (void *)&dinfo, (void *)&S, (void *)&gmaxi};
+                      (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0};
       int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
-      hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3((unsigned)G), dim3(NR), args, 0, stream);
+      hipError_t e = hipLaunchCooperativeKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream);
       prof_end(stream, tok, 0.0);
       if (e != hipSuccess) { rc = -3; break; }
       // this panel's interchanges on the block's other columns
       laswp_plans(A, lda, K, K + jb, j0, j0 + w, S.plan, 1, stream);
       long nr = K + jb - (j0 + w);
       if (nr > 0) {
-        trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
+        if (fast)
+          trmm_strip((const T *)S.linv + (j0 / PW) * PW * PW, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
+        else
+          trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
         gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
                  A + (j0 + w) + (j0 + w) * lda, lda, stream);
       }
@@ -753,14 +775,20 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
     if (nnext > 0) {
       hipStreamWaitEvent(stream, evR, 0);
       laswp_plans(A, lda, c1, c2, 0, 0, plans + p0 * PLAN_LEN, bplans, stream);
-      trsm_block(A, lda, K, (int)jb, A + K + c1 * lda, lda, nnext, stream);
+      if (fast)
+        trsm_block_inv(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c1 * lda, lda, nnext, stream);
+      else
+        trsm_block(A, lda, K, (int)jb, A + K + c1 * lda, lda, nnext, stream);
       gemm_sub(n - c1, nnext, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, stream);
     }
     // the rest of the trailing matrix (side stream)
     if (c2 < n) {
       hipStreamWaitEvent(side, evP, 0);
       laswp_plans(A, lda, c2, n, 0, 0, plans + p0 * PLAN_LEN, bplans, side);
-      trsm_block(A, lda, K, (int)jb, A + K + c2 * lda, lda, n - c2, side);
+      if (fast)
+        trsm_block_inv(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c2 * lda, lda, n - c2, side);
+      else
+        trsm_block(A, lda, K, (int)jb, A + K + c2 * lda, lda, n - c2, side);
       gemm_sub(n - c1, n - c2, jb, A + c1 + K * lda, lda, A + K + c2 * lda, lda, A + c1 + c2 * lda, lda, side);
       hipEventRecord(evR, side);
     }
@@ -1088,8 +1116,8 @@ template int lu_logdet<double2>(const double2 *, long, long, const int *, double
 template int lu_logdet<float2>(const float2 *, long, long, const int *, double *, hipStream_t);
 template int mat_trace<double2>(const double2 *, long, long, double *, hipStream_t);
 template int mat_trace<float2>(const float2 *, long, long, double *, hipStream_t);
-template int lu_factor<double2>(double2 *, long, long, int *, int *, hipStream_t, int);
-template int lu_factor<float2>(float2 *, long, long, int *, int *, hipStream_t, int);
+template int lu_factor<double2>(double2 *, long, long, int *, int *, hipStream_t, int, int);
+template int lu_factor<float2>(float2 *, long, long, int *, int *, hipStream_t, int, int);
 template int lu_solve<double2>(const double2 *, long, long, const int *, char, long, double2 *, long, hipStream_t);
 template int lu_solve<float2>(const float2 *, long, long, const int *, char, long, float2 *, long, hipStream_t);
 

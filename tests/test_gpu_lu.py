@@ -155,3 +155,39 @@ def test_mixed_refinement(sx, n):
     assert np.linalg.norm(xs - x_true) <= 1e-12 * np.linalg.norm(x_true)
     # the complex128 matrix is left untouched
     assert np.array_equal(M.numpy(), A)
+
+
+@pytest.mark.parametrize("n", [40, 260, 700, 2600])
+def test_lu32_mixed_factors(sx, O, n):
+    """complex64 factorization of the mixed solver (partial pivoting,
+    block-row solves through the panel blocks' inverses): ipiv is a valid
+    interchange sequence, P A = L U to complex64 accuracy, |L| <= sqrt 2, and
+    the pivots agree with the oracle's partial pivoting on the rounded
+    matrix except near ties.  n = 260 leaves a 4-row workgroup."""
+    import ctypes as C
+    import torch
+    print(f"lu32 mixed n={n}", flush=True)
+    A = rand(n, SEED + 13 * n)
+    M = sx.HMatrix.from_numpy(A)
+    LU32 = torch.zeros((n, n), dtype=torch.complex64, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = sx.lib().scuffx_lu32_from_c128(sx._ptr(M.t), n, n, sx._ptr(LU32), n, sx._ptr(ipiv))
+    assert info == 0
+    F = LU32.cpu().numpy().T.astype(np.complex128)
+    piv = ipiv.cpu().numpy()
+    assert np.all((piv >= np.arange(1, n + 1)) & (piv <= n))
+    L = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    perm = np.arange(n)
+    for i in range(n):
+        p = piv[i] - 1
+        perm[[i, p]] = perm[[p, i]]
+    A32 = A.astype(np.complex64).astype(np.complex128)
+    berr = np.linalg.norm(A32[perm] - L @ U) / np.linalg.norm(A32)
+    print(f"  backward error {berr:.2e}, max |L| {np.abs(L).max():.2f}", flush=True)
+    assert berr <= 2e-5 * np.sqrt(n / 40)
+    # pivots maximize |re|+|im| (izamax), so |l| <= sqrt(2) in modulus
+    assert np.abs(L).max() <= np.sqrt(2.0) + 1e-5
+    if n <= 700:
+        _, ipo, _ = O.zgetrf(A32)
+        assert np.mean(piv == ipo) > 0.95
