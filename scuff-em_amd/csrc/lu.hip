@@ -751,7 +751,11 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
                       (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0};
       int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
       hipError_t e = hipLaunchCooperativeKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream);
-      prof_end(stream, tok, 0.0);
+      // algorithmic flops of the m x w panel: per column, one complex
+      // reciprocal-multiply per row below it and a complex rank-1 update
+      double pflops = 0;
+      for (int j = 0; j < w; j++) pflops += (double)(m - j - 1) * (6.0 + 8.0 * (double)(w - j - 1));
+      prof_end(stream, tok, pflops);
       if (e != hipSuccess) { rc = -3; break; }
       // this panel's interchanges on the block's other columns
       laswp_plans(A, lda, K, K + jb, j0, j0 + w, S.plan, 1, stream);
