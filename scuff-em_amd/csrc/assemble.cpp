@@ -184,6 +184,7 @@ Assembler::~Assembler()
   hipFree(d_jobs_); hipFree(d_records_);
   hipFree(d_HP_); hipFree(d_HN_); hipFree(d_HS_); hipFree(d_scratch_); hipFree(d_stage_);
   hipFree(d_regs_);
+  hipFree(d_swjobs_);
 }
 
 int Assembler::ensure(double **p, size_t *cap, size_t need)
@@ -442,7 +443,7 @@ static int common_regions(const Surface &Sa, const Surface &Sb, int CR[2], doubl
 
 // regions shared by surfaces sa and sb, with wavenumber and the three
 // prefactors of the 2x2 block (SurfaceSurfaceInteractions.cc:62-90,
-// :162-176); -5 for a region in the short-wavelength regime
+// :162-176)
 int Assembler::regions_for(int sa, int sb, const std::vector<cdouble> &eps, const std::vector<cdouble> &mu,
                            cdouble omega, RegionK rk[2])
 {
@@ -459,18 +460,64 @@ int Assembler::regions_for(int sa, int sb, const std::vector<cdouble> &eps, cons
     rk[used].pf1 = mk(pf1.real(), pf1.imag());
     rk[used].pf2 = mk(pf2.real(), pf2.imag());
     rk[used].pf3 = mk(pf3.real(), pf3.imag());
-    // regimes handled on the GPU: every near pair in the
-    // desingularization regime |k| R_max <= pi (PanelPanelInteractions.cc:429-491)
-    double Rmax = 0;
-    for (auto &P : Sa.Panels) Rmax = std::max(Rmax, P.Radius);
-    for (auto &P : Sb.Panels) Rmax = std::max(Rmax, P.Radius);
-    if (std::abs(k) * Rmax > M_PI) {
-      err_ = "short-wavelength regime (|k| R_panel > pi) is not implemented on the GPU path";
-      return -5;
-    }
     used++;
   }
   return used;
+}
+
+// Short-wavelength panel pairs (PanelPanelInteractions.cc:426-491): a near
+// pair whose larger panel has |k| R > pi leaves the desingularization
+// branch -- no common vertex: 78-point product rule; common edge or
+// vertex: Taylor-Duffy.  (Self panels stay on Taylor-Duffy with the
+// Helmholtz kernel, which the self kernel already does, up to |k| R = 20.)
+// The near kernel has already written every record; these jobs overwrite
+// the affected ones.
+int Assembler::run_sw(int sa, int sb, int bi, const RegionK *rk, int nreg, const RegionK *dreg, int needH1,
+                      hipStream_t stream)
+{
+  const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
+  const NearBlock &B = blocks_[bi];
+  const int same = (sa == sb);
+  std::vector<SWJob> jobs;
+  std::vector<char> seen(B.pairs.size(), 0);
+  for (int r = 0; r < nreg; r++) {
+    const double ak = std::hypot(rk[r].k.r, rk[r].k.i);
+    std::fill(seen.begin(), seen.end(), 0);
+    for (int pa = 0; pa + 1 < (int)B.rowPtr.size(); pa++)
+      for (int e = B.rowPtr[pa]; e < B.rowPtr[pa + 1]; e++) {
+        const int id = B.idx[e];
+        if (id < 0 || seen[id]) continue;
+        seen[id] = 1;
+        const int pb = B.col[e];
+        const double kR = ak * std::max(Sa.Panels[pa].Radius, Sb.Panels[pb].Radius);
+        if (!(kR > M_PI)) continue;
+        const int ncv = B.pairs[id].ncv;
+        if (ncv == 0) { jobs.push_back(SWJob{id, r, 0, 0}); continue; }
+        if (kR > 20.0) {
+          err_ = "common-edge/vertex pair with |k| R_panel > 20: the reference's high-k gradient kernel "
+                 "has no defined value there (TaylorDuffy.cc:812)";
+          return -5;
+        }
+        for (int q = 0; q < 9; q++) jobs.push_back(SWJob{id, r, q, 1});
+      }
+    // a self panel past |k| R > 20 (high-k Taylor-Duffy) always comes
+    // with common-edge neighbours past 20 on a closed mesh, rejected above
+  }
+  if (jobs.empty()) return 0;
+  if (jobs.size() > cap_swjobs_) {
+    if (d_swjobs_) hipFree(d_swjobs_);
+    d_swjobs_ = nullptr;
+    cap_swjobs_ = 0;
+    if (hipMalloc(&d_swjobs_, sizeof(SWJob) * jobs.size()) != hipSuccess) { err_ = "device allocation failed"; return -2; }
+    cap_swjobs_ = jobs.size();
+  }
+  HIPCHK(hipMemcpy(d_swjobs_, jobs.data(), sizeof(SWJob) * jobs.size(), hipMemcpyHostToDevice));
+  const int chunk = 2048;
+  if (ensure(&d_scratch_, &cap_scratch_, (size_t)chunk * SCUFFX_SW_SLAB)) return -2;
+  HIPCHK(launch_sw(stream, d_swjobs_, (int)jobs.size(), B.d_pairs, ds_[sa].V, ds_[sa].PVI, nreg, dreg, needH1,
+                   d_HN_, d_HS_, d_scratch_, chunk));
+  stats_.nsw += (long)jobs.size();
+  return 0;
 }
 
 int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
@@ -486,7 +533,7 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
   for (auto &e : ev) hipEventCreate(&e);
   double t_self = 0, t_far = 0, t_near = 0, t_stamp = 0;
   int bi = 0;
-  stats_.nfar = stats_.nnear = stats_.nself = 0;
+  stats_.nfar = stats_.nnear = stats_.nself = stats_.nsw = 0;
   for (int sa = 0; sa < NS; sa++)
     for (int sb = sa; sb < NS; sb++, bi++) {
       const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
@@ -535,6 +582,13 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
         int tok = prof_begin(stream, PK_NEAR);
         HIPCHK(launch_near(stream, B.d_pairs, (int)B.pairs.size(), d_records_, nreg, dreg, needH1, d_HN_));
         prof_end(stream, tok, wpair * (double)B.pairs.size() * nreg);
+      }
+      if (nreg > 0) {
+        int rc2 = run_sw(sa, sb, bi, rk, nreg, dreg, needH1, stream);
+        if (rc2) {
+          for (auto &e : ev) hipEventDestroy(e);
+          return rc2;
+        }
       }
       hipEventRecord(ev[3], stream);
       {
@@ -650,6 +704,8 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
                       nreg, d_regs_, needH1, d_HP_));
     if (!NB.pairs.empty())
       HIPCHK(launch_near(stream, NB.d_pairs, (int)NB.pairs.size(), d_records_, nreg, d_regs_, needH1, d_HN_));
+    int rc2 = run_sw(lo, hi, bi, rk, nreg, d_regs_, needH1, stream);
+    if (rc2) return rc2;
   }
   HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same, Sa.IsPEC, Sb.IsPEC,
                       nreg, d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx, target, ldt, 0, 0));

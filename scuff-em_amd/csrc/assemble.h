@@ -31,7 +31,7 @@ struct NearBlock {
 
 struct AsmStats {
   double fippi_ms = 0, self_ms = 0, far_ms = 0, near_ms = 0, stamp_ms = 0, mirror_ms = 0;
-  long nfar = 0, nnear = 0, nself = 0, nfippi = 0;
+  long nfar = 0, nnear = 0, nself = 0, nfippi = 0, nsw = 0;
 };
 
 class Assembler {
@@ -63,12 +63,18 @@ class Assembler {
   double *d_HP_ = nullptr, *d_HN_ = nullptr, *d_HS_ = nullptr, *d_scratch_ = nullptr, *d_stage_ = nullptr;
   size_t cap_HP_ = 0, cap_HN_ = 0, cap_HS_ = 0, cap_scratch_ = 0, cap_stage_ = 0;
   RegionK *d_regs_ = nullptr;
+  SWJob *d_swjobs_ = nullptr;
+  size_t cap_swjobs_ = 0;
   AsmStats stats_;
   std::string err_;
   void build_near();
   int regions_for(int sa, int sb, const std::vector<cdouble> &eps, const std::vector<cdouble> &mu, cdouble omega,
                   RegionK rk[2]);
   int ensure(double **p, size_t *cap, size_t need);
+  // short-wavelength pairs of block bi at this frequency, launched after
+  // the near (and self) kernels; -5 for a pair with no defined value
+  int run_sw(int sa, int sb, int bi, const RegionK *rk, int nreg, const RegionK *dreg, int needH1,
+             hipStream_t stream);
 };
 
 }  // namespace scuffx

@@ -71,6 +71,17 @@ hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const doubl
                         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
                         double2 *M, long ldm, int rowOff, int colOff);
 hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n);
+// short-wavelength panel pairs (kern_sw.hip): kind 0 = near pair without a
+// common vertex (all 9 combinations, 78-point rules), 1 = common edge /
+// vertex near pair (Taylor-Duffy, combination q); pair = near-pair index
+struct SWJob {
+  int pair, region, q, kind;
+};
+// doubles of scratch per Taylor-Duffy job
+#define SCUFFX_SW_SLAB (3 * 6 * 5 * 22 + 4 * 48 + 2112 * 6)
+hipError_t launch_sw(hipStream_t stream, const SWJob *jobs, int njobs, const NearPair *nps, const double *V,
+                     const int *PVI, int nreg, const RegionK *regs, int needH1, double *HN, double *HS,
+                     double *scratch, int chunk);
 // rotation generators of up to three torque axes, Gamma[9 t + i + 3 j]
 struct GammaSet {
   int n;

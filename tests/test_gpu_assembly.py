@@ -138,14 +138,6 @@ def test_dielectric_l3_sampled_rows(sx, O, omega):
     assert max(errs) <= TOL
 
 
-def test_short_wavelength_regime_is_rejected(sx, O):
-    V, T = O.icosphere(1)
-    G = sx.RWGGeometry()
-    G.AddMesh(V, T, "PEC")
-    with pytest.raises(sx.SCUFFXError, match="short-wavelength"):
-        G.AssembleBEMMatrix(30.0)
-
-
 def test_wrong_size_matrix_is_replaced(sx, O):
     V, T = O.icosphere(0)
     G = sx.RWGGeometry()
