@@ -76,6 +76,17 @@ int scuffx_fippi_build(scuffx_geom *G);
 int scuffx_fippi_count(const scuffx_geom *G);
 /* host copy, 36 doubles per record (QIFIPPIData, libscuffInternals.h:282-292) */
 int scuffx_fippi_records(scuffx_geom *G, double *out);
+/* ---- PreloadCache / StoreCache (libscuff.h:662-663, FIPPICache.cc:189-414) ----
+   process-wide FIPPI record cache: every table built later takes the records
+   whose 15-float key it finds there, and adds the ones it computes.  File
+   format of the reference: "FIPPICACHE\0", then 352-byte records (15 float32
+   key, 4 bytes padding, 36 doubles).  Preload returns the number of records
+   read (an existing key keeps its record); store returns the number written
+   (0, file untouched, when the cache was preloaded from that same file and has
+   not grown, as the reference does). */
+long scuffx_preload_cache(const char *path);
+long scuffx_store_cache(const char *path);
+long scuffx_cache_size(void);
 
 /* ---- RWGGeometry::AssembleBEMMatrix(Omega, M) (AssembleBEMMatrix.cc:684) ----
    fills all N x N entries of the complex128 matrix at M_dev */

@@ -11,6 +11,7 @@
 // rule), near pairs (desingularized rule + cached singular terms), self
 // pairs (Taylor-Duffy); a final kernel combines them per edge pair.
 #include "assemble.h"
+#include "fippi_cache.h"
 #include "prof.h"
 
 #include <algorithm>
@@ -125,19 +126,6 @@ static int canonical(const double *Va[3], const double *Vb[3], int ncv, const do
   return 1;
 }
 
-struct Key15 {
-  float k[15];
-  bool operator==(const Key15 &o) const { return std::memcmp(k, o.k, sizeof(k)) == 0; }
-};
-struct Key15Hash {
-  size_t operator()(const Key15 &K) const
-  {
-    const unsigned char *p = (const unsigned char *)K.k;
-    uint64_t h = 1469598103934665603ULL;
-    for (int i = 0; i < 60; i++) { h ^= p[i]; h *= 1099511628211ULL; }
-    return (size_t)h;
-  }
-};
 
 Assembler::Assembler(const Geometry &G) : G_(G)
 {
@@ -323,10 +311,7 @@ void Assembler::build_near()
     }
     P.flipped = fl;
     P.ncv = ncv;
-    Key15 K;
-    const double *src[5] = {OVa[1], OVa[2], OVb[0], OVb[1], OVb[2]};
-    for (int s = 0; s < 5; s++)
-      for (int m = 0; m < 3; m++) K.k[3 * s + m] = ((float)src[s][m]) - ((float)OVa[0][m]);
+    const Key15 K = fippi_key(OVa, OVb);
     auto it = keymap.find(K);
     if (it == keymap.end()) {
       FippiJob J;
@@ -335,6 +320,7 @@ void Assembler::build_near()
       J.ncv = ncv;
       int id = (int)jobs_.size();
       jobs_.push_back(J);
+      job_keys_.push_back(K);
       keymap.emplace(K, id);
       P.rec = id;
     } else P.rec = it->second;
@@ -362,29 +348,44 @@ int Assembler::build_fippi(hipStream_t stream)
   if (!err_.empty()) return -3;
   const int nj = (int)jobs_.size();
   auto t0 = std::chrono::steady_clock::now();
+  // records already in the process-wide cache (preloaded from a file or
+  // computed for an earlier geometry) are copied, the others computed
+  std::vector<double> rec(std::max(nj, 1) * (size_t)SCUFFX_QIF_LEN, 0.0);
+  std::vector<int> todo;
+  for (int j = 0; j < nj; j++)
+    if (!fippi_cache_find(job_keys_[j], rec.data() + (size_t)j * SCUFFX_QIF_LEN)) todo.push_back(j);
+  stats_.nfippi_cached = nj - (long)todo.size();
   if (nj > 0) {
     HIPCHK(hipMalloc(&d_jobs_, sizeof(FippiJob) * nj));
     HIPCHK(hipMalloc(&d_records_, sizeof(double) * SCUFFX_QIF_LEN * nj));
     HIPCHK(hipMemcpyAsync(d_jobs_, jobs_.data(), sizeof(FippiJob) * nj, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpy(d_records_, rec.data(), sizeof(double) * SCUFFX_QIF_LEN * nj, hipMemcpyHostToDevice));
+  }
+  const int nt = (int)todo.size();
+  if (nt > 0) {
     int *d_counter = nullptr, *d_ovf = nullptr;
     HIPCHK(hipMalloc(&d_counter, sizeof(int)));
     HIPCHK(hipMalloc(&d_ovf, sizeof(int) * nj));
     HIPCHK(hipMemsetAsync(d_ovf, 0, sizeof(int) * nj, stream));
     HIPCHK(hipMemsetAsync(d_counter, 0, sizeof(int), stream));
-    // pass 1: small heaps (4096 regions) for every job, 2048 waves
-    int cap1 = 4096, slots1 = std::min(2048, std::max(1, nj));
+    // pass 1: small heaps (4096 regions) for every job to compute, 2048 waves
+    int cap1 = 4096, slots1 = std::min(2048, std::max(1, nt));
     double *hd = nullptr, *hk = nullptr;
-    int *hs = nullptr;
+    int *hs = nullptr, *d_todo = nullptr;
+    if (nt < nj) {
+      HIPCHK(hipMalloc(&d_todo, sizeof(int) * nt));
+      HIPCHK(hipMemcpy(d_todo, todo.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMalloc(&hd, sizeof(double) * (size_t)cap1 * 76 * slots1));
     HIPCHK(hipMalloc(&hk, sizeof(double) * (size_t)cap1 * slots1));
     HIPCHK(hipMalloc(&hs, sizeof(int) * (size_t)cap1 * slots1));
     int tok = prof_begin(stream, PK_FIPPI);
-    HIPCHK(launch_fippi(stream, d_jobs_, nj, d_records_, hd, hs, hk, cap1, slots1, d_counter, d_ovf, nullptr));
+    HIPCHK(launch_fippi(stream, d_jobs_, nt, d_records_, hd, hs, hk, cap1, slots1, d_counter, d_ovf, d_todo));
     prof_end(stream, tok, 0.0);
     std::vector<int> ovf(nj);
     HIPCHK(hipMemcpyAsync(ovf.data(), d_ovf, sizeof(int) * nj, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    hipFree(hd); hipFree(hk); hipFree(hs);
+    hipFree(hd); hipFree(hk); hipFree(hs); hipFree(d_todo);
     std::vector<int> redo;
     for (int j = 0; j < nj; j++) if (ovf[j]) redo.push_back(j);
     if (!redo.empty()) {
@@ -410,6 +411,9 @@ int Assembler::build_fippi(hipStream_t stream)
     }
     hipFree(d_counter); hipFree(d_ovf);
     if (prof_enabled()) prof_collect();
+    // the new records join the process-wide cache (StoreCache writes them)
+    HIPCHK(hipMemcpy(rec.data(), d_records_, sizeof(double) * SCUFFX_QIF_LEN * nj, hipMemcpyDeviceToHost));
+    for (int j : todo) fippi_cache_insert(job_keys_[j], rec.data() + (size_t)j * SCUFFX_QIF_LEN);
   }
   fippi_ready_ = true;
   stats_.fippi_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "bem.h"
+#include "fippi_cache.h"
 #include "geometry.h"
 
 namespace scuffx {
@@ -31,7 +32,7 @@ struct NearBlock {
 
 struct AsmStats {
   double fippi_ms = 0, self_ms = 0, far_ms = 0, near_ms = 0, stamp_ms = 0, mirror_ms = 0;
-  long nfar = 0, nnear = 0, nself = 0, nfippi = 0, nsw = 0;
+  long nfar = 0, nnear = 0, nself = 0, nfippi = 0, nsw = 0, nfippi_cached = 0;
 };
 
 class Assembler {
@@ -57,6 +58,7 @@ class Assembler {
   std::vector<DevSurface> ds_;
   std::vector<NearBlock> blocks_;
   std::vector<FippiJob> jobs_;
+  std::vector<Key15> job_keys_;
   FippiJob *d_jobs_ = nullptr;
   double *d_records_ = nullptr;
   bool fippi_ready_ = false;

@@ -10,6 +10,7 @@
 
 #include "../../include/scuffx.h"
 #include "assemble.h"
+#include "fippi_cache.h"
 #include "geometry.h"
 #include "lu.h"
 #include "prof.h"
@@ -179,6 +180,26 @@ int scuffx_fippi_count(const scuffx_geom *G)
   if (!G) return fail(SCUFFX_ERR_ARG, "null geometry");
   return G->A ? G->A->num_fippi() : 0;
 }
+
+long scuffx_preload_cache(const char *path)
+{
+  if (!path) return fail(SCUFFX_ERR_ARG, "null file name");
+  std::string err;
+  long n = fippi_cache_preload(path, err);
+  if (n < 0) return fail(SCUFFX_ERR_ARG, err + " (cache not preloaded)");
+  return n;
+}
+
+long scuffx_store_cache(const char *path)
+{
+  if (!path) return fail(SCUFFX_ERR_ARG, "null file name");
+  std::string err;
+  long n = fippi_cache_store(path, err);
+  if (n < 0) return fail(SCUFFX_ERR_ARG, err);
+  return n;
+}
+
+long scuffx_cache_size(void) { return (long)fippi_cache_size(); }
 
 int scuffx_fippi_records(scuffx_geom *G, double *out)
 {

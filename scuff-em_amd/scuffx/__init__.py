@@ -76,6 +76,9 @@ def lib():
             "scuffx_power": ([vp, C.c_double, C.c_double, vp, vp, dp, dp], C.c_int),
             "scuffx_lu_logdet": ([vp, C.c_long, C.c_long, vp, C.c_int, dp, dp], C.c_int),
             "scuffx_trace_solve": ([vp, C.c_long, C.c_long, vp, vp, C.c_long, C.c_long, C.c_int, dp, dp], C.c_int),
+            "scuffx_preload_cache": ([C.c_char_p], C.c_long),
+            "scuffx_store_cache": ([C.c_char_p], C.c_long),
+            "scuffx_cache_size": ([], C.c_long),
             "scuffx_profile": ([C.c_int], C.c_int),
             "scuffx_profile_read": ([C.c_char_p, dp, dp, lp], C.c_int),
         }
@@ -251,6 +254,30 @@ def GetTraceMInvdM(M, dM):
     import math
     t = 2.0 * M.TraceSolve(dM).real
     return 0.0 if not math.isfinite(t) else -t / (2.0 * math.pi)
+
+
+def PreloadCache(FileName):
+    """Fill the process-wide FIPPI cache from a file in the reference's
+    format (PreloadCache, libscuff.h:662; FIPPICache.cc:304-400).  As the
+    reference does, an unreadable or malformed file only warns and the
+    cache stays as it was.  Returns the number of records read."""
+    import warnings
+    n = lib().scuffx_preload_cache(str(FileName).encode())
+    if n < 0:
+        warnings.warn("PreloadCache: " + lib().scuffx_last_error().decode())
+        return 0
+    return n
+
+
+def StoreCache(FileName):
+    """Write the process-wide FIPPI cache to a file in the reference's
+    format (StoreCache, libscuff.h:663; FIPPICache.cc:230-302); returns the
+    number of records written."""
+    return _check(lib().scuffx_store_cache(str(FileName).encode()))
+
+
+def CacheSize():
+    return lib().scuffx_cache_size()
 
 
 class PlaneWave:
