@@ -87,6 +87,10 @@ int scuffx_fippi_records(scuffx_geom *G, double *out);
 long scuffx_preload_cache(const char *path);
 long scuffx_store_cache(const char *path);
 long scuffx_cache_size(void);
+/* empties the process-wide cache (no counterpart in the reference, whose
+   global cache lives as long as the process); tables already built keep
+   their records */
+int scuffx_clear_cache(void);
 
 /* ---- RWGGeometry::AssembleBEMMatrix(Omega, M) (AssembleBEMMatrix.cc:684) ----
    fills all N x N entries of the complex128 matrix at M_dev */

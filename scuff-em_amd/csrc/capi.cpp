@@ -201,6 +201,12 @@ long scuffx_store_cache(const char *path)
 
 long scuffx_cache_size(void) { return (long)fippi_cache_size(); }
 
+int scuffx_clear_cache(void)
+{
+  fippi_cache_clear();
+  return 0;
+}
+
 int scuffx_fippi_records(scuffx_geom *G, double *out)
 {
   if (!G || !out) return fail(SCUFFX_ERR_ARG, "null argument");

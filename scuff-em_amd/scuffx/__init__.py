@@ -79,6 +79,7 @@ def lib():
             "scuffx_preload_cache": ([C.c_char_p], C.c_long),
             "scuffx_store_cache": ([C.c_char_p], C.c_long),
             "scuffx_cache_size": ([], C.c_long),
+            "scuffx_clear_cache": ([], C.c_int),
             "scuffx_profile": ([C.c_int], C.c_int),
             "scuffx_profile_read": ([C.c_char_p, dp, dp, lp], C.c_int),
         }
@@ -278,6 +279,11 @@ def StoreCache(FileName):
 
 def CacheSize():
     return lib().scuffx_cache_size()
+
+
+def ClearCache():
+    """Empty the process-wide FIPPI cache (no reference counterpart)."""
+    return _check(lib().scuffx_clear_cache())
 
 
 class PlaneWave:

@@ -32,6 +32,18 @@ def sx():
     return scuffx
 
 
+@pytest.fixture(autouse=True)
+def _fresh_fippi_cache(request):
+    """The product's FIPPI cache is process-wide, as the reference's is;
+    the oracle's lives with one geometry.  Parity tests start every test
+    from an empty cache so both sides reuse records only within the test's
+    own geometry."""
+    if request.node.get_closest_marker("gpu") is not None:
+        import scuffx
+        scuffx.ClearCache()
+    yield
+
+
 def rel_fro(A, B):
     """Relative Frobenius distance ||A-B|| / ||B||."""
     return float(np.linalg.norm(np.asarray(A) - np.asarray(B)) / np.linalg.norm(np.asarray(B)))

@@ -1,7 +1,9 @@
 """StoreCache / PreloadCache round trip on the GPU path: the records a
 geometry computes are written in the reference's file format, and a
 fresh process that preloads the file builds the same geometry's table
-from it (bit-identical records and matrix, no cubature run)."""
+from it (bit-identical records and matrix; the cache does not grow, so
+no record is computed).  In one process the cache is shared by every
+geometry, as the reference's GlobalFIPPICache is."""
 import os
 import subprocess
 import sys
@@ -27,7 +29,7 @@ M = G.AssembleBEMMatrix(0.9).numpy()
 R = G.FIPPIRecords()
 np.save({out_m!r}, M)
 np.save({out_r!r}, R)
-print("fippi_ms", G.AssemblyStats()["fippi_ms"], flush=True)
+print("cache", n, scuffx.CacheSize(), "fippi_ms", G.AssemblyStats()["fippi_ms"], flush=True)
 """
 
 
@@ -49,8 +51,9 @@ def test_store_then_preload_in_fresh_process(sx, O, tmp_path):
     print(p.stdout, p.stderr[-2000:], flush=True)
     assert p.returncode == 0
     assert f"preloaded {nst}" in p.stdout
-    t_pre = float(p.stdout.split("fippi_ms")[1].split()[0])
+    # every record of the new table came from the file: the cache did not grow
+    assert f"cache {nst} {nst} " in p.stdout
     assert np.array_equal(np.load(out_r), R)
     assert np.array_equal(np.load(out_m), M)
-    print(f"  FIPPI table: {t_build:.1f} ms computed, {t_pre:.1f} ms preloaded", flush=True)
-    assert t_pre < 0.5 * t_build
+    t_pre = float(p.stdout.split("fippi_ms")[1].split()[0])
+    print(f"  FIPPI table: {t_build:.1f} ms in this process, {t_pre:.1f} ms from the file", flush=True)
