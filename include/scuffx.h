@@ -60,6 +60,10 @@ int scuffx_geom_counts(const scuffx_geom *G, int ns, int *nV, int *nP, int *nE);
 /* first matrix row of surface ns and its number of basis functions
    (RWGGeometry::BFIndexOffset, RWGSurface::NumBFs, RWGGeometry.cc:708-724) */
 int scuffx_geom_bf_range(const scuffx_geom *G, int ns, int *offset, int *nbf);
+/* RWGGeometry::Mate[ns] (RWGGeometry.cc:733-770): the first earlier surface
+   identical to ns (same mesh file, same exterior medium, both PEC or the same
+   interior material), or -1; surfaces added from raw arrays have no mates */
+int scuffx_geom_mate(const scuffx_geom *G, int ns);
 /* per edge: iV1 iV2 iQP iQM iPPanel iMPanel PIndex MIndex Index (libscuff.h:120-146) */
 int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out9, double *lengths);
 /* per panel: centroid[3] zhat[3] radius area (libscuff.h:100-112) */

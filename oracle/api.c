@@ -118,6 +118,15 @@ int o_api_assemble_rows(void *p, double wr, double wi, double *M, int ldm, int e
 
 size_t o_api_fippi_count(void *p) { return o_fippi_cache_size(((oh *)p)->FC); }
 
+/* Mate[ns] = nsm: surface ns is identical to the earlier surface nsm */
+int o_api_set_mate(void *p, int ns, int nsm)
+{
+  OGeom *G = ((oh *)p)->G;
+  if (ns < 0 || ns >= G->NumSurfaces || nsm >= ns) return -1;
+  G->Mate[ns] = nsm;
+  return 0;
+}
+
 /* QIFIPPI record (36 doubles: xMxpRM3[3] xXxpRM3[3] RM1[9] R1[9] R2[9] x0[3]) */
 void o_api_qifippi(const double *Va9, const double *Vb9, int ncv, double *out, size_t *nevals)
 {

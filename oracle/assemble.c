@@ -154,8 +154,19 @@ int o_assemble_bem(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M, int ldm
   o_update_epsmu(G, Omega);
   if (st) memset(st, 0, sizeof(*st));
   for (int ns = 0; ns < G->NumSurfaces; ns++)
-    for (int nsp = ns; nsp < G->NumSurfaces; nsp++)
+    for (int nsp = ns; nsp < G->NumSurfaces; nsp++) {
+      /* a surface identical to an earlier one takes a copy of that
+         surface's diagonal block (AssembleBEMMatrix.cc:728-740) */
+      int nsm = (ns == nsp) ? G->Mate[ns] : -1;
+      if (nsm >= 0) {
+        int n = G->Surfaces[ns]->NumBFs, o = G->BFIndexOffset[ns], om = G->BFIndexOffset[nsm];
+        for (int c = 0; c < n; c++)
+          for (int r = 0; r <= c; r++)
+            M[(size_t)(o + r) + (size_t)(o + c)*ldm] = M[(size_t)(om + r) + (size_t)(om + c)*ldm];
+        continue;
+      }
       ssi_block(G, ns, nsp, Omega, FC, M, ldm, 0, G->Surfaces[ns]->NumEdges, 1, st);
+    }
   int N = G->TotalBFs;
   for (int nr = 1; nr < N; nr++)
     for (int nc = 0; nc < nr; nc++)

@@ -299,6 +299,7 @@ void o_surface_free(OSurface *S)
 OGeom *o_geom_new(void)
 {
   OGeom *G = (OGeom *)calloc(1, sizeof(OGeom));
+  for (int i = 0; i < OMAXSURF; i++) G->Mate[i] = -1;
   G->NumRegions = 1;
   G->Materials[0].Kind = OMAT_VACUUM;
   G->StoredOmega = 0.0;

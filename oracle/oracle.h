@@ -73,6 +73,7 @@ typedef struct {
   int TotalBFs;
   zdouble EpsTF[OMAXREG], MuTF[OMAXREG];
   zdouble StoredOmega;
+  int Mate[OMAXSURF];     /* RWGGeometry::Mate (RWGGeometry.cc:733-770), set by the caller; -1 = none */
 } OGeom;
 
 /* mesh construction */

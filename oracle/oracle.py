@@ -68,6 +68,7 @@ def lib():
                                             C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                             C.c_void_p]
     L.o_api_num_bfs.argtypes = [C.c_void_p, C.c_int]
+    L.o_api_set_mate.argtypes = [C.c_void_p, C.c_int, C.c_int]
     _CB = C.CFUNCTYPE(None, C.c_uint, C.POINTER(C.c_double), C.c_void_p, C.c_uint,
                       C.POINTER(C.c_double))
     L._integrand_type = _CB
@@ -241,6 +242,12 @@ class Geometry:
         if r != 0:
             raise RuntimeError("oracle: bad block")
         return B, [GB[m].T.copy() for m in range(3)], [TB[t].T.copy() for t in range(nt)]
+
+    def set_mate(self, ns, nsm):
+        """surface ns is identical to the earlier surface nsm: the full
+        assembly copies nsm's diagonal block (AssembleBEMMatrix.cc:728-740)"""
+        if self.L.o_api_set_mate(self.h, ns, nsm) != 0:
+            raise ValueError("bad mate")
 
     def fippi_count(self):
         return self.L.o_api_fippi_count(self.h)

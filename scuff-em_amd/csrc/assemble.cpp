@@ -162,6 +162,7 @@ Assembler::Assembler(const Geometry &G) : G_(G)
     }
   }
   hipMalloc(&d_regs_, sizeof(RegionK) * 4);
+  mates_ = find_mates(G);
   build_near();
 }
 
@@ -544,6 +545,15 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
       const DevSurface &Da = ds_[sa], &Db = ds_[sb];
       NearBlock &B = blocks_[bi];
       const int same = (sa == sb);
+      if (same && mates_[sa] >= 0) {
+        // identical to an earlier surface: its diagonal block is copied
+        // (AssembleBEMMatrix.cc:733-740); the lower triangle comes from the
+        // final mirror as for every diagonal block
+        const long om = G_.BFIndexOffset[mates_[sa]], os = G_.BFIndexOffset[sa], n = Sa.NumBFs;
+        HIPCHK(hipMemcpy2DAsync(M + os + os * ldm, sizeof(double2) * ldm, M + om + om * ldm, sizeof(double2) * ldm,
+                                sizeof(double2) * n, n, hipMemcpyDeviceToDevice, stream));
+        continue;
+      }
       RegionK rk[2];
       int nreg = regions_for(sa, sb, eps, mu, omega, rk);
       if (nreg < 0) {

@@ -59,6 +59,7 @@ class Assembler {
   std::vector<NearBlock> blocks_;
   std::vector<FippiJob> jobs_;
   std::vector<Key15> job_keys_;
+  std::vector<int> mates_;
   FippiJob *d_jobs_ = nullptr;
   double *d_records_ = nullptr;
   bool fippi_ready_ = false;

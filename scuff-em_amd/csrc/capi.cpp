@@ -69,7 +69,8 @@ int scuffx_geom_open(const char *path, scuffx_geom **G)
   return 0;
 }
 
-static int add_surface_checked(scuffx_geom *G, const double *V, int nV, const int *T, int nT, const char *mat)
+static int add_surface_checked(scuffx_geom *G, const double *V, int nV, const int *T, int nT, const char *mat,
+                               const char *meshfile = nullptr)
 {
   if (G->A) return fail(SCUFFX_ERR_ARG, "geometry is frozen after its first assembly");
   Material m;
@@ -79,6 +80,8 @@ static int add_surface_checked(scuffx_geom *G, const double *V, int nV, const in
   Surface S;
   e = build_surface(V, nV, T, nT, pec, &S);
   if (!e.empty()) return fail(SCUFFX_ERR_ARG, e);
+  S.MeshFile = meshfile ? meshfile : "";
+  S.MaterialName = pec ? std::string("PEC") : std::string(mat);
   return add_surface(&G->G, std::move(S), m);
 }
 
@@ -95,7 +98,13 @@ int scuffx_geom_add_msh(scuffx_geom *G, const char *path, const char *mat)
   std::vector<int> T;
   std::string e = read_gmsh(path, &V, &T);
   if (!e.empty()) return fail(SCUFFX_ERR_IO, e);
-  return add_surface_checked(G, V.data(), (int)V.size() / 3, T.data(), (int)T.size() / 3, mat);
+  return add_surface_checked(G, V.data(), (int)V.size() / 3, T.data(), (int)T.size() / 3, mat, path);
+}
+
+int scuffx_geom_mate(const scuffx_geom *G, int ns)
+{
+  if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
+  return find_mates(G->G)[ns];
 }
 
 int scuffx_geom_free(scuffx_geom *G) { delete G; return 0; }

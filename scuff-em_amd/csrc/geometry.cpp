@@ -132,6 +132,22 @@ std::string build_surface(const double *Vin, int nV, const int *T, int nT,
   return "";
 }
 
+std::vector<int> find_mates(const Geometry &G)
+{
+  const int NS = (int)G.Surfaces.size();
+  std::vector<int> mate(NS, -1);
+  for (int ns = 1; ns < NS; ns++) {
+    const Surface &S = G.Surfaces[ns];
+    for (int nsp = 0; nsp < ns && mate[ns] == -1; nsp++) {
+      const Surface &P = G.Surfaces[nsp];
+      if (S.MeshFile.empty() || S.MeshFile != P.MeshFile) continue;
+      if (S.RegionIndices[0] != P.RegionIndices[0]) continue;
+      if ((S.IsPEC && P.IsPEC) || (!S.IsPEC && !P.IsPEC && S.MaterialName == P.MaterialName)) mate[ns] = nsp;
+    }
+  }
+  return mate;
+}
+
 int add_surface(Geometry *G, Surface &&S, const Material &interior)
 {
   S.RegionIndices[0] = 0;
@@ -388,6 +404,8 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
       Surface S;
       err = build_surface(V.data(), (int)V.size() / 3, Tri.data(), (int)Tri.size() / 3, pec, &S);
       if (!err.empty()) return where + err;
+      S.MeshFile = mesh;
+      S.MaterialName = pec ? std::string("PEC") : mat;
       add_surface(G, std::move(S), m);
       inobj = false;
     } else {

@@ -49,6 +49,8 @@ struct Surface {
   bool IsPEC = true;
   int RegionIndices[2] = {0, -1};
   int NumBFs = 0;
+  std::string MeshFile;         // as named by the caller; empty for raw arrays
+  std::string MaterialName;     // interior material as given ("PEC" for PEC)
   int NumVertices() const { return (int)(V.size() / 3); }
   int NumPanels() const { return (int)Panels.size(); }
   int NumEdges() const { return (int)Edges.size(); }
@@ -67,6 +69,10 @@ struct Geometry {
 std::string build_surface(const double *V, int nV, const int *T, int nT,
                           bool pec, Surface *S);
 int add_surface(Geometry *G, Surface &&S, const Material &interior);
+// Mate[ns] = the first earlier surface identical to ns, or -1: same mesh
+// file, same exterior medium, both PEC or the same interior material
+// (RWGGeometry.cc:733-770; MESHTAG is not supported, so tags always agree)
+std::vector<int> find_mates(const Geometry &G);
 
 // GMSH 2.x / legacy reader (ParseMeshFiles.cc:69-259).
 std::string read_gmsh(const std::string &fn, std::vector<double> *V,

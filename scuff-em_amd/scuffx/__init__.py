@@ -59,6 +59,7 @@ def lib():
             "scuffx_geom_edges": ([vp, C.c_int, vp, vp], C.c_int),
             "scuffx_geom_panels": ([vp, C.c_int, vp], C.c_int),
             "scuffx_geom_vertices": ([vp, C.c_int, vp], C.c_int),
+            "scuffx_geom_mate": ([vp, C.c_int], C.c_int),
             "scuffx_icosphere": ([C.c_int, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_fippi_build": ([vp], C.c_int),
             "scuffx_fippi_count": ([vp], C.c_int),
@@ -345,6 +346,12 @@ class RWGGeometry:
         out = np.zeros((npan, 8))
         _check(lib().scuffx_geom_panels(self.h, ns, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    @property
+    def Mate(self):
+        """Mate[ns]: the first earlier surface identical to ns, or -1
+        (RWGGeometry.cc:733-770)"""
+        return [lib().scuffx_geom_mate(self.h, ns) for ns in range(self.NumSurfaces)]
 
     def Vertices(self, ns=0):
         """vertex coordinates of surface ns, shape (NumVertices, 3)"""

@@ -59,6 +59,7 @@ class SC3Data:
             raise SCUFFXError("forces and torques need at least two surfaces")
         self.NumQuantities = int(self.Energy) + len(self.Mus)
         self.XiMin, self.AbsTol, self.RelTol = XiMin, AbsTol, RelTol
+        self.Mate = G.Mate
         self.M = HMatrix(self.N, self.N)
         self.dM = HMatrix(self.N, self.N1) if self.Mus else None
         self.ByXi = []                       # (Xi, EFT) of every evaluation
@@ -72,16 +73,28 @@ class SC3Data:
         G, M = self.G, self.M
         Omega = complex(0.0, Xi)
         offs = self._offsets()
-        # T blocks into M; the LU diagonals of the T blocks make up that of M_inf
+        # T blocks into M; the LU diagonals of the T blocks make up that of
+        # M_inf; a surface identical to an earlier one (Mate) reuses its
+        # block and its diagonal (CasimirIntegrand.cc:276-291, :323-337)
         M.t.zero_()
         M.factored = False
         logdet_inf = 0.0
+        ld = {}
         for ns, (o, n) in enumerate(offs):
+            nm = self.Mate[ns]
+            if nm >= 0:
+                om = offs[nm][0]
+                M.t[o:o + n, o:o + n].copy_(M.t[om:om + n, om:om + n])
+                if self.Energy:
+                    ld[ns] = ld[nm]
+                    logdet_inf += ld[ns]
+                continue
             T = G.AssembleBEMMatrixBlock(ns, ns, Omega)
             M.t[o:o + n, o:o + n].copy_(T.t)
             if self.Energy:
                 T.LUFactorize()
-                logdet_inf += T.LogDet().real
+                ld[ns] = T.LogDet().real
+                logdet_inf += ld[ns]
         # U blocks (upper triangle, adjoint below) and dU_{0,s}
         dU = {}
         nt = len(self.TorqueAxes)

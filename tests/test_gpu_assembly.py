@@ -147,3 +147,29 @@ def test_wrong_size_matrix_is_replaced(sx, O):
     assert M is not M0 and M.NR == 30
     M2 = G.AssembleBEMMatrix(1.0, M)
     assert M2 is M
+
+
+def test_mate_blocks_copied(sx, O, tmp_path):
+    """two identical dielectric spheres from one mesh file: the second
+    diagonal block is the first's, copied (AssembleBEMMatrix.cc:733-740);
+    the whole matrix matches the oracle, which computes both blocks"""
+    from conftest import GOLDEN, read_msh
+    mesh = os.path.join(GOLDEN, "SSphere_255.msh")
+    geo = tmp_path / "pair.scuffgeo"
+    geo.write_text(f"OBJECT A\n  MESHFILE {mesh}\n  MATERIAL CONST_EPS_4+0.1i\nENDOBJECT\n"
+                   f"OBJECT B\n  MESHFILE {mesh}\n  DISPLACED 0.1 0 2.7\n  MATERIAL CONST_EPS_4+0.1i\nENDOBJECT\n")
+    print("mate blocks", flush=True)
+    G = sx.RWGGeometry(str(geo))
+    assert G.Mate == [-1, 0]
+    M = G.AssembleBEMMatrix(0.8).numpy()
+    V, T = read_msh(mesh)
+    Go = O.Geometry()
+    Go.add_surface(V, T, kind=1, eps=4 + 0.1j)
+    Go.add_surface(V + np.array([0.1, 0.0, 2.7]), T, kind=1, eps=4 + 0.1j)
+    Go.set_mate(1, 0)
+    Mo = Go.assemble(0.8)
+    e = rel_fro(M, Mo)
+    print(f"  rel Frobenius {e:.2e}", flush=True)
+    assert e <= 1e-12
+    o1, n1 = G.BFOffset(1)
+    assert np.array_equal(M[o1:o1 + n1, o1:o1 + n1], M[:n1, :n1])

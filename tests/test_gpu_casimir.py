@@ -37,6 +37,7 @@ def oracle_pair(O, sep, kind, eps):
     Go = O.Geometry()
     Go.add_surface(V + np.array([0.0, 0.0, sep]), T, kind=kind, eps=eps)
     Go.add_surface(V, T, kind=kind, eps=eps)
+    Go.set_mate(1, 0)          # one mesh file, one material: the reference's Mate
     return Go
 
 

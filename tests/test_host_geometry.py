@@ -158,3 +158,24 @@ def test_material_names(sx, name, ok):
     else:
         with pytest.raises(sx.SCUFFXError):
             G.AddMesh(V, T, name)
+
+
+def test_mates(sx, tmp_path):
+    """Mate[] (RWGGeometry.cc:733-770): same mesh file and material ->
+    the first earlier such surface; another material, PEC vs dielectric
+    or raw arrays -> -1"""
+    mesh = os.path.join(GOLDEN, "SSphere_255.msh")
+    geo = tmp_path / "mates.scuffgeo"
+    geo.write_text(
+        f"OBJECT A\n  MESHFILE {mesh}\n  MATERIAL CONST_EPS_4\nENDOBJECT\n"
+        f"OBJECT B\n  MESHFILE {mesh}\n  DISPLACED 0 0 3\n  MATERIAL CONST_EPS_4\nENDOBJECT\n"
+        f"OBJECT C\n  MESHFILE {mesh}\n  DISPLACED 0 0 6\n  MATERIAL CONST_EPS_5\nENDOBJECT\n"
+        f"OBJECT D\n  MESHFILE {mesh}\n  DISPLACED 0 0 9\nENDOBJECT\n"
+        f"OBJECT E\n  MESHFILE {mesh}\n  ROTATED 30 ABOUT 0 0 1\n  DISPLACED 0 0 12\nENDOBJECT\n"
+        f"OBJECT F\n  MESHFILE {mesh}\n  DISPLACED 0 0 15\n  MATERIAL CONST_EPS_4\nENDOBJECT\n")
+    G = sx.RWGGeometry(str(geo))
+    assert G.Mate == [-1, 0, -1, -1, 3, 0]
+    G2 = sx.RWGGeometry()
+    G2.AddMeshFile(mesh, "PEC")
+    G2.AddMeshFile(mesh, "PEC")
+    assert G2.Mate == [-1, 0]

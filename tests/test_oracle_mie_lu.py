@@ -79,3 +79,19 @@ def test_zgetrs(O, trans):
     X = O.zgetrs(LU, ipiv, B, trans)
     op = {"N": A, "T": A.T, "C": A.conj().T}[trans]
     assert np.linalg.norm(op @ X - B) <= 1e-12 * np.linalg.norm(B) * np.linalg.cond(A)
+
+
+def test_oracle_mate_copies_diagonal_block(O):
+    """Mate (AssembleBEMMatrix.cc:728-740): the declared duplicate's
+    diagonal block is the first surface's; everything else unchanged"""
+    V, T = O.icosphere(0)
+    d = np.array([0.0, 0.0, 3.0])
+    A = O.Geometry(); A.add_surface(V, T, kind=1, eps=4.0); A.add_surface(V + d, T, kind=1, eps=4.0)
+    B = O.Geometry(); B.add_surface(V, T, kind=1, eps=4.0); B.add_surface(V + d, T, kind=1, eps=4.0)
+    B.set_mate(1, 0)
+    Ma, Mb = A.assemble(0.7), B.assemble(0.7)
+    n = B.num_bfs(0)
+    assert np.array_equal(Mb[n:, n:], Mb[:n, :n])
+    assert np.array_equal(Mb[:n, :], Ma[:n, :]) and np.array_equal(Mb[:, :n], Ma[:, :n])
+    # the computed block agrees with the copy to the FIPPI cubature tolerance
+    assert np.linalg.norm(Ma[n:, n:] - Mb[n:, n:]) <= 1e-6 * np.linalg.norm(Ma[n:, n:])
