@@ -245,6 +245,8 @@ __device__ inline double2 ld_coh(const double2 *p)
 // polls before a wait gives up (about a second); the kernel then
 // finishes and the host reports the time-out
 #define GBAR_SPIN (1u << 20)
+// keys one lane of the polling wave holds: up to 64 * 8 workgroups
+#define KEYS_PER_LANE 8
 
 // every memory operation this thread issued has completed; the signal
 // fences keep the compiler from moving memory operations across it
@@ -261,6 +263,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
 {
   typedef decltype(T().x) R;
   constexpr int NR = PanelRows<T>::n;
+  constexpr bool kPackKey = std::is_same<T, float2>::value;
   __shared__ T slab[PW][NR];           // column-major: slab[q][t] = A(lo + t, r0 + q)
   __shared__ R wv[NR / 64];
   __shared__ int wi[NR / 64];
@@ -309,22 +312,42 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
       if (bb >= 0) st_coh(candrow + ((long)buf * Gmax + g) * PW + t, slab[t][bb]);
       if (c >= lo && c < lo + nrow) st_coh(rowc + buf * PW + t, slab[t][c - lo]);
     }
-    if (t == 0)
+    // complex64: the value travels in the key itself (value bits << 32 |
+    // local row << 23 | 23-bit sequence number, row 511 = no candidate), so
+    // the winner is known from the polled keys alone; complex128 needs all
+    // 64 bits of its value and publishes it beside the key
+    if (!kPackKey && t == 0)
       st_coh((unsigned long long *)S.candv + buf * Gmax + g, (unsigned long long)__double_as_longlong((double)b0));
     drain();
     __syncthreads();
-    if (t == 0)
-      st_coh(S.ckey + buf * Gmax + g, ((unsigned long long)(unsigned)(bb >= 0 ? bb : -1) << 32) | seq);
+    if (t == 0) {
+      unsigned long long key;
+      if (kPackKey)
+        key = ((unsigned long long)__float_as_uint((float)b0) << 32) |
+              ((unsigned long long)(bb >= 0 ? bb : 511) << 23) | (seq & 0x7FFFFFu);
+      else
+        key = ((unsigned long long)(unsigned)(bb >= 0 ? bb : -1) << 32) | seq;
+      st_coh(S.ckey + buf * Gmax + g, key);
+    }
     // global pivot: wave 0 waits for every workgroup's key of this column
     if (t < 64) {
       // bounded: past GBAR_SPIN polls the error word is set, and once it
       // is set no workgroup waits again (the panel then finishes with
       // meaningless values and the host reports the time-out)
+      // the keys of the last poll stay in registers (G <= 64 * KEYS_PER_LANE)
+      unsigned long long kk[KEYS_PER_LANE];
+      const unsigned smask = kPackKey ? 0x7FFFFFu : 0xFFFFFFFFu;
       unsigned spins = 0;
       for (;;) {
         bool ok = true;
-        for (int q = lane; q < G; q += 64)
-          if ((unsigned)ld_coh(S.ckey + buf * Gmax + q) != seq) ok = false;
+#pragma unroll
+        for (int u = 0; u < KEYS_PER_LANE; u++) {
+          const int q = lane + 64 * u;
+          if (q < G) {
+            kk[u] = ld_coh(S.ckey + buf * Gmax + q);
+            if (((unsigned)kk[u] & smask) != (seq & smask)) ok = false;
+          }
+        }
         if (__ballot(!ok) == 0) break;
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (++spins > GBAR_SPIN) { __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
@@ -332,11 +355,22 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
       }
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       R b = -1; long p = -1; int gw = -1;
-      for (int q = lane; q < G; q += 64) {
-        unsigned long long k = ld_coh(S.ckey + buf * Gmax + q);
-        int li = (int)(unsigned)(k >> 32);
-        if (li < 0) continue;
-        R v = (R)__longlong_as_double((long long)ld_coh((const unsigned long long *)S.candv + buf * Gmax + q));
+#pragma unroll
+      for (int u = 0; u < KEYS_PER_LANE; u++) {
+        const int q = lane + 64 * u;
+        if (q >= G) continue;
+        const unsigned long long k = kk[u];
+        int li;
+        R v;
+        if (kPackKey) {
+          li = (int)((k >> 23) & 511u);
+          if (li == 511) continue;
+          v = (R)__uint_as_float((unsigned)(k >> 32));
+        } else {
+          li = (int)(unsigned)(k >> 32);
+          if (li < 0) continue;
+          v = (R)__longlong_as_double((long long)ld_coh((const unsigned long long *)S.candv + buf * Gmax + q));
+        }
         long ii = r0 + (long)q * NR + li;
         if (v > b || (v == b && ii < p)) { b = v; p = ii; gw = q; }
       }
@@ -701,6 +735,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, panel_fn, NR, 0);
   if ((long)std::max(1, ncu) * std::max(1, perCU) < Gmax) return -5;   // panel would not be co-resident
+  if (Gmax > 64 * KEYS_PER_LANE) return -5;                           // more workgroups than polled keys
   const long npanels = (n + PW - 1) / PW;
   // panel scratch, carved from one workspace: barrier words (3 cache
   // lines), info, all panel plans, candidate values / indices / rows,
