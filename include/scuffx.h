@@ -71,6 +71,34 @@ int scuffx_geom_panels(const scuffx_geom *G, int ns, double *out8);
 /* vertex coordinates of surface ns after its birth transform, 3 per vertex
    (RWGSurface::Vertices, libscuff.h:249) */
 int scuffx_geom_vertices(const scuffx_geom *G, int ns, double *out3);
+
+/* ---- geometrical transformations (.trans files) ----
+   A scuffx_trans holds the list of GTComplex of a .trans file
+   (ReadTransFile, GTransformation.cc:531-580): TRANSFORMATION tag ...
+   ENDTRANSFORMATION sections and one-line TRANS entries; NULL or "" gives
+   one identity entry tagged DEFAULT. */
+typedef struct scuffx_trans scuffx_trans;
+int scuffx_trans_open(const char *path, scuffx_trans **T);
+int scuffx_trans_count(const scuffx_trans *T);
+/* tag of entry i, NUL-terminated into buf[len] */
+int scuffx_trans_tag(const scuffx_trans *T, int i, char *buf, int len);
+int scuffx_trans_free(scuffx_trans *T);
+/* surface label (OBJECT label, mesh file, or ByHand; RWGSurface::Label) */
+int scuffx_geom_label(const scuffx_geom *G, int ns, char *buf, int len);
+int scuffx_geom_set_label(scuffx_geom *G, int ns, const char *label);
+/* RWGGeometry::CheckGTCList (RWGGeometry.cc:931-943): 0 if every
+   transformation of every entry names an existing surface */
+int scuffx_geom_check_trans(const scuffx_geom *G, const scuffx_trans *T);
+/* RWGGeometry::Transform(GTCs[i]) (RWGGeometry.cc:888-912): moves the named
+   surfaces (vertices, edge centroids; panels re-initialized) and sets
+   SurfaceMoved; fails without moving anything if a label is unknown.
+   The next assembly rebuilds the device geometry and near-pair lists; FIPPI
+   records come from the process-wide cache where their keys recur. */
+int scuffx_geom_transform(scuffx_geom *G, const scuffx_trans *T, int i);
+/* RWGGeometry::UnTransform (RWGGeometry.cc:917-921) */
+int scuffx_geom_untransform(scuffx_geom *G);
+/* SurfaceMoved[ns] of the last scuffx_geom_transform (RWGGeometry.cc:774-781) */
+int scuffx_geom_surface_moved(const scuffx_geom *G, int ns);
 /* test-mesh generator: V[3*(10*4^L+2)], T[3*20*4^L] */
 int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T);
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -29,19 +30,38 @@ struct DevRhs {
   int *PEI = nullptr, *EV = nullptr;
 };
 
+struct scuffx_trans {
+  std::vector<GTComplex> C;
+};
+
 struct scuffx_geom {
   Geometry G;
   std::unique_ptr<Assembler> A;
+  bool frozen = false;          // surfaces fixed once an assembler existed
+  bool transformed = false;     // a surface moved since the last UnTransform
   std::vector<DevRhs> rhs;
   double *d_out = nullptr;
   ~scuffx_geom()
   {
-    for (auto &r : rhs) { hipFree(r.PA); hipFree(r.PZ); hipFree(r.work); hipFree(r.PEI); hipFree(r.EV); }
+    drop_rhs();
     hipFree(d_out);
+  }
+  void drop_rhs()
+  {
+    for (auto &r : rhs) { hipFree(r.PA); hipFree(r.PZ); hipFree(r.work); hipFree(r.PEI); hipFree(r.EV); }
+    rhs.clear();
+  }
+  // the geometry moved: device copies are rebuilt on their next use
+  void moved()
+  {
+    if (A || !rhs.empty()) hipStreamSynchronize(g_stream);
+    A.reset();
+    drop_rhs();
   }
   Assembler *assembler()
   {
     if (!A) A.reset(new Assembler(G));
+    frozen = true;
     return A.get();
   }
 };
@@ -72,7 +92,7 @@ int scuffx_geom_open(const char *path, scuffx_geom **G)
 static int add_surface_checked(scuffx_geom *G, const double *V, int nV, const int *T, int nT, const char *mat,
                                const char *meshfile = nullptr)
 {
-  if (G->A) return fail(SCUFFX_ERR_ARG, "geometry is frozen after its first assembly");
+  if (G->frozen) return fail(SCUFFX_ERR_ARG, "geometry is frozen after its first assembly");
   Material m;
   bool pec;
   std::string e = parse_material(mat ? mat : "PEC", &m, &pec);
@@ -82,6 +102,7 @@ static int add_surface_checked(scuffx_geom *G, const double *V, int nV, const in
   if (!e.empty()) return fail(SCUFFX_ERR_ARG, e);
   S.MeshFile = meshfile ? meshfile : "";
   S.MaterialName = pec ? std::string("PEC") : std::string(mat);
+  S.Label = meshfile ? meshfile : "ByHand";
   return add_surface(&G->G, std::move(S), m);
 }
 
@@ -162,6 +183,84 @@ int scuffx_geom_vertices(const scuffx_geom *G, int ns, double *out)
   const Surface &S = G->G.Surfaces[ns];
   for (size_t i = 0; i < S.V.size(); i++) out[i] = S.V[i];
   return 0;
+}
+
+static int copy_str(const std::string &v, char *buf, int len)
+{
+  if (!buf || len <= 0) return fail(SCUFFX_ERR_ARG, "no buffer");
+  std::snprintf(buf, (size_t)len, "%s", v.c_str());
+  return (int)v.size() < len ? 0 : fail(SCUFFX_ERR_ARG, "buffer too short");
+}
+
+int scuffx_trans_open(const char *path, scuffx_trans **T)
+{
+  if (!T) return fail(SCUFFX_ERR_ARG, "null handle pointer");
+  scuffx_trans *t = new scuffx_trans();
+  std::string e = read_trans_file(path ? path : "", &t->C);
+  if (!e.empty()) { delete t; return fail(SCUFFX_ERR_IO, e); }
+  *T = t;
+  return 0;
+}
+
+int scuffx_trans_count(const scuffx_trans *T) { return T ? (int)T->C.size() : fail(SCUFFX_ERR_ARG, "null handle"); }
+
+int scuffx_trans_tag(const scuffx_trans *T, int i, char *buf, int len)
+{
+  if (!T || i < 0 || i >= (int)T->C.size()) return fail(SCUFFX_ERR_ARG, "bad transformation index");
+  return copy_str(T->C[i].Tag, buf, len);
+}
+
+int scuffx_trans_free(scuffx_trans *T) { delete T; return 0; }
+
+int scuffx_geom_label(const scuffx_geom *G, int ns, char *buf, int len)
+{
+  if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
+  return copy_str(G->G.Surfaces[ns].Label, buf, len);
+}
+
+int scuffx_geom_set_label(scuffx_geom *G, int ns, const char *label)
+{
+  if (!G || !label || !*label || ns < 0 || ns >= (int)G->G.Surfaces.size())
+    return fail(SCUFFX_ERR_ARG, "bad surface index or label");
+  G->G.Surfaces[ns].Label = label;
+  return 0;
+}
+
+int scuffx_geom_check_trans(const scuffx_geom *G, const scuffx_trans *T)
+{
+  if (!G || !T) return fail(SCUFFX_ERR_ARG, "null argument");
+  for (const GTComplex &C : T->C) {
+    std::string e = check_gtc(G->G, C);
+    if (!e.empty()) return fail(SCUFFX_ERR_ARG, e);
+  }
+  return 0;
+}
+
+int scuffx_geom_transform(scuffx_geom *G, const scuffx_trans *T, int i)
+{
+  if (!G || !T || i < 0 || i >= (int)T->C.size()) return fail(SCUFFX_ERR_ARG, "bad transformation index");
+  std::string e = check_gtc(G->G, T->C[i]);
+  if (!e.empty()) return fail(SCUFFX_ERR_ARG, e);
+  transform_geometry(&G->G, T->C[i]);
+  for (int m : G->G.SurfaceMoved)
+    if (m) { G->moved(); G->transformed = true; break; }
+  return 0;
+}
+
+int scuffx_geom_untransform(scuffx_geom *G)
+{
+  if (!G) return fail(SCUFFX_ERR_ARG, "null geometry");
+  if (!G->transformed) return 0;    // nothing moved since the last UnTransform
+  untransform_geometry(&G->G);
+  G->moved();
+  G->transformed = false;
+  return 0;
+}
+
+int scuffx_geom_surface_moved(const scuffx_geom *G, int ns)
+{
+  if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
+  return ns < (int)G->G.SurfaceMoved.size() ? G->G.SurfaceMoved[ns] : 0;
 }
 
 int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T)

@@ -272,84 +272,234 @@ std::string parse_material(const std::string &s, Material *m, bool *pec)
   return "unsupported material " + s;
 }
 
-// One-time rigid transformation of an OBJECT at birth: X' = R X + D
-// (libs/libscuff/GTransformation.cc:107-142, :186-235, :273-315).  Later
-// lines compose on the left, as the reference's running transformation
-// does (RWGSurface.cc:165-186), and the vertices are transformed before
-// the panels and edges are built (RWGSurface.cc:427-435).
-struct BirthTransform {
-  double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  double D[3] = {0, 0, 0};
-  // v <- (u.v) u + cos(t) (v - (u.v) u) + sin(t) (u x v), same operation
-  // order as the reference so that rotated meshes agree to the bit
-  static void rotate_vec(const double u[3], double c, double sn, double v[3])
-  {
-    double ud = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
-    double par[3] = {u[0] * ud, u[1] * ud, u[2] * ud};
-    double perp[3] = {v[0] - par[0], v[1] - par[1], v[2] - par[2]};
-    double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
-    for (int i = 0; i < 3; i++) v[i] = par[i] + (perp[i] * c + cr[i] * sn);
-  }
-  void displace(const double d[3]) { for (int i = 0; i < 3; i++) D[i] += d[i]; }
-  void rotate(const double axis[3], double degrees)
-  {
-    double n = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
-    double u[3] = {axis[0] / n, axis[1] / n, axis[2] / n};
-    double t = degrees * (3.14159265358979323846 / 180.0);
-    double c = std::cos(t), sn = std::sin(t);
-    rotate_vec(u, c, sn, D);
-    for (int j = 0; j < 3; j++) {
-      double col[3] = {R[0][j], R[1][j], R[2][j]};
-      rotate_vec(u, c, sn, col);
-      for (int i = 0; i < 3; i++) R[i][j] = col[i];
-    }
-  }
-  // this <- B o this
-  void compose(const BirthTransform &B)
-  {
-    double d[3];
-    for (int i = 0; i < 3; i++) d[i] = B.R[i][0] * D[0] + B.R[i][1] * D[1] + B.R[i][2] * D[2] + B.D[i];
-    double r[3][3];
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) {
-        r[i][j] = 0;
-        for (int k = 0; k < 3; k++) r[i][j] += B.R[i][k] * R[k][j];
-      }
-    std::memcpy(D, d, sizeof D);
-    std::memcpy(R, r, sizeof R);
-  }
-  void apply(std::vector<double> *V) const
-  {
-    for (size_t n = 0; n + 2 < V->size(); n += 3) {
-      double x0 = (*V)[n], x1 = (*V)[n + 1], x2 = (*V)[n + 2];
-      for (int i = 0; i < 3; i++) (*V)[n + i] = R[i][0] * x0 + R[i][1] * x1 + R[i][2] * x2 + D[i];
-    }
-  }
-};
+// ---------------------------------------------------------------------
+// rigid transformations (GTransformation.cc).  An OBJECT's DISPLACED /
+// ROTATED lines compose on the left into one transformation applied to
+// the vertices before the panels and edges are built (RWGSurface.cc:
+// 165-186, :427-435); .trans files move surfaces of a built geometry.
+// ---------------------------------------------------------------------
+// v <- (u.v) u + cos(t) (v - (u.v) u) + sin(t) (u x v), in the
+// reference's operation order so that rotated meshes agree to the bit
+static void rotate_vec(const double u[3], double c, double sn, double v[3])
+{
+  double ud = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+  double par[3] = {u[0] * ud, u[1] * ud, u[2] * ud};
+  double perp[3] = {v[0] - par[0], v[1] - par[1], v[2] - par[2]};
+  double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+  for (int i = 0; i < 3; i++) v[i] = par[i] + (perp[i] * c + cr[i] * sn);
+}
 
-// parses "DISPLACED x y z" or "ROTATED t ABOUT x y z"
-static std::string parse_transform(const std::vector<std::string> &tok, BirthTransform *out)
+void GTransform::displace(const double d[3]) { for (int i = 0; i < 3; i++) DX[i] += d[i]; }
+
+void GTransform::rotate(const double axis[3], double degrees)
+{
+  double n = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  double u[3] = {axis[0] / n, axis[1] / n, axis[2] / n};
+  double t = degrees * (3.14159265358979323846 / 180.0);
+  double c = std::cos(t), sn = std::sin(t);
+  rotate_vec(u, c, sn, DX);
+  for (int j = 0; j < 3; j++) {
+    double col[3] = {M[0][j], M[1][j], M[2][j]};
+    rotate_vec(u, c, sn, col);
+    for (int i = 0; i < 3; i++) M[i][j] = col[i];
+  }
+}
+
+void GTransform::compose(const GTransform &B)
+{
+  double d[3];
+  for (int i = 0; i < 3; i++) d[i] = B.M[i][0] * DX[0] + B.M[i][1] * DX[1] + B.M[i][2] * DX[2] + B.DX[i];
+  double r[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      r[i][j] = 0;
+      for (int k = 0; k < 3; k++) r[i][j] += B.M[i][k] * M[k][j];
+    }
+  std::memcpy(DX, d, sizeof DX);
+  std::memcpy(M, r, sizeof M);
+}
+
+void GTransform::apply(double *X, size_t npoints) const
+{
+  for (size_t n = 0; n < npoints; n++) {
+    double *x = X + 3 * n;
+    double x0 = x[0], x1 = x[1], x2 = x[2];
+    for (int i = 0; i < 3; i++) x[i] = M[i][0] * x0 + M[i][1] * x1 + M[i][2] * x2 + DX[i];
+  }
+}
+
+void GTransform::unapply(double *X, size_t npoints) const
+{
+  for (size_t n = 0; n < npoints; n++) {
+    double *x = X + 3 * n;
+    double x0 = x[0] - DX[0], x1 = x[1] - DX[1], x2 = x[2] - DX[2];
+    for (int i = 0; i < 3; i++) x[i] = M[0][i] * x0 + M[1][i] * x1 + M[2][i] * x2;
+  }
+}
+
+std::string parse_transform(const std::vector<std::string> &tok, size_t first, GTransform *out, size_t *consumed)
 {
   char *end;
   auto num = [&](const std::string &s, double *v) {
     *v = std::strtod(s.c_str(), &end);
     return end != s.c_str() && *end == 0;
   };
-  if (!strcasecmp(tok[0].c_str(), "DISPLACED") || !strcasecmp(tok[0].c_str(), "DISP")) {
+  const size_t n = tok.size() - first;
+  *consumed = 0;
+  if (n == 0) return "no transformation specified";
+  const std::string &kw = tok[first];
+  if (!strcasecmp(kw.c_str(), "DISPLACED") || !strcasecmp(kw.c_str(), "DISP")) {
     double d[3];
-    if (tok.size() != 4) return "DISPLACED needs 3 values";
+    if (n < 4) return "too few values specified for " + kw;
     for (int i = 0; i < 3; i++)
-      if (!num(tok[1 + i], d + i)) return "bad value in DISPLACED";
+      if (!num(tok[first + 1 + i], d + i)) return "bad value specified for " + kw;
     out->displace(d);
+    *consumed = 4;
     return "";
   }
-  double t, a[3];
-  if (tok.size() != 6 || strcasecmp(tok[2].c_str(), "ABOUT")) return "ROTATED needs: ROTATED angle ABOUT x y z";
-  if (!num(tok[1], &t)) return "bad angle in ROTATED";
-  for (int i = 0; i < 3; i++)
-    if (!num(tok[3 + i], a + i)) return "bad axis in ROTATED";
-  out->rotate(a, t);
+  if (!strcasecmp(kw.c_str(), "ROTATED") || !strcasecmp(kw.c_str(), "ROT")) {
+    double t, a[3];
+    if (n < 6) return "too few values specified for " + kw;
+    if (strcasecmp(tok[first + 2].c_str(), "ABOUT")) return "invalid syntax for " + kw + " statement";
+    if (!num(tok[first + 1], &t)) return "bad value specified for " + kw;
+    for (int i = 0; i < 3; i++)
+      if (!num(tok[first + 3 + i], a + i)) return "bad value specified for " + kw;
+    out->rotate(a, t);
+    *consumed = 6;
+    return "";
+  }
+  return "unknown keyword " + kw;
+}
+
+static std::vector<std::string> tokenize(const std::string &line)
+{
+  std::stringstream ss(line);
+  std::vector<std::string> tok;
+  std::string w;
+  while (ss >> w) tok.push_back(w);
+  return tok;
+}
+
+std::string read_trans_file(const std::string &fn, std::vector<GTComplex> *out)
+{
+  out->clear();
+  if (fn.empty()) {
+    GTComplex C;
+    C.Tag = "DEFAULT";
+    out->push_back(C);
+    return "";
+  }
+  std::ifstream in(fn);
+  if (!in) return "could not open file " + fn;
+  std::string line;
+  int lineno = 0;
+  while (std::getline(in, line)) {
+    lineno++;
+    std::vector<std::string> tok = tokenize(line);
+    if (tok.empty() || tok[0][0] == '#') continue;
+    std::string where = fn + ":" + std::to_string(lineno) + ": ";
+    GTComplex C;
+    int cur = -1;   // the transformation later DISP / ROT apply to
+    auto new_gt = [&](const std::string &label) {
+      C.GTs.push_back(GTransform());
+      C.Labels.push_back(label);
+      cur = (int)C.GTs.size() - 1;
+    };
+    if (!strcasecmp(tok[0].c_str(), "TRANSFORMATION")) {
+      C.Tag = tok.size() > 1 ? tok[1] : "DEFAULT";
+      bool closed = false;
+      while (!closed && std::getline(in, line)) {
+        lineno++;
+        where = fn + ":" + std::to_string(lineno) + ": ";
+        std::vector<std::string> t = tokenize(line);
+        if (t.empty() || t[0][0] == '#') continue;
+        if (!strcasecmp(t[0].c_str(), "OBJECT") || !strcasecmp(t[0].c_str(), "SURFACE")) {
+          if (t.size() != 2) return where + "OBJECT/SURFACE keyword requires one argument";
+          new_gt(t[1]);
+        } else if (!strcasecmp(t[0].c_str(), "ENDTRANSFORMATION")) {
+          closed = true;
+        } else {
+          if (cur < 0) new_gt("");
+          size_t used;
+          std::string e = parse_transform(t, 0, &C.GTs[cur], &used);
+          if (!e.empty()) return where + e;
+          if (used != t.size()) return where + "junk at end of line";
+        }
+      }
+      if (!closed) return fn + ": unexpected end of file in TRANSFORMATION " + C.Tag;
+    } else if (!strcasecmp(tok[0].c_str(), "TRANS")) {
+      if (tok.size() < 2) return where + "TRANS without a tag";
+      C.Tag = tok[1];
+      for (size_t nt = 2; nt < tok.size();) {
+        if (!strcasecmp(tok[nt].c_str(), "OBJECT") || !strcasecmp(tok[nt].c_str(), "SURFACE")) {
+          if (nt + 1 == tok.size()) return where + "empty " + tok[nt] + " statement";
+          new_gt(tok[nt + 1]);
+          nt += 2;
+        } else if (!strcasecmp(tok[nt].c_str(), "DISP") || !strcasecmp(tok[nt].c_str(), "DISPLACED") ||
+                   !strcasecmp(tok[nt].c_str(), "ROT") || !strcasecmp(tok[nt].c_str(), "ROTATED")) {
+          if (cur < 0) new_gt("");
+          size_t used;
+          std::string e = parse_transform(tok, nt, &C.GTs[cur], &used);
+          if (!e.empty()) return where + e;
+          nt += used;
+        } else {
+          return where + "unknown keyword " + tok[nt];
+        }
+      }
+    } else {
+      return where + "unknown token " + tok[0];
+    }
+    out->push_back(C);
+  }
   return "";
+}
+
+int surface_by_label(const Geometry &G, const std::string &label)
+{
+  for (size_t ns = 0; ns < G.Surfaces.size(); ns++)
+    if (!strcasecmp(label.c_str(), G.Surfaces[ns].Label.c_str())) return (int)ns;
+  return -1;
+}
+
+std::string check_gtc(const Geometry &G, const GTComplex &C)
+{
+  for (size_t k = 0; k < C.GTs.size(); k++) {
+    if (C.Labels[k].empty()) return "no OBJECT/SURFACE specified for GTransformation " + C.Tag;
+    if (surface_by_label(G, C.Labels[k]) < 0)
+      return "unknown OBJECT/SURFACE " + C.Labels[k] + " specified for GTransformation " + C.Tag;
+  }
+  return "";
+}
+
+static void reinit_panels(Surface *S)
+{
+  for (Panel &P : S->Panels) init_panel(&P, S->V.data());
+}
+
+void transform_geometry(Geometry *G, const GTComplex &C)
+{
+  G->SurfaceMoved.assign(G->Surfaces.size(), 0);
+  for (size_t k = 0; k < C.GTs.size(); k++) {
+    const int ns = C.Labels[k].empty() ? -1 : surface_by_label(*G, C.Labels[k]);
+    if (ns < 0) continue;   // the reference warns and ignores it
+    Surface &S = G->Surfaces[ns];
+    const GTransform &D = C.GTs[k];
+    D.apply(S.V.data(), S.V.size() / 3);
+    for (Edge &E : S.Edges) D.apply(E.Centroid, 1);
+    reinit_panels(&S);
+    S.GT.compose(D);
+    G->SurfaceMoved[ns] = 1;
+  }
+}
+
+void untransform_geometry(Geometry *G)
+{
+  for (Surface &S : G->Surfaces) {
+    S.GT.unapply(S.V.data(), S.V.size() / 3);
+    for (Edge &E : S.Edges) S.GT.unapply(E.Centroid, 1);
+    S.GT.reset();
+    reinit_panels(&S);
+  }
 }
 
 std::string read_scuffgeo(const std::string &fn, Geometry *G)
@@ -359,9 +509,9 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
   std::string dir;
   size_t sl = fn.rfind('/');
   if (sl != std::string::npos) dir = fn.substr(0, sl + 1);
-  std::string line, mesh, mat = "PEC";
+  std::string line, mesh, mat = "PEC", label;
   bool inobj = false, moved = false;
-  BirthTransform T;
+  GTransform T;
   int lineno = 0;
   while (std::getline(in, line)) {
     lineno++;
@@ -376,7 +526,8 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
     std::string where = fn + ":" + std::to_string(lineno) + ": ";
     if (!strcasecmp(kw.c_str(), "OBJECT")) {
       if (inobj) return where + "OBJECT inside OBJECT";
-      inobj = true; mesh.clear(); mat = "PEC"; moved = false; T = BirthTransform();
+      inobj = true; mesh.clear(); mat = "PEC"; moved = false; T = GTransform();
+      label = tok.size() > 1 ? tok[1] : "";
     } else if (!inobj) {
       return where + "unsupported keyword " + kw + " (only OBJECT sections are handled)";
     } else if (!strcasecmp(kw.c_str(), "MESHFILE")) {
@@ -387,9 +538,11 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
       mat = tok[1];
     } else if (!strcasecmp(kw.c_str(), "DISPLACED") || !strcasecmp(kw.c_str(), "DISP") ||
                !strcasecmp(kw.c_str(), "ROTATED") || !strcasecmp(kw.c_str(), "ROT")) {
-      BirthTransform B;
-      std::string e = parse_transform(tok, &B);
+      GTransform B;
+      size_t used;
+      std::string e = parse_transform(tok, 0, &B, &used);
       if (!e.empty()) return where + e;
+      if (used != tok.size()) return where + "junk at end of line";
       if (moved) T.compose(B); else T = B;
       moved = true;
     } else if (!strcasecmp(kw.c_str(), "ENDOBJECT")) {
@@ -397,7 +550,7 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
       std::vector<double> V; std::vector<int> Tri;
       std::string err = read_gmsh(mesh[0] == '/' ? mesh : dir + mesh, &V, &Tri);
       if (!err.empty()) return err;
-      if (moved) T.apply(&V);
+      if (moved) T.apply(V.data(), V.size() / 3);
       Material m; bool pec;
       err = parse_material(mat, &m, &pec);
       if (!err.empty()) return where + err;
@@ -406,6 +559,7 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
       if (!err.empty()) return where + err;
       S.MeshFile = mesh;
       S.MaterialName = pec ? std::string("PEC") : mat;
+      S.Label = label.empty() ? mesh : label;
       add_surface(G, std::move(S), m);
       inobj = false;
     } else {

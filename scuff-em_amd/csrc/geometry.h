@@ -41,6 +41,33 @@ struct Material {
   void eps_mu(cdouble omega, cdouble *e, cdouble *m) const;
 };
 
+// rigid transformation X' = M X + DX (GTransformation.cc:26-142,
+// :234-315); rotations and compositions in the reference's operation order
+struct GTransform {
+  double M[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double DX[3] = {0, 0, 0};
+  void reset() { *this = GTransform(); }
+  void displace(const double d[3]);
+  void rotate(const double axis[3], double degrees);
+  void compose(const GTransform &B);          // this <- B o this (B after this)
+  void apply(double *X, size_t npoints) const;   // in place
+  void unapply(double *X, size_t npoints) const;
+};
+// "DISPLACED x y z" / "ROTATED t ABOUT x y z" (and DISP / ROT) at tok[0];
+// applies it to *out; *consumed = tokens used (GTransformation.cc:159-245)
+std::string parse_transform(const std::vector<std::string> &tok, size_t first, GTransform *out, size_t *consumed);
+
+// one named set of surface transformations (GTComplex, GTransformation.h)
+struct GTComplex {
+  std::string Tag;
+  std::vector<std::string> Labels;   // empty label: none was given
+  std::vector<GTransform> GTs;
+};
+// .trans file: TRANSFORMATION tag ... ENDTRANSFORMATION sections and
+// one-line TRANS tag OBJECT a DISP ... entries (GTransformation.cc:373-580);
+// an empty file name gives one identity complex tagged DEFAULT
+std::string read_trans_file(const std::string &fn, std::vector<GTComplex> *out);
+
 struct Surface {
   std::vector<double> V;        // 3*NumVertices
   std::vector<Panel> Panels;
@@ -51,6 +78,8 @@ struct Surface {
   int NumBFs = 0;
   std::string MeshFile;         // as named by the caller; empty for raw arrays
   std::string MaterialName;     // interior material as given ("PEC" for PEC)
+  std::string Label;            // OBJECT label, mesh file name, or ByHand
+  GTransform GT;                // transformations since the last UnTransform
   int NumVertices() const { return (int)(V.size() / 3); }
   int NumPanels() const { return (int)Panels.size(); }
   int NumEdges() const { return (int)Edges.size(); }
@@ -60,9 +89,22 @@ struct Geometry {
   std::vector<Surface> Surfaces;
   std::vector<Material> Regions;     // Regions[0] is the exterior medium
   std::vector<int> BFIndexOffset;
+  std::vector<int> SurfaceMoved;     // set by the last transform_geometry
   int TotalBFs = 0;
   Geometry() { Regions.push_back(Material()); }
 };
+
+// index of the surface with this label (case-insensitive), or -1
+int surface_by_label(const Geometry &G, const std::string &label);
+// empty if every transformation of C names an existing surface
+// (RWGGeometry::CheckGTCList, RWGGeometry.cc:931-943)
+std::string check_gtc(const Geometry &G, const GTComplex &C);
+// RWGGeometry::Transform / UnTransform (RWGGeometry.cc:888-921,
+// RWGSurface.cc:588-667): vertices and edge centroids are transformed,
+// panels re-initialized from the vertices; UnTransform applies the
+// inverse of the accumulated transformation to the current coordinates
+void transform_geometry(Geometry *G, const GTComplex &C);
+void untransform_geometry(Geometry *G);
 
 // Builds a surface from vertex coordinates and triangle vertex indices.
 // Returns an empty string on success, an error message otherwise.

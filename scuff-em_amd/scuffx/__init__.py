@@ -60,6 +60,16 @@ def lib():
             "scuffx_geom_panels": ([vp, C.c_int, vp], C.c_int),
             "scuffx_geom_vertices": ([vp, C.c_int, vp], C.c_int),
             "scuffx_geom_mate": ([vp, C.c_int], C.c_int),
+            "scuffx_trans_open": ([C.c_char_p, vp], C.c_int),
+            "scuffx_trans_count": ([vp], C.c_int),
+            "scuffx_trans_tag": ([vp, C.c_int, C.c_char_p, C.c_int], C.c_int),
+            "scuffx_trans_free": ([vp], C.c_int),
+            "scuffx_geom_label": ([vp, C.c_int, C.c_char_p, C.c_int], C.c_int),
+            "scuffx_geom_set_label": ([vp, C.c_int, C.c_char_p], C.c_int),
+            "scuffx_geom_check_trans": ([vp, vp], C.c_int),
+            "scuffx_geom_transform": ([vp, vp, C.c_int], C.c_int),
+            "scuffx_geom_untransform": ([vp], C.c_int),
+            "scuffx_geom_surface_moved": ([vp, C.c_int], C.c_int),
             "scuffx_icosphere": ([C.c_int, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_fippi_build": ([vp], C.c_int),
             "scuffx_fippi_count": ([vp], C.c_int),
@@ -108,6 +118,14 @@ def _torch():
 def _sync_stream():
     torch = _torch()
     lib().scuffx_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+def _sync_stream_if_gpu():
+    """the caller's stream, where a GPU is present (geometry edits work
+    without one)"""
+    import torch
+    if torch.cuda.is_available():
+        _sync_stream()
 
 
 def _ptr(t):
@@ -296,6 +314,44 @@ class PlaneWave:
         self.nHat = np.asarray(nHat, dtype=np.float64)
 
 
+class GTComplex:
+    """one entry of a .trans file: a tag and the transformations of the
+    surfaces it names (GTransformation.h); index i into its GTCList"""
+
+    def __init__(self, lst, i):
+        self.List, self.Index = lst, i
+        buf = C.create_string_buffer(1024)
+        _check(lib().scuffx_trans_tag(lst.h, i, buf, len(buf)))
+        self.Tag = buf.value.decode()
+
+    def __repr__(self):
+        return "GTComplex(%r)" % self.Tag
+
+
+class GTCList(list):
+    """the GTComplex entries of a .trans file, in file order"""
+
+    def __init__(self, h):
+        super().__init__()
+        self.h = h
+        for i in range(_check(lib().scuffx_trans_count(h))):
+            self.append(GTComplex(self, i))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _LIB is not None:
+            _LIB.scuffx_trans_free(self.h)
+            self.h = C.c_void_p()
+
+
+def ReadTransFile(FileName=None):
+    """reads a .trans file (GTransformation.cc:531-580): TRANSFORMATION tag
+    ... ENDTRANSFORMATION sections and one-line TRANS entries; no file
+    name gives a single identity transformation tagged DEFAULT"""
+    h = C.c_void_p()
+    _check(lib().scuffx_trans_open(FileName.encode() if FileName else None, C.byref(h)))
+    return GTCList(h)
+
+
 class RWGGeometry:
     """Surfaces, RWG edges and regions; BEM assembly on the GPU."""
 
@@ -311,12 +367,15 @@ class RWGGeometry:
             _LIB.scuffx_geom_free(self.h)
             self.h = C.c_void_p()
 
-    def AddMesh(self, V, T, Material="PEC"):
+    def AddMesh(self, V, T, Material="PEC", Label=None):
         V = np.ascontiguousarray(V, dtype=np.float64)
         T = np.ascontiguousarray(T, dtype=np.int32)
-        return _check(lib().scuffx_geom_add_mesh(self.h, V.ctypes.data_as(C.c_void_p), V.size // 3,
-                                                 T.ctypes.data_as(C.c_void_p), T.size // 3,
-                                                 Material.encode()))
+        ns = _check(lib().scuffx_geom_add_mesh(self.h, V.ctypes.data_as(C.c_void_p), V.size // 3,
+                                               T.ctypes.data_as(C.c_void_p), T.size // 3,
+                                               Material.encode()))
+        if Label is not None:
+            self.SetLabel(ns, Label)
+        return ns
 
     def AddMeshFile(self, path, Material="PEC"):
         return _check(lib().scuffx_geom_add_msh(self.h, path.encode(), Material.encode()))
@@ -352,6 +411,44 @@ class RWGGeometry:
         """Mate[ns]: the first earlier surface identical to ns, or -1
         (RWGGeometry.cc:733-770)"""
         return [lib().scuffx_geom_mate(self.h, ns) for ns in range(self.NumSurfaces)]
+
+    def Label(self, ns=0):
+        """surface label: OBJECT label, mesh file name, or ByHand"""
+        buf = C.create_string_buffer(4096)
+        _check(lib().scuffx_geom_label(self.h, ns, buf, len(buf)))
+        return buf.value.decode()
+
+    def SetLabel(self, ns, Label):
+        _check(lib().scuffx_geom_set_label(self.h, ns, Label.encode()))
+
+    def GetSurfaceByLabel(self, Label):
+        """index of the surface with this label (case-insensitive), or -1
+        (RWGGeometry.cc:868-880)"""
+        for ns in range(self.NumSurfaces):
+            if self.Label(ns).lower() == Label.lower():
+                return ns
+        return -1
+
+    def CheckGTCList(self, GTCs):
+        """raises if a transformation names no surface or an unknown one
+        (RWGGeometry.cc:931-943)"""
+        _check(lib().scuffx_geom_check_trans(self.h, GTCs.h))
+
+    def Transform(self, GTC):
+        """applies one .trans entry (RWGGeometry.cc:888-912): the named
+        surfaces move, SurfaceMoved flags them"""
+        _sync_stream_if_gpu()
+        _check(lib().scuffx_geom_transform(self.h, GTC.List.h, GTC.Index))
+
+    def UnTransform(self):
+        """undoes every transformation since the last UnTransform
+        (RWGGeometry.cc:917-921)"""
+        _sync_stream_if_gpu()
+        _check(lib().scuffx_geom_untransform(self.h))
+
+    @property
+    def SurfaceMoved(self):
+        return [bool(_check(lib().scuffx_geom_surface_moved(self.h, ns))) for ns in range(self.NumSurfaces)]
 
     def Vertices(self, ns=0):
         """vertex coordinates of surface ns, shape (NumVertices, 3)"""

@@ -16,16 +16,22 @@ integral uses the reference's default 'cliff' rule
 0.7/d_min of SumsIntegrals.cc:606-660) or its fixed trapezoid/Simpson
 rule (SumsIntegrals.cc:532-587).
 
+With a transformation file the quantities are computed for every entry
+(NumTransformations x NumQuantities values per Xi, entry-major): the T
+blocks are assembled once per Xi at the untransformed positions, and a U
+block is reassembled only when one of its two surfaces has moved under
+some entry so far at this Xi (CasimirIntegrand.cc:370-494).
+
 Every block, factorization, determinant and trace runs on the GPU through
 the C ABI; this module only orchestrates them and moves device blocks
 into place.  Scope: compact (non-periodic) geometries at zero
-temperature, one spatial configuration per geometry object.
+temperature.
 """
 import math
 
 import numpy as np
 
-from . import HMatrix, SCUFFXError
+from . import HMatrix, SCUFFXError, ReadTransFile
 
 XIMIN_DEFAULT = 0.001          # scuff-cas3D.cc:80
 RELTOL_DEFAULT = 1.0e-2        # scuff-cas3D.cc:84
@@ -39,11 +45,15 @@ class SC3Data:
     at every frequency.
 
     Quantities are packed in the reference's order: energy, x-, y-,
-    z-force, then one torque per axis in TorqueAxes."""
+    z-force, then one torque per axis in TorqueAxes; with a TransFile,
+    one such group per transformation in file order."""
 
     def __init__(self, G, Energy=False, XForce=False, YForce=False, ZForce=False, TorqueAxes=None,
-                 XiMin=XIMIN_DEFAULT, AbsTol=0.0, RelTol=RELTOL_DEFAULT):
+                 XiMin=XIMIN_DEFAULT, AbsTol=0.0, RelTol=RELTOL_DEFAULT, TransFile=None):
         self.G = G
+        self.GTCs = ReadTransFile(TransFile)          # CreateSC3Data.cc:67-77
+        G.CheckGTCList(self.GTCs)
+        self.NumTransformations = len(self.GTCs)
         self.NS = G.NumSurfaces
         self.N = G.TotalBFs
         self.N1 = G.BFOffset(0)[1]
@@ -58,6 +68,7 @@ class SC3Data:
         if self.Mus and self.NS < 2:
             raise SCUFFXError("forces and torques need at least two surfaces")
         self.NumQuantities = int(self.Energy) + len(self.Mus)
+        self.NTNQ = self.NumTransformations * self.NumQuantities
         self.XiMin, self.AbsTol, self.RelTol = XiMin, AbsTol, RelTol
         self.Mate = G.Mate
         self.M = HMatrix(self.N, self.N)
@@ -67,55 +78,81 @@ class SC3Data:
     def _offsets(self):
         return [self.G.BFOffset(ns) for ns in range(self.NS)]
 
-    def GetCasimirIntegrand(self, Xi):
-        """EFT at imaginary frequency Xi (GetCasimirIntegrand,
-        CasimirIntegrand.cc:298-473, for a compact geometry)."""
+    def GetCasimirIntegrand(self, Xi, Converged=None):
+        """EFT at imaginary frequency Xi for every transformation
+        (GetCasimirIntegrand, CasimirIntegrand.cc:298-494, compact
+        geometry).  Converged (NTNQ flags, optional): a transformation
+        whose quantities have all converged is skipped and reports zeros."""
         G, M = self.G, self.M
         Omega = complex(0.0, Xi)
+        NQ = self.NumQuantities
         offs = self._offsets()
-        # T blocks into M; the LU diagonals of the T blocks make up that of
-        # M_inf; a surface identical to an earlier one (Mate) reuses its
-        # block and its diagonal (CasimirIntegrand.cc:276-291, :323-337)
-        M.t.zero_()
-        M.factored = False
+        # T blocks, at the untransformed positions; a surface identical to
+        # an earlier one (Mate) reuses its block and its LU diagonal
+        # (CasimirIntegrand.cc:276-337)
+        TBlocks, ld = {}, {}
         logdet_inf = 0.0
-        ld = {}
         for ns, (o, n) in enumerate(offs):
             nm = self.Mate[ns]
             if nm >= 0:
-                om = offs[nm][0]
-                M.t[o:o + n, o:o + n].copy_(M.t[om:om + n, om:om + n])
+                TBlocks[ns] = TBlocks[nm]
                 if self.Energy:
                     ld[ns] = ld[nm]
                     logdet_inf += ld[ns]
                 continue
             T = G.AssembleBEMMatrixBlock(ns, ns, Omega)
-            M.t[o:o + n, o:o + n].copy_(T.t)
+            TBlocks[ns] = T
             if self.Energy:
-                T.LUFactorize()
-                ld[ns] = T.LogDet().real
+                TF = HMatrix(n, n, data=T.t.clone())
+                TF.LUFactorize()
+                ld[ns] = TF.LogDet().real
                 logdet_inf += ld[ns]
-        # U blocks (upper triangle, adjoint below) and dU_{0,s}
-        dU = {}
-        nt = len(self.TorqueAxes)
-        for ns in range(self.NS):
-            for nsp in range(ns + 1, self.NS):
-                (o, n), (op, nq) = offs[ns], offs[nsp]
-                if ns == 0 and self.Mus:
-                    r = G.AssembleBEMMatrixBlock(0, nsp, Omega, GradM=True,
-                                                 TorqueAxes=self.TorqueAxes if nt else None)
-                    U = r[0]
-                    dU[nsp] = list(r[1]) + (list(r[2]) if nt else [])
-                else:
-                    U = G.AssembleBEMMatrixBlock(ns, nsp, Omega)
-                M.t[op:op + nq, o:o + n].copy_(U.t)                      # InsertBlock
-                M.t[o:o + n, op:op + nq].copy_(U.t.transpose(0, 1).conj())   # InsertBlockAdjoint
-        # Factorize (CasimirIntegrand.cc:152-184)
+        nt_q = len(self.TorqueAxes)
+        never_moved = [True] * self.NS
+        UBlocks, dU = {}, {}
+        EFT = np.zeros(self.NTNQ)
+        for nt, GTC in enumerate(self.GTCs):
+            if Converged is not None and all(Converged[nt * NQ:(nt + 1) * NQ]):
+                continue                                   # CasimirIntegrand.cc:385-418
+            G.Transform(GTC)
+            for ns, moved in enumerate(G.SurfaceMoved):
+                if moved:
+                    never_moved[ns] = False
+            # U blocks and dU_{0,s}; reused when neither surface has moved
+            for ns in range(self.NS):
+                for nsp in range(ns + 1, self.NS):
+                    if (ns, nsp) in UBlocks and never_moved[ns] and never_moved[nsp]:
+                        continue
+                    if ns == 0 and self.Mus:
+                        r = G.AssembleBEMMatrixBlock(0, nsp, Omega, GradM=True,
+                                                     TorqueAxes=self.TorqueAxes if nt_q else None)
+                        UBlocks[(ns, nsp)] = r[0]
+                        dU[nsp] = list(r[1]) + (list(r[2]) if nt_q else [])
+                    else:
+                        UBlocks[(ns, nsp)] = G.AssembleBEMMatrixBlock(ns, nsp, Omega)
+            EFT[nt * NQ:(nt + 1) * NQ] = self._quantities(offs, TBlocks, UBlocks, dU, logdet_inf)
+            G.UnTransform()
+        self.ByXi.append((Xi, EFT.copy()))
+        return EFT
+
+    def _quantities(self, offs, TBlocks, UBlocks, dU, logdet_inf):
+        """stamps the blocks into M, factorizes it and reads the energy,
+        force and torque integrands (Factorize, GetLNDetMInvMInf,
+        GetTraceMInvdM; CasimirIntegrand.cc:51-184)"""
+        M = self.M
+        M.t.zero_()
+        M.factored = False
+        for ns, (o, n) in enumerate(offs):
+            M.t[o:o + n, o:o + n].copy_(TBlocks[ns].t)
+        for (ns, nsp), U in UBlocks.items():
+            (o, n), (op, nq) = offs[ns], offs[nsp]
+            M.t[op:op + nq, o:o + n].copy_(U.t)                          # InsertBlock
+            M.t[o:o + n, op:op + nq].copy_(U.t.transpose(0, 1).conj())   # InsertBlockAdjoint
         M.LUFactorize()
-        EFT = []
+        out = []
         if self.Energy:
             v = logdet_inf - M.LogDet().real          # GetLNDetMInvMInf, method 1
-            EFT.append(0.0 if not math.isfinite(v) else -v / (2.0 * math.pi))
+            out.append(0.0 if not math.isfinite(v) else -v / (2.0 * math.pi))
         for Mu in self.Mus:
             dM = self.dM
             dM.t.zero_()
@@ -124,27 +161,28 @@ class SC3Data:
                 D = dU[nsp][Mu]
                 dM.t[:, op:op + nq].copy_(D.t.transpose(0, 1).conj())   # InsertBlockAdjoint(dU, off, 0)
             tr = 2.0 * M.TraceSolve(dM).real
-            EFT.append(0.0 if not math.isfinite(tr) else -tr / (2.0 * math.pi))
-        EFT = np.array(EFT)
-        self.ByXi.append((Xi, EFT.copy()))
-        return EFT
+            out.append(0.0 if not math.isfinite(tr) else -tr / (2.0 * math.pi))
+        return out
 
     def MinSurfaceDistance(self):
-        """smallest vertex-vertex distance between distinct surfaces
-        (SumsIntegrals.cc:614-631); 1.0 if it is zero"""
-        V = [self.G.Vertices(ns) for ns in range(self.NS)]
+        """smallest vertex-vertex distance between distinct surfaces under
+        any transformation (SumsIntegrals.cc:606-631); 1.0 if it is zero"""
         d = math.inf
-        for a in range(self.NS):
-            for b in range(a + 1, self.NS):
-                diff = V[a][:, None, :] - V[b][None, :, :]
-                d = min(d, float(np.sqrt((diff * diff).sum(-1)).min()))
+        for GTC in self.GTCs:
+            self.G.Transform(GTC)
+            V = [self.G.Vertices(ns) for ns in range(self.NS)]
+            for a in range(self.NS):
+                for b in range(a + 1, self.NS):
+                    diff = V[a][:, None, :] - V[b][None, :, :]
+                    d = min(d, float(np.sqrt((diff * diff).sum(-1)).min()))
+            self.G.UnTransform()
         return 1.0 if d == 0.0 or not math.isfinite(d) else d
 
     def GetXiIntegral_Cliff(self):
         """integral over Xi from XiMin to infinity with the cliff rule
         (SumsIntegrals.cc:606-660); returns (EFT, Error)"""
         XiCliff = 0.7 / self.MinSurfaceDistance()
-        return IntegrateCliffFunction(lambda xi, skip: self.GetCasimirIntegrand(xi), self.NumQuantities,
+        return IntegrateCliffFunction(lambda xi, skip: self.GetCasimirIntegrand(xi, skip), self.NTNQ,
                                       self.XiMin, 0.0, XiCliff, self.AbsTol, self.RelTol)
 
     def GetXiIntegral_TrapSimp(self, NumIntervals=50):

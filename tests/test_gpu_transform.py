@@ -1,0 +1,168 @@
+"""GPU parity of the assembly under geometrical transformations
+(RWGGeometry::Transform / UnTransform, RWGGeometry.cc:888-921) and of the
+callers that reuse blocks across the entries of a transformation file:
+the scattering caller's T blocks (scuff-scatter.cc:345-451) and the
+Casimir caller's T and unmoved U blocks (CasimirIntegrand.cc:298-494).
+
+Pins: a transformed geometry assembles to 1e-14 what a geometry built
+directly at the transformed vertex coordinates assembles (edge lengths
+are kept, not recomputed), and to 1e-12 what the oracle assembles there; UnTransform restores the original matrix up to
+the vertices' rounding (1e-9: adaptive cubature); block reuse reproduces the full assembly at every
+entry; the Casimir integrands of a two-entry file equal separate
+geometries (1e-6, T blocks reused from the untransformed positions); and the z-force of the reference's own multi-entry run
+(Spheres.trans over PECSpheres_327.scuffgeo) within its error estimates."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+MESH = os.path.join(GOLDEN, "Sphere_327.msh")
+
+
+def write(tmp_path, text, name="t.trans"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def pair(sx, O, level, mat, kind, eps, dz=-2.6):
+    V, T = O.icosphere(level)
+    G = sx.RWGGeometry()
+    G.AddMesh(V, T, mat, Label="A")
+    G.AddMesh(V + np.array([0.0, 0.0, dz]), T, mat, Label="B")
+    return G, T
+
+
+def rebuilt(sx, O, G, T, mat, kind, eps):
+    """the same surfaces built directly at G's current vertex coordinates,
+    on the GPU and in the oracle"""
+    Gd = sx.RWGGeometry()
+    Go = O.Geometry()
+    for ns in range(G.NumSurfaces):
+        V = G.Vertices(ns)
+        Gd.AddMesh(V, T, mat)
+        Go.add_surface(V, T, kind=kind, eps=eps)
+    return Gd, Go
+
+
+@pytest.mark.parametrize("mat,kind,eps,omega", [("PEC", 0, 1.0, 1.0), ("CONST_EPS_4+0.1i", 1, 4 + 0.1j, 0.8)])
+def test_transformed_assembly(sx, O, tmp_path, mat, kind, eps, omega):
+    print(f"transform {mat} omega={omega}", flush=True)
+    G, T = pair(sx, O, 1, mat, kind, eps)
+    M0 = G.AssembleBEMMatrix(omega).numpy()
+    GT = sx.ReadTransFile(write(tmp_path, "TRANS t OBJECT A ROT 40 ABOUT 1 1 0 DISP 0.2 0 0.7 OBJECT B DISP 0 0.3 0\n"))
+    G.Transform(GT[0])
+    # records cached at the old positions would be reused where their
+    # float32 keys recur (the reference's process-wide cache); the oracle
+    # computes every record at the new positions
+    sx.ClearCache()
+    M1 = G.AssembleBEMMatrix(omega).numpy()
+    Gd, Go = rebuilt(sx, O, G, T, mat, kind, eps)
+    # Transform keeps each edge's length (RWGSurface.cc:588-627 moves
+    # vertices and centroids only); a rebuilt geometry recomputes it from
+    # the moved vertices, which differs in the last bit
+    assert rel_fro(M1, Gd.AssembleBEMMatrix(omega).numpy()) <= 1e-14
+    e = rel_fro(M1, Go.assemble(omega))
+    print(f"  vs oracle {e:.2e}", flush=True)
+    assert e <= 1e-12
+    assert rel_fro(M1, M0) > 1e-3            # it did move
+    G.UnTransform()
+    sx.ClearCache()          # else the moved positions' records serve the old keys
+    # the restored vertices differ from the original ones in the last bit,
+    # and the adaptive FIPPI cubature can take a different refinement path
+    # there; a stale device geometry would differ by O(1)
+    e = rel_fro(G.AssembleBEMMatrix(omega).numpy(), M0)
+    print(f"  restored {e:.2e}", flush=True)
+    assert e <= 1e-9
+
+
+def test_scatter_block_reuse(sx, O, tmp_path):
+    """T blocks from the untransformed positions, U blocks per entry, lower
+    blocks transposed: each entry's matrix is the full assembly there"""
+    from scuffx.scatter import TransformedBEMMatrices
+    print("scatter block reuse", flush=True)
+    G, T = pair(sx, O, 1, "CONST_EPS_4+0.1i", 1, 4 + 0.1j)
+    GT = sx.ReadTransFile(write(tmp_path, "TRANS d1 OBJECT B DISP 0 0 -0.4\nTRANS r OBJECT A ROT 25 ABOUT 0 1 0\n"
+                                          "TRANS d2 OBJECT A DISP 0.5 0 0.5 OBJECT B ROT 10 ABOUT 1 0 0\n"))
+    omega = 1.0
+    o, n = G.BFOffset(1)
+    seen = []
+    for gtc, M in TransformedBEMMatrices(G, omega, GT):
+        Mf = G.AssembleBEMMatrix(omega).numpy()
+        Mb = M.numpy()
+        # U blocks and their transposes: the same assembly
+        assert np.array_equal(Mb[:o, o:], Mf[:o, o:]) and np.array_equal(Mb[o:, :o], Mf[o:, :o])
+        # T blocks: assembled at the untransformed positions.  A moved
+        # surface's own block is the same integral in rotated / shifted
+        # coordinates; the adaptive FIPPI cubature (relative tolerance 1e-6
+        # per record) refines differently there, which shows at ~1e-9
+        e = max(rel_fro(Mb[:o, :o], Mf[:o, :o]), rel_fro(Mb[o:, o:], Mf[o:, o:]))
+        print(f"  {gtc.Tag}: T blocks {e:.2e}", flush=True)
+        assert e <= 1e-8
+        seen.append(gtc.Tag)
+    assert seen == ["d1", "r", "d2"]
+    # a single entry: one full assembly after the transformation
+    one = sx.ReadTransFile(write(tmp_path, "TRANS only OBJECT B DISP 0 0 -0.4\n", "one.trans"))
+    for gtc, M in TransformedBEMMatrices(G, omega, one):
+        assert np.array_equal(M.numpy(), G.AssembleBEMMatrix(omega).numpy())
+    assert G.SurfaceMoved == [False, True]
+
+
+def spheres_geo(tmp_path):
+    geo = tmp_path / "PECSpheres_327.scuffgeo"
+    geo.write_text(f"OBJECT UpperSphere\n\tMESHFILE {MESH}\n\tDISPLACED 0 0 3\nENDOBJECT\n\n"
+                   f"OBJECT LowerSphere\n\tMESHFILE {MESH}\nENDOBJECT\n")
+    return str(geo)
+
+
+def test_casimir_integrand_per_entry(sx, tmp_path):
+    """two entries in one workspace equal two separate geometries"""
+    from scuffx.cas3d import SC3Data
+    print("casimir integrand per entry", flush=True)
+    trans = write(tmp_path, "TRANS 3.0 SURFACE UpperSphere DISP 0 0 0.0\nTRANS 3.8 SURFACE UpperSphere DISP 0 0 0.8\n")
+    G = sx.RWGGeometry(spheres_geo(tmp_path))
+    S = SC3Data(G, Energy=True, ZForce=True, TransFile=trans)
+    assert S.NTNQ == 4
+    xi = 0.6
+    EFT = S.GetCasimirIntegrand(xi)
+    for nt, sep in enumerate((3.0, 3.8)):
+        geo = tmp_path / f"s{nt}.scuffgeo"
+        geo.write_text(f"OBJECT UpperSphere\n\tMESHFILE {MESH}\n\tDISPLACED 0 0 {sep}\nENDOBJECT\n\n"
+                       f"OBJECT LowerSphere\n\tMESHFILE {MESH}\nENDOBJECT\n")
+        ref = SC3Data(sx.RWGGeometry(str(geo)), Energy=True, ZForce=True).GetCasimirIntegrand(xi)
+        rel = np.abs(EFT[2 * nt:2 * nt + 2] - ref) / np.abs(ref)
+        print(f"  {sep}: {EFT[2 * nt:2 * nt + 2]} vs {ref}, relative {rel}", flush=True)
+        # the workspace's T blocks come from the untransformed positions and
+        # the separate geometry's from its own; they agree to the FIPPI
+        # cubature's refinement noise (~1e-9), which the energy's
+        # difference of log determinants amplifies.  Entries mixed up would
+        # differ by O(1).
+        assert np.all(rel <= 1e-6)
+    # a converged entry is skipped and reports zeros
+    EFT2 = S.GetCasimirIntegrand(xi, Converged=[True, True, False, False])
+    print(f"  skipped first entry: {EFT2}", flush=True)
+    assert EFT2[0] == 0 and EFT2[1] == 0
+    assert np.all(np.abs(EFT2[2:] - EFT[2:]) <= 1e-10 * np.abs(EFT[2:]))
+
+
+def test_casimir_zforce_reference_run(sx, tmp_path):
+    """the reference's PECSpheres run (PECSpheres_327.scuffgeo with
+    Spheres.trans), first five entries, against PECSpheres_327.out"""
+    from scuffx.cas3d import SC3Data
+    lines = open(os.path.join(GOLDEN, "Spheres.trans")).read().splitlines()[:5]
+    trans = write(tmp_path, "\n".join(lines) + "\n")
+    G = sx.RWGGeometry(spheres_geo(tmp_path))
+    S = SC3Data(G, ZForce=True, TransFile=trans)
+    print("casimir zforce over 5 entries", flush=True)
+    I, E = S.GetXiIntegral_Cliff()
+    rows = [ln.split() for ln in open(os.path.join(GOLDEN, "PECSpheres_327.out")) if ln.strip() and ln[0] != "#"]
+    ref = {r[0]: (float(r[1]), float(r[2])) for r in rows}
+    for nt, gtc in enumerate(S.GTCs):
+        r, re_ = ref[gtc.Tag]
+        print(f"  {gtc.Tag}: F={I[nt]:.6e} (+- {E[nt]:.1e}) reference {r:.6e} (+- {re_:.1e})", flush=True)
+        assert abs(I[nt] - r) <= E[nt] + 3.0 * re_
+    print(f"  evaluations {len(S.ByXi)}", flush=True)
