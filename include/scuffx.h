@@ -99,6 +99,23 @@ int scuffx_geom_transform(scuffx_geom *G, const scuffx_trans *T, int i);
 int scuffx_geom_untransform(scuffx_geom *G);
 /* SurfaceMoved[ns] of the last scuffx_geom_transform (RWGGeometry.cc:774-781) */
 int scuffx_geom_surface_moved(const scuffx_geom *G, int ns);
+
+/* ---- HDF5 files (libs/libhmat/HDF5IO.cc) ----
+   HDF5 is loaded at run time (libhdf5.so + libhdf5_hl.so from the loader path,
+   $SCUFFX_HDF5_DIR or /opt/conda/lib); without it these return
+   SCUFFX_ERR_IO.  Data are host arrays, column-major, complex interleaved.
+   HMatrix::OpenHDF5Context / CloseHDF5Context (HDF5IO.cc:120-160): */
+int scuffx_hdf5_open(const char *path, void **ctx);
+int scuffx_hdf5_close(void *ctx);
+/* HMatrix::ExportToHDF5 (HDF5IO.cc:171-236): dims {nc, nr} or {nc, 2 nr},
+   int attributes Storage_Type = 0 and RealComplex */
+int scuffx_hdf5_write_matrix(void *ctx, const char *name, const double *data, int nr, int nc, int is_complex);
+/* HVector::ExportToHDF5 (HDF5IO.cc:436-475): dims {n} or {2 n}, RealComplex */
+int scuffx_hdf5_write_vector(void *ctx, const char *name, const double *data, int n, int is_complex);
+/* HMatrix(FileName, LHM_HDF5, Name) / HVector(...) (HDF5IO.cc:252-418,
+   :497-540): call with data = NULL for the sizes, then with a buffer */
+int scuffx_hdf5_read_matrix(const char *path, const char *name, double *data, int *nr, int *nc, int *is_complex);
+int scuffx_hdf5_read_vector(const char *path, const char *name, double *data, int *n, int *is_complex);
 /* test-mesh generator: V[3*(10*4^L+2)], T[3*20*4^L] */
 int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T);
 

@@ -13,6 +13,7 @@
 #include "assemble.h"
 #include "fippi_cache.h"
 #include "geometry.h"
+#include "hdf5_io.h"
 #include "lu.h"
 #include "prof.h"
 #include "rhs.h"
@@ -261,6 +262,44 @@ int scuffx_geom_surface_moved(const scuffx_geom *G, int ns)
 {
   if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
   return ns < (int)G->G.SurfaceMoved.size() ? G->G.SurfaceMoved[ns] : 0;
+}
+
+static int io_status(const std::string &e) { return e.empty() ? 0 : fail(SCUFFX_ERR_IO, e); }
+
+int scuffx_hdf5_open(const char *path, void **ctx)
+{
+  if (!path || !ctx) return fail(SCUFFX_ERR_ARG, "null argument");
+  return io_status(hdf5_open(path, ctx));
+}
+
+int scuffx_hdf5_close(void *ctx)
+{
+  if (!ctx) return fail(SCUFFX_ERR_ARG, "null context");
+  return io_status(hdf5_close(ctx));
+}
+
+int scuffx_hdf5_write_matrix(void *ctx, const char *name, const double *data, int nr, int nc, int is_complex)
+{
+  if (!ctx || !name || !data || nr < 0 || nc < 0) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  return io_status(hdf5_write_matrix(ctx, name, data, nr, nc, is_complex ? 1 : 0));
+}
+
+int scuffx_hdf5_write_vector(void *ctx, const char *name, const double *data, int n, int is_complex)
+{
+  if (!ctx || !name || !data || n < 0) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  return io_status(hdf5_write_vector(ctx, name, data, n, is_complex ? 1 : 0));
+}
+
+int scuffx_hdf5_read_matrix(const char *path, const char *name, double *data, int *nr, int *nc, int *is_complex)
+{
+  if (!path || !name || !nr || !nc || !is_complex) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  return io_status(hdf5_read_matrix(path, name, data, nr, nc, is_complex));
+}
+
+int scuffx_hdf5_read_vector(const char *path, const char *name, double *data, int *n, int *is_complex)
+{
+  if (!path || !name || !n || !is_complex) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  return io_status(hdf5_read_vector(path, name, data, n, is_complex));
 }
 
 int scuffx_icosphere(int level, double radius, const double *center, double *V, int *T)

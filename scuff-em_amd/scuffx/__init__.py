@@ -70,6 +70,12 @@ def lib():
             "scuffx_geom_transform": ([vp, vp, C.c_int], C.c_int),
             "scuffx_geom_untransform": ([vp], C.c_int),
             "scuffx_geom_surface_moved": ([vp, C.c_int], C.c_int),
+            "scuffx_hdf5_open": ([C.c_char_p, vp], C.c_int),
+            "scuffx_hdf5_close": ([vp], C.c_int),
+            "scuffx_hdf5_write_matrix": ([vp, C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),
+            "scuffx_hdf5_write_vector": ([vp, C.c_char_p, vp, C.c_int, C.c_int], C.c_int),
+            "scuffx_hdf5_read_matrix": ([C.c_char_p, C.c_char_p, vp, ip, ip, ip], C.c_int),
+            "scuffx_hdf5_read_vector": ([C.c_char_p, C.c_char_p, vp, ip, ip], C.c_int),
             "scuffx_icosphere": ([C.c_int, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_fippi_build": ([vp], C.c_int),
             "scuffx_fippi_count": ([vp], C.c_int),
@@ -165,6 +171,17 @@ class HVector:
     def numpy(self):
         return self.t.cpu().numpy()
 
+    def ExportToHDF5(self, dest, Name):
+        """HVector::ExportToHDF5 (HDF5IO.cc:436-490)"""
+        ExportToHDF5(dest, Name, self.numpy())
+
+    @classmethod
+    def FromHDF5(cls, FileName, Name):
+        """HVector(FileName, LHM_HDF5, Name) (HDF5IO.cc:497-540)"""
+        torch = _torch()
+        v = ImportFromHDF5(FileName, Name, Vector=True)
+        return cls(len(v), data=torch.from_numpy(v.astype(np.complex128)).cuda())
+
     def GetEntry(self, n):
         return complex(self.t[n].item())
 
@@ -196,6 +213,15 @@ class HMatrix:
 
     def numpy(self):
         return self.t.cpu().numpy().T.copy()
+
+    def ExportToHDF5(self, dest, Name):
+        """HMatrix::ExportToHDF5 (HDF5IO.cc:171-250)"""
+        ExportToHDF5(dest, Name, self.numpy())
+
+    @classmethod
+    def FromHDF5(cls, FileName, Name):
+        """HMatrix(FileName, LHM_HDF5, Name) (HDF5IO.cc:252-418)"""
+        return cls.from_numpy(ImportFromHDF5(FileName, Name).astype(np.complex128))
 
     def GetEntry(self, r, c):
         return complex(self.t[c, r].item())
@@ -312,6 +338,61 @@ class PlaneWave:
     def __init__(self, E0, nHat):
         self.E0 = np.asarray(E0, dtype=np.complex128)
         self.nHat = np.asarray(nHat, dtype=np.float64)
+
+
+def OpenHDF5Context(FileName):
+    """creates (truncates) an HDF5 file for several datasets
+    (HMatrix::OpenHDF5Context, libs/libhmat/HDF5IO.cc:120-145)"""
+    ctx = C.c_void_p()
+    _check(lib().scuffx_hdf5_open(FileName.encode(), C.byref(ctx)))
+    return ctx
+
+
+def CloseHDF5Context(ctx):
+    _check(lib().scuffx_hdf5_close(ctx))
+
+
+def ExportToHDF5(dest, Name, A):
+    """writes a host array in the reference's layout: 2-D as an HMatrix
+    (column-major, HDF5IO.cc:171-236), 1-D as an HVector (:436-475); dest
+    is a context from OpenHDF5Context or a file name (a file of its own)"""
+    A = np.asarray(A)
+    cplx = np.iscomplexobj(A)
+    own = isinstance(dest, str)
+    ctx = OpenHDF5Context(dest) if own else dest
+    try:
+        if A.ndim == 2:
+            data = np.asfortranarray(A, dtype=np.complex128 if cplx else np.float64)
+            _check(lib().scuffx_hdf5_write_matrix(ctx, Name.encode(), data.ctypes.data_as(C.c_void_p),
+                                                  A.shape[0], A.shape[1], int(cplx)))
+        elif A.ndim == 1:
+            data = np.ascontiguousarray(A, dtype=np.complex128 if cplx else np.float64)
+            _check(lib().scuffx_hdf5_write_vector(ctx, Name.encode(), data.ctypes.data_as(C.c_void_p),
+                                                  A.shape[0], int(cplx)))
+        else:
+            raise SCUFFXError("only matrices and vectors are exported")
+    finally:
+        if own:
+            CloseHDF5Context(ctx)
+
+
+def ImportFromHDF5(FileName, Name, Vector=False):
+    """reads a dataset written in the reference's layout: a matrix (2-D
+    array) or, with Vector=True, a vector (HDF5IO.cc:252-418, :497-540)"""
+    c = C.c_int()
+    if Vector:
+        n = C.c_int()
+        _check(lib().scuffx_hdf5_read_vector(FileName.encode(), Name.encode(), None, C.byref(n), C.byref(c)))
+        out = np.zeros(n.value, dtype=np.complex128 if c.value else np.float64)
+        _check(lib().scuffx_hdf5_read_vector(FileName.encode(), Name.encode(), out.ctypes.data_as(C.c_void_p),
+                                             C.byref(n), C.byref(c)))
+        return out
+    nr, nc = C.c_int(), C.c_int()
+    _check(lib().scuffx_hdf5_read_matrix(FileName.encode(), Name.encode(), None, C.byref(nr), C.byref(nc), C.byref(c)))
+    out = np.zeros((nr.value, nc.value), dtype=np.complex128 if c.value else np.float64, order="F")
+    _check(lib().scuffx_hdf5_read_matrix(FileName.encode(), Name.encode(), out.ctypes.data_as(C.c_void_p),
+                                         C.byref(nr), C.byref(nc), C.byref(c)))
+    return out
 
 
 class GTComplex:
