@@ -928,6 +928,54 @@ __global__ void k_laswp_rev(T *B, long ldb, long n, long nrhs, const int *ipiv)
   }
 }
 
+// The row interchanges of ipiv (LAPACK order, each row s swapped with a
+// row p >= s) composed into one permutation: perm[s] is the row whose
+// content ends up in row s.  One wave walks the interchanges in order on
+// an LDS copy of the permutation; a ballot over 64 pivots at a time skips
+// the rows that stay (most of them for a BEM matrix).  The right-hand
+// sides are then gathered (forward) or scattered (reverse) in parallel,
+// instead of one thread per column walking all n interchanges in memory.
+#define PERM_LDS_MAX 20480
+__global__ void __launch_bounds__(64) k_perm_build(const int *ipiv, long n, int *perm)
+{
+  __shared__ int P[PERM_LDS_MAX], V[PERM_LDS_MAX];
+  const int lane = threadIdx.x;
+  for (long i = lane; i < n; i += 64) { P[i] = (int)i; V[i] = ipiv[i] - 1; }
+  __syncthreads();
+  for (long s0 = 0; s0 < n; s0 += 64) {
+    const long s = s0 + lane;
+    const int p = s < n ? V[s] : (int)s;
+    unsigned long long m = __ballot(s < n && p != s && p > s && p < n);   // interchanges stay below row n
+    while (m) {
+      const int b = __ffsll(m) - 1;
+      m &= m - 1;
+      const int pp = __shfl(p, b);
+      // one lane swaps; the wave's LDS operations complete in order
+      if (lane == 0) { const int a = P[s0 + b]; P[s0 + b] = P[pp]; P[pp] = a; }
+    }
+  }
+  __syncthreads();
+  for (long i = lane; i < n; i += 64) perm[i] = P[i];
+}
+
+// forward: out[s] = in[perm[s]]; reverse: out[perm[s]] = in[s]
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) k_perm_apply(const T *in, long ldi, T *out, long ldo, const int *perm, long n)
+{
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const long c = blockIdx.y;
+  const long p = perm[s];
+  if (FWD) out[s + c * ldo] = in[p + c * ldi];
+  else out[p + c * ldo] = in[s + c * ldi];
+}
+
+// applies the interchanges of ipiv to the nrhs columns of B (forward, as
+// laswp with increment 1) or their inverse (reverse order); tmp holds n x
+// nrhs elements
+template <typename T>
+static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s);
+
 // Triangular solves with few right-hand sides, in 64-row block steps.
 // The diagonal 64x64 blocks of L (unit lower) and U (upper) are inverted
 // first, all blocks at once (k_diag_inv), so every block step is a small
@@ -1038,6 +1086,23 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
 }
 
 template <typename T>
+static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s)
+{
+  int *perm = n <= PERM_LDS_MAX ? (int *)ws_get(WS_PERM, sizeof(int) * n) : nullptr;
+  if (!perm || !tmp) {
+    // beyond the LDS copy: the interchanges one after another, per column
+    if (fwd) laswp(B, ldb, 0, nrhs, 0, n, ipiv, s);
+    else hipLaunchKernelGGL(k_laswp_rev<T>, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, B, ldb, n, nrhs, ipiv);
+    return;
+  }
+  hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(64), 0, s, ipiv, n, perm);
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nrhs);
+  if (fwd) hipLaunchKernelGGL((k_perm_apply<T, true>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n);
+  else hipLaunchKernelGGL((k_perm_apply<T, false>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n);
+  hipMemcpy2DAsync(B, sizeof(T) * ldb, tmp, sizeof(T) * n, sizeof(T) * n, nrhs, hipMemcpyDeviceToDevice, s);
+}
+
+template <typename T>
 int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B, long ldb,
              hipStream_t stream)
 {
@@ -1051,7 +1116,7 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
     if (!W || !invL || !invU) return -2;
     hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL);
     hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU);
-    laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);
+    apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream);
     // L y = P b: y accumulates in W, B holds the running right-hand side
     for (long kb = 0; kb < n; kb += TB) {
       int bs = (int)std::min<long>(TB, n - kb);
@@ -1070,8 +1135,9 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
   }
+  T *tmp = (T *)ws_get(WS_PERM_TMP, sizeof(T) * n * nrhs);
   if (trans == 'N' || trans == 'n') {
-    laswp(B, ldb, 0, nrhs, 0, n, d_ipiv, stream);
+    apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, tmp, stream);
     const int bs = 256;
     for (long kb = 0; kb < n; kb += bs) {
       int w = (int)std::min<long>(bs, n - kb);
@@ -1087,7 +1153,7 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
   } else if (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c') {
     int cj = (trans == 'C' || trans == 'c');
     hipLaunchKernelGGL(k_trsv_trans<T>, dim3((unsigned)nrhs), dim3(256), 0, stream, LU, lda, n, B, ldb, 1, cj);
-    hipLaunchKernelGGL(k_laswp_rev<T>, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, stream, B, ldb, n, nrhs, d_ipiv);
+    apply_row_perm(B, ldb, n, nrhs, d_ipiv, false, tmp, stream);
   } else return -1;
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

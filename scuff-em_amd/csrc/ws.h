@@ -15,6 +15,8 @@ enum WsSlot {
   WS_SOLVE_INVL,   // triangular solve: inverses of the diagonal blocks of L
   WS_SOLVE_INVU,   // ... and of U
   WS_REFINE,       // refinement: residual, partial sums, complex64 residual, scalar
+  WS_PERM,         // triangular solve: composed row permutation of ipiv
+  WS_PERM_TMP,     // ... and the permuted right-hand sides (many-column solves)
   WS_COUNT
 };
 
