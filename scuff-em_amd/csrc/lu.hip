@@ -1085,6 +1085,150 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
   }
 }
 
+// One right-hand side, all block steps in one cooperative launch.
+// Workgroup g owns the 64 rows of block g: it subtracts the off-diagonal
+// blocks' products with every earlier solution block as that block is
+// published (forward: blocks k < g of L; backward: k > g of U), then
+// applies the inverse diagonal block and publishes its own 64 values
+// followed by a flag.  Values and flags are agent-scope relaxed atomics,
+// the flag stored after the values' stores have completed; every wait is
+// bounded (error word, as in the panel).  Thread t handles row t % 64 and
+// the 16 columns 16 (t / 64) .. of a block; the four partial sums meet in
+// LDS.  The next block's 16 values per thread are loaded before the wait.
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256)
+k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout, unsigned long long *flags,
+             unsigned *err, unsigned long long epoch)
+{
+  __shared__ T part[4][TB];
+  __shared__ T bx[TB];
+  const int t = threadIdx.x, r = t & (TB - 1), q = t >> 6;
+  const long nblk = (n + TB - 1) / TB;
+  const long g = blockIdx.x;
+  const long row = g * TB + r;
+  const int bs = (int)std::min<long>(TB, n - g * TB);
+  T b;
+  b.x = 0; b.y = 0;
+  if (q == 0 && r < bs) b = bin[row];
+  // blocks k in solve order before g
+  const long nk = LOWER ? g : nblk - 1 - g;
+  auto blk = [&](long i) { return LOWER ? i : nblk - 1 - i; };
+  auto load = [&](long k, T (&a)[16]) {
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const long col = k * TB + q * 16 + u;
+      if (r < bs && col < n) a[u] = LU[row + col * lda];
+      else { a[u].x = 0; a[u].y = 0; }
+    }
+  };
+  // the inverse diagonal block, in the same registers once no block is left
+  const T *ib = inv + g * TB * TB;
+  auto load_inv = [&](T (&m)[16]) {
+#pragma unroll
+    for (int u = 0; u < 16; u++) m[u] = ib[r + (long)(q * 16 + u) * TB];
+  };
+  T a[16];
+  if (nk > 0) load(blk(0), a);
+  else load_inv(a);
+  for (long i = 0; i < nk; i++) {
+    const long k = blk(i);
+    // wait for block k's values (wave 0 polls; bounded)
+    if (t < 64) {
+      unsigned spins = 0;
+      for (;;) {
+        const unsigned long long f = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__ballot(f == epoch) != 0) break;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (++spins > GBAR_SPIN) { if (t == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    T x[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const long col = k * TB + q * 16 + u;
+      if (col < n) x[u] = ld_coh(xout + col);
+      else { x[u].x = 0; x[u].y = 0; }
+    }
+    T acc;
+    acc.x = 0; acc.y = 0;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      acc.x += a[u].x * x[u].x - a[u].y * x[u].y;
+      acc.y += a[u].x * x[u].y + a[u].y * x[u].x;
+    }
+    if (i + 1 < nk) load(blk(i + 1), a);
+    else load_inv(a);
+    part[q][r] = acc;
+    __syncthreads();
+    if (q == 0) {
+      T s4 = part[0][r];
+      s4 = cadd(s4, part[1][r]); s4 = cadd(s4, part[2][r]); s4 = cadd(s4, part[3][r]);
+      b = csub(b, s4);
+    }
+    __syncthreads();
+  }
+  // own block: x_g = inv(D_g) b_g
+  if (q == 0) bx[r] = b;
+  __syncthreads();
+  T acc;
+  acc.x = 0; acc.y = 0;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const T m = a[u], v = bx[q * 16 + u];
+    acc.x += m.x * v.x - m.y * v.y;
+    acc.y += m.x * v.y + m.y * v.x;
+  }
+  part[q][r] = acc;
+  __syncthreads();
+  if (q == 0 && r < bs) {
+    T xg = part[0][r];
+    xg = cadd(xg, part[1][r]); xg = cadd(xg, part[2][r]); xg = cadd(xg, part[3][r]);
+    st_coh(xout + row, xg);
+  }
+  drain();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(flags + g, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// lower (forward) then upper (backward) solve of one right-hand side in
+// two cooperative launches; returns -1 if it cannot run (grid not
+// co-resident, workspace), -6 on a time-out
+template <typename T>
+static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *invU, T *B, T *W, hipStream_t s)
+{
+  const long nblk = (n + TB - 1) / TB;
+  static int ncu = -1, perCU = 0;
+  if (ncu < 0) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_trsv_chain<T, true>, 256, 0);
+  }
+  if ((long)ncu * perCU < nblk) return -1;
+  char *w = (char *)ws_get(WS_CHAIN, sizeof(unsigned long long) * (2 * nblk + 8));
+  if (!w) return -1;
+  unsigned *err = (unsigned *)w;
+  unsigned long long *fl = (unsigned long long *)(w + 64), *fu = fl + nblk;
+  hipMemsetAsync(w, 0, sizeof(unsigned long long) * (2 * nblk + 8), s);
+  unsigned long long epoch = 1;
+  void *al[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invL, (void *)&B, (void *)&W, (void *)&fl,
+                (void *)&err, (void *)&epoch};
+  if (hipLaunchCooperativeKernel((const void *)k_trsv_chain<T, true>, dim3((unsigned)nblk), dim3(256), al, 0, s) !=
+      hipSuccess)
+    return -3;
+  void *au[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invU, (void *)&W, (void *)&B, (void *)&fu,
+                (void *)&err, (void *)&epoch};
+  if (hipLaunchCooperativeKernel((const void *)k_trsv_chain<T, false>, dim3((unsigned)nblk), dim3(256), au, 0, s) !=
+      hipSuccess)
+    return -3;
+  unsigned tmo = 0;
+  hipMemcpyAsync(&tmo, err, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return tmo ? -6 : 0;
+}
+
 template <typename T>
 static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s)
 {
@@ -1117,6 +1261,10 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
     hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL);
     hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU);
     apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream);
+    if (nrhs == 1) {
+      int rc = trsv_chain(LU, lda, n, invL, invU, B, W, stream);
+      if (rc != -1) return rc;
+    }
     // L y = P b: y accumulates in W, B holds the running right-hand side
     for (long kb = 0; kb < n; kb += TB) {
       int bs = (int)std::min<long>(TB, n - kb);

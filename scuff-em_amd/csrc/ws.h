@@ -17,6 +17,7 @@ enum WsSlot {
   WS_REFINE,       // refinement: residual, partial sums, complex64 residual, scalar
   WS_PERM,         // triangular solve: composed row permutation of ipiv
   WS_PERM_TMP,     // ... and the permuted right-hand sides (many-column solves)
+  WS_CHAIN,        // one-launch triangular solves: error word and block flags
   WS_COUNT
 };
 
