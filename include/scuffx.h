@@ -95,10 +95,39 @@ int scuffx_geom_check_trans(const scuffx_geom *G, const scuffx_trans *T);
    The next assembly rebuilds the device geometry and near-pair lists; FIPPI
    records come from the process-wide cache where their keys recur. */
 int scuffx_geom_transform(scuffx_geom *G, const scuffx_trans *T, int i);
+/* the Transform of one entry naming surface ns, given as X' = R X + D
+   (R row-major 3x3, NULL = identity; D NULL = 0); UnTransform reverts it */
+int scuffx_geom_transform_surface(scuffx_geom *G, int ns, const double *R9, const double *D3);
 /* RWGGeometry::UnTransform (RWGGeometry.cc:917-921) */
 int scuffx_geom_untransform(scuffx_geom *G);
 /* SurfaceMoved[ns] of the last scuffx_geom_transform (RWGGeometry.cc:774-781) */
 int scuffx_geom_surface_moved(const scuffx_geom *G, int ns);
+
+/* ---- adjoint contraction ----
+   For a rigid motion of surface ns (displacement along x, y, z, then
+   rotations about up to three axes through the origin given by their
+   generators gamma[9 t .. 9 t + 8], as in scuffx_assemble_block):
+       out[m] = sum_ij conj(x_i) (dM/dtheta_m)_ij y_j      (complex, re/im)
+   over the whole matrix, i.e. the blocks (ns, b) and (b, ns), b != ns, of
+   the GradM / dMdT path (AssembleBEMMatrixBlock, libscuff.h:500-505).
+   With x the adjoint solution, -Re(out) is the gradient of a real function
+   of the solution y of M y = b.  The derivative blocks are assembled one
+   surface pair at a time and contracted on the GPU; the full dM is never
+   stored.  x, y: device vectors of TotalBFs complex128. */
+int scuffx_adjoint(scuffx_geom *G, int ns, double wr, double wi, const void *x, const void *y, int ntorque,
+                   const double *gamma, double *out);
+
+/* ---- frequency sweep ----
+   The scattering caller's loop over frequencies on one geometry
+   (scuff-scatter.cc:373-567, no transformations): per frequency the BEM
+   matrix, its factorization (prec 64: complex128 LU; prec 32: complex64
+   LU + complex128 refinement to 1e-13), the plane-wave right-hand side
+   (E0: 3 complex as 6 doubles, nhat: 3), the solve and the power closure.
+   omegas: count complex frequencies as re, im pairs; out[3k .. 3k+2] =
+   PAbs, PExt, PScat of frequency k.  One stream, buffers allocated once
+   per call; the FIPPI table is built at the first frequency. */
+int scuffx_batch_sweep(scuffx_geom *G, const double *omegas, int count, const double *E0, const double *nhat,
+                       int prec, double *out);
 
 /* ---- HDF5 files (libs/libhmat/HDF5IO.cc) ----
    HDF5 is loaded at run time (libhdf5.so + libhdf5_hl.so from the loader path,

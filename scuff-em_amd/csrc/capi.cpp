@@ -14,6 +14,7 @@
 #include "fippi_cache.h"
 #include "geometry.h"
 #include "hdf5_io.h"
+#include "adjoint.h"
 #include "lu.h"
 #include "prof.h"
 #include "rhs.h"
@@ -245,6 +246,21 @@ int scuffx_geom_transform(scuffx_geom *G, const scuffx_trans *T, int i)
   transform_geometry(&G->G, T->C[i]);
   for (int m : G->G.SurfaceMoved)
     if (m) { G->moved(); G->transformed = true; break; }
+  return 0;
+}
+
+int scuffx_geom_transform_surface(scuffx_geom *G, int ns, const double *R9, const double *D3)
+{
+  if (!G || ns < 0 || ns >= (int)G->G.Surfaces.size()) return fail(SCUFFX_ERR_ARG, "bad surface index");
+  GTransform T;
+  for (int i = 0; i < 3; i++) {
+    if (R9) for (int j = 0; j < 3; j++) T.M[i][j] = R9[3 * i + j];
+    if (D3) T.DX[i] = D3[i];
+  }
+  G->G.SurfaceMoved.assign(G->G.Surfaces.size(), 0);
+  transform_surface(&G->G, ns, T);
+  G->moved();
+  G->transformed = true;
   return 0;
 }
 
@@ -585,6 +601,99 @@ int scuffx_power(scuffx_geom *G, double wr, double wi, const void *KN, const voi
   if (PExt) *PExt = tot[0];
   if (PAbs) *PAbs = tot[1];
   return 0;
+}
+
+int scuffx_adjoint(scuffx_geom *G, int ns, double wr, double wi, const void *x, const void *y, int ntorque,
+                   const double *gamma, double *out)
+{
+  if (!G || !x || !y || !out || ns < 0 || ns >= (int)G->G.Surfaces.size() || ntorque < 0 || ntorque > 3 ||
+      (ntorque > 0 && !gamma))
+    return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
+  const int NS = (int)G->G.Surfaces.size(), nq = 3 + ntorque, nparts = 64;
+  const int na = G->G.Surfaces[ns].NumBFs, oa = G->G.BFIndexOffset[ns];
+  int nbmax = 0;
+  for (int b = 0; b < NS; b++)
+    if (b != ns) nbmax = std::max(nbmax, G->G.Surfaces[b].NumBFs);
+  for (int m = 0; m < 2 * nq; m++) out[m] = 0;
+  if (nbmax == 0) return 0;
+  Assembler *A = G->assembler();
+  const size_t blk = (size_t)na * nbmax;
+  double2 *B = nullptr, *GB = nullptr, *TB = nullptr, *part = nullptr;
+  int rc = 0;
+  if (hipMalloc(&B, sizeof(double2) * blk) || hipMalloc(&GB, sizeof(double2) * 3 * blk) ||
+      (ntorque && hipMalloc(&TB, sizeof(double2) * ntorque * blk)) ||
+      hipMalloc(&part, sizeof(double2) * nparts * nq * NS))
+    rc = fail(SCUFFX_ERR_ALLOC, "allocation failed");
+  const double2 *X = (const double2 *)x, *Y = (const double2 *)y;
+  for (int b = 0; b < NS && rc == 0; b++) {
+    if (b == ns) continue;
+    const int nb = G->G.Surfaces[b].NumBFs, ob = G->G.BFIndexOffset[b];
+    const long st = (long)na * nb;
+    int r = A->assemble_block(ns, b, cdouble(wr, wi), B, na, GB, na, st, ntorque, gamma, TB, g_stream);
+    if (r) { rc = fail(r, A->error()); break; }
+    for (int m = 0; m < nq; m++) {
+      const double2 *Gm = m < 3 ? GB + m * st : TB + (m - 3) * st;
+      if (launch_adj_contract(g_stream, Gm, na, na, nb, X + oa, Y + oa, X + ob, Y + ob,
+                              part + ((long)b * nq + m) * nparts, nparts) != hipSuccess)
+        rc = fail(SCUFFX_ERR_HIP, "contraction kernel failed");
+    }
+  }
+  if (rc == 0) {
+    std::vector<double2> h((size_t)nparts * nq * NS);
+    hipMemcpyAsync(h.data(), part, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, g_stream);
+    if (hipStreamSynchronize(g_stream) != hipSuccess) rc = fail(SCUFFX_ERR_HIP, "contraction failed");
+    for (int b = 0; b < NS && rc == 0; b++) {
+      if (b == ns) continue;
+      for (int m = 0; m < nq; m++)
+        for (int k = 0; k < nparts; k++) {
+          out[2 * m] += h[((size_t)b * nq + m) * nparts + k].x;
+          out[2 * m + 1] += h[((size_t)b * nq + m) * nparts + k].y;
+        }
+    }
+  }
+  hipFree(B); hipFree(GB); hipFree(TB); hipFree(part);
+  return rc;
+}
+
+int scuffx_batch_sweep(scuffx_geom *G, const double *omegas, int count, const double *E0, const double *nhat,
+                       int prec, double *out)
+{
+  if (!G || !omegas || count < 0 || !E0 || !nhat || !out || G->G.Surfaces.empty())
+    return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (prec != 64 && prec != 32) return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  const long n = G->G.TotalBFs;
+  double2 *M = nullptr, *rhs = nullptr, *kn = nullptr;
+  float2 *lu32 = nullptr;
+  int *ipiv = nullptr;
+  int rc = 0;
+  if (hipMalloc(&M, sizeof(double2) * n * n) || hipMalloc(&rhs, sizeof(double2) * n) ||
+      hipMalloc(&kn, sizeof(double2) * n) || hipMalloc(&ipiv, sizeof(int) * n) ||
+      (prec == 32 && hipMalloc(&lu32, sizeof(float2) * n * n)))
+    rc = fail(SCUFFX_ERR_ALLOC, "allocation failed");
+  for (int k = 0; k < count && rc == 0; k++) {
+    const double wr = omegas[2 * k], wi = omegas[2 * k + 1];
+    if ((rc = scuffx_assemble(G, wr, wi, M, n)) != 0) break;
+    if ((rc = scuffx_rhs_planewave(G, wr, wi, E0, nhat, rhs)) != 0) break;
+    if (prec == 64) {
+      hipMemcpyAsync(kn, rhs, sizeof(double2) * n, hipMemcpyDeviceToDevice, g_stream);
+      if ((rc = scuffx_lu_factor(M, n, n, ipiv, 64)) != 0) break;
+      if ((rc = scuffx_lu_solve(M, n, n, ipiv, 'N', 1, kn, n, 64)) != 0) break;
+    } else {
+      if ((rc = scuffx_lu32_from_c128(M, n, n, lu32, n, ipiv)) != 0) break;
+      int it = 0;
+      double rel = 0;
+      if ((rc = scuffx_refine(M, n, n, lu32, n, ipiv, rhs, kn, 1e-13, 30, &it, &rel)) != 0) break;
+    }
+    double pa = 0, pe = 0;
+    if ((rc = scuffx_power(G, wr, wi, kn, rhs, &pa, &pe)) != 0) break;
+    out[3 * k] = pa;
+    out[3 * k + 1] = pe;
+    out[3 * k + 2] = pe - pa;
+  }
+  hipStreamSynchronize(g_stream);
+  hipFree(M); hipFree(rhs); hipFree(kn); hipFree(ipiv); hipFree(lu32);
+  return rc;
 }
 
 int scuffx_profile(int on)

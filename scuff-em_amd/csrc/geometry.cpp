@@ -476,19 +476,24 @@ static void reinit_panels(Surface *S)
   for (Panel &P : S->Panels) init_panel(&P, S->V.data());
 }
 
+void transform_surface(Geometry *G, int ns, const GTransform &D)
+{
+  Surface &S = G->Surfaces[ns];
+  D.apply(S.V.data(), S.V.size() / 3);
+  for (Edge &E : S.Edges) D.apply(E.Centroid, 1);
+  reinit_panels(&S);
+  S.GT.compose(D);
+  if ((int)G->SurfaceMoved.size() != (int)G->Surfaces.size()) G->SurfaceMoved.assign(G->Surfaces.size(), 0);
+  G->SurfaceMoved[ns] = 1;
+}
+
 void transform_geometry(Geometry *G, const GTComplex &C)
 {
   G->SurfaceMoved.assign(G->Surfaces.size(), 0);
   for (size_t k = 0; k < C.GTs.size(); k++) {
     const int ns = C.Labels[k].empty() ? -1 : surface_by_label(*G, C.Labels[k]);
     if (ns < 0) continue;   // the reference warns and ignores it
-    Surface &S = G->Surfaces[ns];
-    const GTransform &D = C.GTs[k];
-    D.apply(S.V.data(), S.V.size() / 3);
-    for (Edge &E : S.Edges) D.apply(E.Centroid, 1);
-    reinit_panels(&S);
-    S.GT.compose(D);
-    G->SurfaceMoved[ns] = 1;
+    transform_surface(G, ns, C.GTs[k]);
   }
 }
 

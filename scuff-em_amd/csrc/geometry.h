@@ -104,6 +104,8 @@ std::string check_gtc(const Geometry &G, const GTComplex &C);
 // panels re-initialized from the vertices; UnTransform applies the
 // inverse of the accumulated transformation to the current coordinates
 void transform_geometry(Geometry *G, const GTComplex &C);
+// one surface, by index (the Transform of a single labelled entry)
+void transform_surface(Geometry *G, int ns, const GTransform &D);
 void untransform_geometry(Geometry *G);
 
 // Builds a surface from vertex coordinates and triangle vertex indices.

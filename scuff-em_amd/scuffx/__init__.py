@@ -70,6 +70,9 @@ def lib():
             "scuffx_geom_transform": ([vp, vp, C.c_int], C.c_int),
             "scuffx_geom_untransform": ([vp], C.c_int),
             "scuffx_geom_surface_moved": ([vp, C.c_int], C.c_int),
+            "scuffx_batch_sweep": ([vp, vp, C.c_int, vp, vp, C.c_int, vp], C.c_int),
+            "scuffx_adjoint": ([vp, C.c_int, C.c_double, C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
+            "scuffx_geom_transform_surface": ([vp, C.c_int, vp, vp], C.c_int),
             "scuffx_hdf5_open": ([C.c_char_p, vp], C.c_int),
             "scuffx_hdf5_close": ([vp], C.c_int),
             "scuffx_hdf5_write_matrix": ([vp, C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -527,6 +530,34 @@ class RWGGeometry:
         _sync_stream_if_gpu()
         _check(lib().scuffx_geom_untransform(self.h))
 
+    def TransformSurface(self, ns, R=None, D=None):
+        """moves surface ns by X' = R X + D (R 3x3, default identity; D
+        3-vector, default 0) as one Transform entry naming it; UnTransform
+        reverts it"""
+        Rm = None if R is None else np.ascontiguousarray(R, dtype=np.float64).reshape(9)
+        Dv = None if D is None else np.ascontiguousarray(D, dtype=np.float64).reshape(3)
+        _sync_stream_if_gpu()
+        _check(lib().scuffx_geom_transform_surface(self.h, ns,
+                                                   None if Rm is None else Rm.ctypes.data_as(C.c_void_p),
+                                                   None if Dv is None else Dv.ctypes.data_as(C.c_void_p)))
+
+    def AdjointContraction(self, ns, Omega, X, Y, TorqueAxes=None):
+        """sum_ij conj(X_i) dM_ij/dtheta_m Y_j for a rigid displacement of
+        surface ns along x, y, z and rotations about TorqueAxes (through the
+        origin): complex array of 3 + len(TorqueAxes) values.  X, Y:
+        HVectors of TotalBFs (the GradM / dMdT path, libscuff.h:500-505,
+        contracted without storing dM)."""
+        nt = 0 if TorqueAxes is None else len(TorqueAxes)
+        gam = np.zeros(9 * max(nt, 1))
+        for t in range(nt):
+            gam[9 * t:9 * t + 9] = CreateGammaMatrix(TorqueAxes[t]).ravel(order="F")
+        out = np.zeros(2 * (3 + nt))
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_adjoint(self.h, ns, Omega.real, Omega.imag, _ptr(X.t), _ptr(Y.t), nt,
+                                    gam.ctypes.data_as(C.c_void_p) if nt else None, out.ctypes.data_as(C.c_void_p)))
+        return out[0::2] + 1j * out[1::2]
+
     @property
     def SurfaceMoved(self):
         return [bool(_check(lib().scuffx_geom_surface_moved(self.h, ns))) for ns in range(self.NumSurfaces)]
@@ -639,6 +670,22 @@ class RWGGeometry:
         _sync_stream()
         _check(lib().scuffx_power(self.h, Omega.real, Omega.imag, _ptr(KN.t), _ptr(RHS.t), C.byref(pa), C.byref(pe)))
         return {"PAbs": pa.value, "PExt": pe.value, "PScat": pe.value - pa.value}
+
+
+def BatchSweep(G, Omegas, IF, prec=64):
+    """power at every frequency of Omegas for the plane wave IF (the
+    scattering caller's frequency loop, scuff-scatter.cc:373-567):
+    array (len(Omegas), 3) of PAbs, PExt, PScat.  prec=32 factorizes in
+    complex64 and refines in complex128."""
+    om = np.asarray([complex(w) for w in Omegas], dtype=np.complex128)
+    E0 = np.ascontiguousarray(np.asarray(IF.E0, dtype=np.complex128)).view(np.float64)
+    nh = np.ascontiguousarray(IF.nHat, dtype=np.float64)
+    out = np.zeros((len(om), 3))
+    _sync_stream()
+    _check(lib().scuffx_batch_sweep(G.h, om.view(np.float64).ctypes.data_as(C.c_void_p), len(om),
+                                    E0.ctypes.data_as(C.c_void_p), nh.ctypes.data_as(C.c_void_p), prec,
+                                    out.ctypes.data_as(C.c_void_p)))
+    return out
 
 
 PROFILED_KERNELS = ("gemm_c128", "gemm_c64", "panel_c128", "panel_c64", "trsm_c128", "trsm_c64",
