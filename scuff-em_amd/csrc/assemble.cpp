@@ -517,7 +517,8 @@ int Assembler::run_sw(int sa, int sb, int bi, const RegionK *rk, int nreg, const
     cap_swjobs_ = jobs.size();
   }
   HIPCHK(hipMemcpy(d_swjobs_, jobs.data(), sizeof(SWJob) * jobs.size(), hipMemcpyHostToDevice));
-  const int chunk = 2048;
+  // one wave per job; 4096 jobs (16 waves per CU) hold 415 MB of value caches
+  const int chunk = 4096;
   if (ensure(&d_scratch_, &cap_scratch_, (size_t)chunk * SCUFFX_SW_SLAB)) return -2;
   HIPCHK(launch_sw(stream, d_swjobs_, (int)jobs.size(), B.d_pairs, ds_[sa].V, ds_[sa].PVI, nreg, dreg, needH1,
                    d_HN_, d_HS_, d_scratch_, chunk));

@@ -77,8 +77,8 @@ hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n);
 struct SWJob {
   int pair, region, q, kind;
 };
-// doubles of scratch per Taylor-Duffy job
-#define SCUFFX_SW_SLAB (3 * 6 * 5 * 22 + 4 * 48 + 2112 * 6)
+// doubles of scratch per Taylor-Duffy job (its cached integrand values)
+#define SCUFFX_SW_SLAB (2112 * 6)
 hipError_t launch_sw(hipStream_t stream, const SWJob *jobs, int njobs, const NearPair *nps, const double *V,
                      const int *PVI, int nreg, const RegionK *regs, int needH1, double *HN, double *HS,
                      double *scratch, int chunk);

@@ -14,8 +14,7 @@
 // :940-1262, :1265-1528; its p-adaptive Clenshaw-Curtis cubature (up to
 // three dimensions, relative tolerance 1e-10, at most 1000 new points)
 // follows libs/libSGJC/pcubature.c:60-430, visiting cached values in the
-// same order.  One thread per (pair, region, Qa, Qb) job; a job's value
-// cache and Upsilon tables live in a slab of scratch memory.
+// same order.  One wave per (pair, region, Qa, Qb) job.
 #include <hip/hip_runtime.h>
 
 #include "bem.h"
@@ -39,8 +38,8 @@ namespace scuffx {
 #define PC_MAXVAL 2112                   // cached points per job (bound: 27 + 1000 + 1027)
 #define PC_MAXENT 48                     // cache entries per job
 #define SW_FDIM (2 * TD_NPK)
-// scratch slab of one job, in doubles
-#define SW_SLAB (TD_NPK * TD_NREG * TD_NWP * TD_NMON + 4 * PC_MAXENT + PC_MAXVAL * SW_FDIM)
+// scratch slab of one job (its cached integrand values), in doubles
+#define SW_SLAB (PC_MAXVAL * SW_FDIM)
 
 enum { T1 = 0, TY1 = 1, TY12 = 2, TY2 = 3, TY1Y2 = 4, TY22 = 5, TY13 = 6,
        TY12Y2 = 7, TY1Y22 = 8, TY23 = 9, TY12Y22 = 10, TY13Y2 = 11, TY13Y22 = 12,
@@ -449,10 +448,13 @@ __device__ void td_sum(const TDGeo &W, const double (*U)[TD_NREG][TD_NWP][TD_NMO
 }
 
 // ComputeGeometricParameters (TaylorDuffy.cc:1265-1528) for P = UNITY,
-// PMCHWG1, PMCHWC (the first NumPKs of them); Upsilon tables into U
+// PMCHWG1, PMCHWC (the first NumPKs of them): the geometric scalars into W
+// (every lane), and, on lane npk < NumPKs, the Upsilon table of polynomial
+// npk into U[npk] and its lowest and highest nonzero power into nm[npk],
+// nm[TD_NPK + npk]
 __device__ void td_geometry(int WhichCase, const double *V1, const double *V2, const double *V3, const double *V2P,
                             const double *V3P, const double *Q, const double *QP, TDGeo &W,
-                            double (*U)[TD_NREG][TD_NWP][TD_NMON])
+                            double (*U)[TD_NREG][TD_NWP][TD_NMON], int *nm)
 {
   if (WhichCase == TD_CT) { V2P = V2; V3P = V3; }
   else if (WhichCase == TD_CE) V2P = V2;
@@ -468,41 +470,44 @@ __device__ void td_geometry(int WhichCase, const double *V1, const double *V2, c
     W.AP2 = dot3(AP, AP); W.BP2 = dot3(BP, BP); W.AdAP = dot3(A, AP); W.AdBP = dot3(A, BP);
     W.BdAP = dot3(B, AP); W.BdBP = dot3(B, BP); W.APdBP = dot3(AP, BP);
   }
+  const int npk = threadIdx.x;
+  if (npk >= W.NumPKs) return;
   const int NumRegions = (WhichCase == TD_CV) ? 2 : (WhichCase == TD_CE ? 6 : 3);
   double D[3], DP[3], DeltaD[3], Scr[3], Scr2[3];
   sub3(V1, Q, D); sub3(V1, QP, DP); sub3(QP, Q, DeltaD);
-  for (int npk = 0; npk < W.NumPKs; npk++) {
-    double C[4] = {0, 0, 0, 0}, M[16], V[4] = {0, 0, 0, 0}, S = 0.0;
-    for (int q = 0; q < 16; q++) M[q] = 0.0;
-    if (npk == 0) S = 1;
-    else if (npk == 1) {          // PMCHWG1
-      M[0 + 2 * 4] = dot3(A, AP); M[0 + 3 * 4] = dot3(A, BP);
-      M[1 + 2 * 4] = dot3(B, AP); M[1 + 3 * 4] = dot3(B, BP);
-      V[0] = dot3(A, DP); V[1] = dot3(B, DP); V[2] = dot3(D, AP); V[3] = dot3(D, BP);
-      S = dot3(D, DP);
-    } else {                      // PMCHWC
-      cross3(A, AP, Scr); M[0 + 2 * 4] = -dot3(Scr, DeltaD);
-      cross3(A, BP, Scr); M[0 + 3 * 4] = -dot3(Scr, DeltaD);
-      cross3(B, AP, Scr); M[1 + 2 * 4] = -dot3(Scr, DeltaD);
-      cross3(B, BP, Scr); M[1 + 3 * 4] = -dot3(Scr, DeltaD);
-      cross3(DP, D, Scr2);
-      V[0] = -dot3(A, Scr2); V[1] = -dot3(B, Scr2); V[2] = dot3(AP, Scr2); V[3] = dot3(BP, Scr2);
-    }
-    td_upsilon(WhichCase, C, M, V, S, U[npk]);
-    int nMin = TD_NWP - 1, nMax = -1;
-    for (int n = 0; n < TD_NWP; n++)
-      for (int nr = 0; nr < NumRegions; nr++)
-        for (int nm = 0; nm < TD_NMON; nm++)
-          if (fabs(U[npk][nr][n][nm]) > 1.0e-8) { if (n < nMin) nMin = n; if (n > nMax) nMax = n; }
-    W.nMin[npk] = nMin; W.nMax[npk] = nMax;
+  double C[4] = {0, 0, 0, 0}, M[16], V[4] = {0, 0, 0, 0}, S = 0.0;
+  for (int q = 0; q < 16; q++) M[q] = 0.0;
+  if (npk == 0) S = 1;
+  else if (npk == 1) {          // PMCHWG1
+    M[0 + 2 * 4] = dot3(A, AP); M[0 + 3 * 4] = dot3(A, BP);
+    M[1 + 2 * 4] = dot3(B, AP); M[1 + 3 * 4] = dot3(B, BP);
+    V[0] = dot3(A, DP); V[1] = dot3(B, DP); V[2] = dot3(D, AP); V[3] = dot3(D, BP);
+    S = dot3(D, DP);
+  } else {                      // PMCHWC
+    cross3(A, AP, Scr); M[0 + 2 * 4] = -dot3(Scr, DeltaD);
+    cross3(A, BP, Scr); M[0 + 3 * 4] = -dot3(Scr, DeltaD);
+    cross3(B, AP, Scr); M[1 + 2 * 4] = -dot3(Scr, DeltaD);
+    cross3(B, BP, Scr); M[1 + 3 * 4] = -dot3(Scr, DeltaD);
+    cross3(DP, D, Scr2);
+    V[0] = -dot3(A, Scr2); V[1] = -dot3(B, Scr2); V[2] = dot3(AP, Scr2); V[3] = dot3(BP, Scr2);
   }
+  td_upsilon(WhichCase, C, M, V, S, U[npk]);
+  int nMin = TD_NWP - 1, nMax = -1;
+  for (int n = 0; n < TD_NWP; n++)
+    for (int nr = 0; nr < NumRegions; nr++)
+      for (int q = 0; q < TD_NMON; q++)
+        if (fabs(U[npk][nr][n][q]) > 1.0e-8) { if (n < nMin) nMin = n; if (n > nMax) nMax = n; }
+  nm[npk] = nMin; nm[TD_NPK + npk] = nMax;
 }
 
 // ---------------------------------------------------------------------
 // p-adaptive Clenshaw-Curtis cubature on [0,1]^dim, dim <= 3, with the
-// value cache and traversal of pcubature.c:60-430.  Cache entry e holds
-// the points of rule orders m[] whose dimension mi is at its newest level
-// (mi = dim: the full initial 3^dim grid), dimension 0 slowest.
+// value cache and traversal of pcubature.c:60-430, run by one wave: the
+// points of a new block are evaluated across the lanes, and the rule sums
+// over the cache (the full rule and its reduction in each dimension) are
+// taken in one lane-strided pass and summed over the wave.  Cache entry e
+// holds the points of rule orders m[] whose dimension mi is at its newest
+// level (mi = dim: the full initial 3^dim grid), dimension 0 slowest.
 // ---------------------------------------------------------------------
 struct PCEntry { int m0, m1, m2, mi, off; };
 
@@ -538,91 +543,142 @@ __device__ inline double pc_weight(int cm, bool isnew, int m, bool red, int l)
   return j < nx ? w[1 + j] : -1.0;
 }
 
-template <typename F>
-__device__ void pc_add(F &f, int dim, const int *m, int mi, PCEntry *E, int &ne, double *vals, int &nval, int fdim)
+// sum over the 64 lanes; every lane gets the same bits (each butterfly
+// step adds the same two operands on both partners)
+__device__ inline double wave_sum(double v)
 {
-  PCEntry &e = E[ne++];
-  e.m0 = m[0]; e.m1 = dim > 1 ? m[1] : 0; e.m2 = dim > 2 ? m[2] : 0; e.mi = mi; e.off = nval;
+  for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+  return v;
+}
+
+// appends the block of points of rule orders m[] whose dimension mi is at
+// its newest level: entry E[ne] (LDS, lane 0), values at vals + nval
+template <typename F>
+__device__ void pc_add(F &f, int dim, const int *m, int mi, PCEntry *E, int &ne, double *vals, int &nval)
+{
+  if (threadIdx.x == 0) {
+    PCEntry &e = E[ne];
+    e.m0 = m[0]; e.m1 = dim > 1 ? m[1] : 0; e.m2 = dim > 2 ? m[2] : 0; e.mi = mi; e.off = nval;
+  }
+  ne++;
   int cnt[3] = {1, 1, 1};
   for (int d = 0; d < dim; d++) cnt[d] = pc_count(m[d], d == mi);
   const int total = cnt[0] * cnt[1] * cnt[2];
-  for (int L = 0; L < total; L++) {
+  for (int L = threadIdx.x; L < total; L += 64) {
     int l[3] = {0, 0, 0}, rem = L;
     for (int d = dim - 1; d >= 0; d--) { l[d] = rem % cnt[d]; rem /= cnt[d]; }
     double y[3];
     for (int d = 0; d < dim; d++) y[d] = pc_coord(m[d], d == mi, l[d]);
-    f(y, vals + (long)(nval + L) * fdim);
+    f(y, vals + (long)(nval + L) * SW_FDIM);
   }
   nval += total;
+  __syncthreads();
 }
 
-// sum over the cache of the rule of order m, reduced in dimension md (md
-// = dim: none), into out[fdim]
-__device__ void pc_evals(int dim, const PCEntry *E, int ne, const double *vals, const int *m, int md, double V,
-                         double *out, int fdim)
+// the rule of order m over the cache, reduced in dimension md (md = dim:
+// none), for md = 0..dim at once: S[md][q], identical on every lane
+__device__ void pc_evals(int dim, const PCEntry *E, int ne, const double *vals, const int *m, double V,
+                         double (*S)[SW_FDIM])
 {
-  for (int q = 0; q < fdim; q++) out[q] = 0.0;
+#pragma unroll
+  for (int md = 0; md < 4; md++)
+#pragma unroll
+    for (int q = 0; q < SW_FDIM; q++) S[md][q] = 0.0;
   for (int c = 0; c < ne; c++) {
-    const PCEntry &e = E[c];
+    const PCEntry e = E[c];
     const int cm[3] = {e.m0, e.m1, e.m2};
-    if (!(e.mi >= dim || cm[e.mi] + (e.mi == md) <= m[e.mi])) continue;
+    bool inc[4];
+#pragma unroll
+    for (int md = 0; md < 4; md++)
+      inc[md] = md <= dim && (e.mi >= dim || cm[e.mi] + (e.mi == md) <= m[e.mi]);
+    if (!(inc[0] || inc[1] || inc[2] || inc[3])) continue;
     int cnt[3] = {1, 1, 1};
     for (int d = 0; d < dim; d++) cnt[d] = pc_count(cm[d], d == e.mi);
     const int total = cnt[0] * cnt[1] * cnt[2];
-    for (int L = 0; L < total; L++) {
+    for (int L = threadIdx.x; L < total; L += 64) {
       int l[3] = {0, 0, 0}, rem = L;
       for (int d = dim - 1; d >= 0; d--) { l[d] = rem % cnt[d]; rem /= cnt[d]; }
-      double wt = V;
-      bool use = true;
-      for (int d = 0; d < dim && use; d++) {
-        double w = pc_weight(cm[d], d == e.mi, m[d], d == md, l[d]);
-        if (w < 0) use = false;
-        else wt *= w;
+      double wf[3] = {1, 1, 1}, wr[3] = {1, 1, 1};
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if (d < dim) {
+          wf[d] = pc_weight(cm[d], d == e.mi, m[d], false, l[d]);
+          wr[d] = pc_weight(cm[d], d == e.mi, m[d], true, l[d]);
+        }
+      const double *v = vals + (long)(e.off + L) * SW_FDIM;
+      double vv[SW_FDIM];
+#pragma unroll
+      for (int q = 0; q < SW_FDIM; q++) vv[q] = v[q];
+#pragma unroll
+      for (int md = 0; md < 4; md++) {
+        if (!inc[md]) continue;
+        double wt = V;
+        bool use = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          if (d < dim) {
+            const double w = d == md ? wr[d] : wf[d];
+            if (w < 0) use = false;
+            else wt *= w;
+          }
+        if (!use) continue;
+#pragma unroll
+        for (int q = 0; q < SW_FDIM; q++) S[md][q] += vv[q] * wt;
       }
-      if (!use) continue;
-      const double *v = vals + (long)(e.off + L) * fdim;
-      for (int q = 0; q < fdim; q++) out[q] += v[q] * wt;
     }
   }
+#pragma unroll
+  for (int md = 0; md < 4; md++)
+    if (md <= dim)
+#pragma unroll
+      for (int q = 0; q < SW_FDIM; q++) S[md][q] = wave_sum(S[md][q]);
 }
 
-// returns the integral in val[fdim]
+// the integral into val[SW_FDIM] (every lane); vals: the job's value
+// cache in scratch memory, E: its entries in LDS
 template <typename F>
-__device__ void pcubature_dev(F &f, int dim, int fdim, double *slab_vals, PCEntry *E, double *val)
+__device__ void pcubature_wave(F &f, int dim, double *vals, PCEntry *E, double *val)
 {
-  if (dim == 0) { double y[3] = {0, 0, 0}; f(y, val); return; }
   int m[3] = {0, 0, 0};
   double V = 1.0;
   for (int d = 0; d < dim; d++) V *= 0.5;
   int ne = 0, nval = 0;
   long numEval = 0;
-  pc_add(f, dim, m, dim, E, ne, slab_vals, nval, fdim);
-  double val1[SW_FDIM], err[SW_FDIM];
+  pc_add(f, dim, m, dim, E, ne, vals, nval);
   for (;;) {
+    double S[4][SW_FDIM];
+    pc_evals(dim, E, ne, vals, m, V, S);
+    double err[SW_FDIM];
+#pragma unroll
+    for (int q = 0; q < SW_FDIM; q++) {
+      val[q] = dim == 1 ? S[1][q] : (dim == 2 ? S[2][q] : S[3][q]);
+      err[q] = 0.0;
+    }
     int mi = 0;
-    pc_evals(dim, E, ne, slab_vals, m, dim, V, val, fdim);
-    for (int q = 0; q < fdim; q++) err[q] = 0.0;
     double maxerr = 0.0;
-    for (int i = 0; i < dim; i++) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      if (i >= dim) continue;
       double emax = 0.0;
-      pc_evals(dim, E, ne, slab_vals, m, i, V, val1, fdim);
-      for (int q = 0; q < fdim; q++) {
-        double e = fabs(val[q] - val1[q]);
+#pragma unroll
+      for (int q = 0; q < SW_FDIM; q++) {
+        double e = fabs(val[q] - S[i][q]);
         if (e > emax) emax = e;
         if (e > err[q]) err[q] = e;
       }
       if (emax > maxerr) { maxerr = emax; mi = i; }
     }
     bool conv = true;
-    for (int q = 0; q < fdim; q++)
-      if (err[q] > 0.0 && err[q] > fabs(val[q]) * TD_RELTOL) { conv = false; break; }
+#pragma unroll
+    for (int q = 0; q < SW_FDIM; q++)
+      if (err[q] > 0.0 && err[q] > fabs(val[q]) * TD_RELTOL) conv = false;
     if (conv || numEval > TD_MAXEVAL) break;
     m[mi] += 1;
     if (m[mi] >= PC_M) break;     // not reached within TD_MAXEVAL points
     int nb = 1;
     for (int d = 0; d < dim; d++) nb *= pc_count(m[d], d == mi);
     if (ne >= PC_MAXENT || nval + nb > PC_MAXVAL) { m[mi] -= 1; break; }
-    pc_add(f, dim, m, mi, E, ne, slab_vals, nval, fdim);
+    pc_add(f, dim, m, mi, E, ne, vals, nval);
     numEval += nb;
   }
 }
@@ -656,7 +712,9 @@ __device__ void sw_combos(const cplx S1, const cplx *Sx, const cplx *Sy, const c
 }
 
 // no common vertex: 78 x 78-point rule without desingularization
-// (GetPPIs_Cubature with HighOrder, PanelPanelInteractions.cc:437-441)
+// (GetPPIs_Cubature with HighOrder, PanelPanelInteractions.cc:437-441);
+// the 6084 point pairs strided over the lanes, the moments summed over the
+// wave (every lane gets them)
 __device__ void sw_hocub(const NearPair &P, cplx k, int needH1, cplx *H0, cplx *H1)
 {
   double aA[3], aB[3], bA[3], bB[3], D[3], Qal[9], Qbl[9];
@@ -666,35 +724,36 @@ __device__ void sw_hocub(const NearPair &P, cplx k, int needH1, cplx *H0, cplx *
     D[c] = P.Va[c] - P.Vb[c];
     for (int q = 0; q < 3; q++) { Qal[3 * q + c] = P.Qa[3 * q + c] - P.Va[c]; Qbl[3 * q + c] = P.Qb[3 * q + c] - P.Vb[c]; }
   }
-  cplx S1 = mk(0, 0), Sxy = mk(0, 0), s = mk(0, 0), Sx[3], Sy[3], u[3], v[3], w[3];
-  for (int c = 0; c < 3; c++) { Sx[c] = Sy[c] = u[c] = v[c] = w[c] = mk(0, 0); }
+  // S1, Sx[3], Sy[3], Sxy, s, u[3], v[3], w[3]
+  cplx acc[18];
+  for (int q = 0; q < 18; q++) acc[q] = mk(0, 0);
   cplx ik = IMUL(k);
-  for (int i = 0; i < 78; i++) {
-    const double wi = c_swtcr20[3 * i + 2];
-    double x[3];
-    for (int c = 0; c < 3; c++) x[c] = c_swtcr20[3 * i] * aA[c] + c_swtcr20[3 * i + 1] * aB[c];
-    for (int j = 0; j < 78; j++) {
-      const double wj = c_swtcr20[3 * j + 2];
-      double y[3];
-      for (int c = 0; c < 3; c++) y[c] = c_swtcr20[3 * j] * bA[c] + c_swtcr20[3 * j + 1] * bB[c];
-      double R[3] = {D[0] + x[0] - y[0], D[1] + x[1] - y[1], D[2] + x[2] - y[2]};
-      double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]), ir = 1.0 / r;
-      double e = exp(-k.i * r), sn, cs;
-      sincos(k.r * r, &sn, &cs);
-      cplx Phi = mk(e * cs, e * sn) * (SCUFFX_INV4PI * ir) * (wi * wj);
-      S1 += Phi;
-      for (int c = 0; c < 3; c++) { Sx[c] += x[c] * Phi; Sy[c] += y[c] * Phi; }
-      Sxy += (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) * Phi;
-      if (needH1) {
-        cplx Psi = Phi * (ik - mk(ir, 0)) * ir;
-        double xy[3], Rx[3], yR[3];
-        cross3(x, y, xy); cross3(R, x, Rx); cross3(y, R, yR);
-        s += (xy[0] * R[0] + xy[1] * R[1] + xy[2] * R[2]) * Psi;
-        for (int c = 0; c < 3; c++) { u[c] += Rx[c] * Psi; v[c] += yR[c] * Psi; w[c] += R[c] * Psi; }
-      }
+  for (int p = threadIdx.x; p < 78 * 78; p += 64) {
+    const int i = p / 78, j = p - 78 * (p / 78);
+    const double wi = c_swtcr20[3 * i + 2], wj = c_swtcr20[3 * j + 2];
+    double x[3], y[3];
+    for (int c = 0; c < 3; c++) {
+      x[c] = c_swtcr20[3 * i] * aA[c] + c_swtcr20[3 * i + 1] * aB[c];
+      y[c] = c_swtcr20[3 * j] * bA[c] + c_swtcr20[3 * j + 1] * bB[c];
+    }
+    double R[3] = {D[0] + x[0] - y[0], D[1] + x[1] - y[1], D[2] + x[2] - y[2]};
+    double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]), ir = 1.0 / r;
+    double e = exp(-k.i * r), sn, cs;
+    sincos(k.r * r, &sn, &cs);
+    cplx Phi = mk(e * cs, e * sn) * (SCUFFX_INV4PI * ir) * (wi * wj);
+    acc[0] += Phi;
+    for (int c = 0; c < 3; c++) { acc[1 + c] += x[c] * Phi; acc[4 + c] += y[c] * Phi; }
+    acc[7] += (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) * Phi;
+    if (needH1) {
+      cplx Psi = Phi * (ik - mk(ir, 0)) * ir;
+      double xy[3], Rx[3], yR[3];
+      cross3(x, y, xy); cross3(R, x, Rx); cross3(y, R, yR);
+      acc[8] += (xy[0] * R[0] + xy[1] * R[1] + xy[2] * R[2]) * Psi;
+      for (int c = 0; c < 3; c++) { acc[9 + c] += Rx[c] * Psi; acc[12 + c] += yR[c] * Psi; acc[15 + c] += R[c] * Psi; }
     }
   }
-  sw_combos(S1, Sx, Sy, Sxy, s, u, v, w, Qal, Qbl, k, needH1, H0, H1);
+  for (int q = 0; q < (needH1 ? 18 : 8); q++) { acc[q].r = wave_sum(acc[q].r); acc[q].i = wave_sum(acc[q].i); }
+  sw_combos(acc[0], acc + 1, acc + 4, acc[7], acc[8], acc + 9, acc + 12, acc + 15, Qal, Qbl, k, needH1, H0, H1);
 }
 
 static_assert(SW_SLAB == SCUFFX_SW_SLAB, "scratch slab size");
@@ -705,42 +764,48 @@ struct TDFun {
   __device__ void operator()(const double *y, double *out) const { td_sum(*W, U, y, out); }
 };
 
-// one thread per job; kind 0: no-common-vertex near pair (all 9
-// combinations), 1: common-edge/vertex near pair, one (Qa, Qb)
+// one wave (block of 64) per job; kind 0: no-common-vertex near pair (all
+// 9 combinations), 1: common-edge/vertex near pair, one (Qa, Qb).  A
+// Taylor-Duffy job keeps its Upsilon tables and cache entries in LDS and
+// its cached integrand values in a slab of scratch memory.
 __global__ void __launch_bounds__(64)
 k_sw(const SWJob *jobs, int job0, int njobs, const NearPair *nps, const double *V, const int *PVI, int nreg,
      const RegionK *regs, int needH1, double *HN, double *HS, double *scratch)
 {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sU[TD_NPK][TD_NREG][TD_NWP][TD_NMON];
+  __shared__ PCEntry sE[PC_MAXENT];
+  __shared__ int sNm[2 * TD_NPK];
+  const int t = blockIdx.x;
   if (t >= njobs) return;
   const SWJob J = jobs[job0 + t];
   const cplx k = regs[J.region].k;
   const int stride = needH1 ? 4 : 2;
+  const int lane = threadIdx.x;
   if (J.kind == 0) {
     cplx H0[9], H1[9];
     sw_hocub(nps[J.pair], k, needH1, H0, H1);
     double *o = HN + ((long long)J.pair * nreg + J.region) * 9 * stride;
-    for (int q = 0; q < 9; q++) {
-      o[q * stride] = H0[q].r; o[q * stride + 1] = H0[q].i;
-      if (needH1) { o[q * stride + 2] = H1[q].r; o[q * stride + 3] = H1[q].i; }
-    }
+    if (lane == 0)
+      for (int q = 0; q < 9; q++) {
+        o[q * stride] = H0[q].r; o[q * stride + 1] = H0[q].i;
+        if (needH1) { o[q * stride + 2] = H1[q].r; o[q * stride + 3] = H1[q].i; }
+      }
     return;
   }
-  double *slab = scratch + (long long)t * SW_SLAB;
-  double (*U)[TD_NREG][TD_NWP][TD_NMON] = (double (*)[TD_NREG][TD_NWP][TD_NMON])slab;
-  PCEntry *E = (PCEntry *)(slab + TD_NPK * TD_NREG * TD_NWP * TD_NMON);
-  double *vals = slab + TD_NPK * TD_NREG * TD_NWP * TD_NMON + 4 * PC_MAXENT;
+  double *vals = scratch + (long long)t * SW_SLAB;
   TDGeo W;
   W.k = k;
   const int a = J.q / 3, b = J.q % 3;
+  const NearPair &P = nps[J.pair];
+  W.WhichCase = P.ncv; W.NumPKs = 3;
+  W.KIndex[0] = K_HELM; W.KIndex[1] = K_HELM; W.KIndex[2] = K_GRADHELM;
+  td_geometry(P.ncv, P.Va, P.Va + 3, P.Va + 6, P.Vb + 3, P.Vb + 6, P.Qa + 3 * a, P.Qb + 3 * b, W, sU, sNm);
+  __syncthreads();
+  for (int npk = 0; npk < TD_NPK; npk++) { W.nMin[npk] = sNm[npk]; W.nMax[npk] = sNm[TD_NPK + npk]; }
+  TDFun f{&W, sU};
   double R[SW_FDIM];
-  {
-    const NearPair &P = nps[J.pair];
-    W.WhichCase = P.ncv; W.NumPKs = 3;
-    W.KIndex[0] = K_HELM; W.KIndex[1] = K_HELM; W.KIndex[2] = K_GRADHELM;
-    td_geometry(P.ncv, P.Va, P.Va + 3, P.Va + 6, P.Vb + 3, P.Vb + 6, P.Qa + 3 * a, P.Qb + 3 * b, W, U);
-    TDFun f{&W, U};
-    pcubature_dev(f, 4 - P.ncv, 2 * W.NumPKs, vals, E, R);
+  pcubature_wave(f, 4 - P.ncv, vals, sE, R);
+  if (lane == 0) {
     cplx H0 = mk(R[2], R[3]) - 4.0 * mk(R[0], R[1]) / (k * k), H1 = mk(R[4], R[5]);
     double *o = HN + ((long long)J.pair * nreg + J.region) * 9 * stride + J.q * stride;
     o[0] = H0.r; o[1] = H0.i;
@@ -794,7 +859,7 @@ hipError_t launch_sw(hipStream_t stream, const SWJob *jobs, int njobs, const Nea
   }
   for (int j0 = 0; j0 < njobs; j0 += chunk) {
     int n = std::min(chunk, njobs - j0);
-    hipLaunchKernelGGL(k_sw, dim3((n + 63) / 64), dim3(64), 0, stream, jobs, j0, n, nps, V, PVI, nreg, regs, needH1,
+    hipLaunchKernelGGL(k_sw, dim3(n), dim3(64), 0, stream, jobs, j0, n, nps, V, PVI, nreg, regs, needH1,
                        HN, HS, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
