@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -200,6 +201,7 @@ struct PanelScratch {
   void *rowc;      // [2][PW] T
   int *plan;       // [2*PW+1]: dst rows, src rows, count of the panel's net row permutation
   void *linv;      // [npanels][PW][PW] T or null: inverse of each panel's unit-lower block (column-major)
+  unsigned long long *ts;   // tournament-pivoting sets and root area (k_panel_ts), or null
 };
 
 
@@ -271,6 +273,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
   __shared__ long spv[PW];
   __shared__ int s_p;
   __shared__ R s_bv;
+  __builtin_amdgcn_s_setprio(3);   // critical path beside the trailing GEMM
   const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
   const long lo = r0 + (long)g * NR;
   const int nrow = (int)std::min<long>(NR, r0 + m - lo);
@@ -451,6 +454,384 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
     long srow = __shfl(row, content);
     if (t < cnt) { S.plan[t] = (int)row; S.plan[PW * 2 + t] = (int)srow; }
     if (t == 0) S.plan[PW * 4] = cnt;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Tournament-pivoting panel for the complex64 factorization of the mixed
+// solver (communication-avoiding LU, "TSLU": Grigori, Demmel and Xiang,
+// SIAM J. Matrix Anal. Appl. 32 (2011) 1317-1350).  Each workgroup factors
+// its 256 rows x w columns in LDS with partial pivoting and nominates the w
+// rows it pivoted on; groups of TS_FAN nominee sets are stacked and factored
+// again, up a tree, until one workgroup factors the last stack: its w pivot
+// rows, in order, are the panel's pivots and its factors the w x w diagonal
+// block L11\U11.  Every workgroup then writes its rows of the permuted panel:
+// the diagonal block, or L21 rows = (original row) U11^-1 by substitution.
+// Pivot choices differ from row-by-row partial pivoting (the complex128
+// path keeps that); the mixed solver's refinement in complex128 takes the
+// factors as a preconditioner.  Cross-workgroup data moves only through
+// agent-scope relaxed atomics with drain-then-flag, as in k_panel: three
+// waits per panel (two at most 64 workgroups) instead of one per column.
+// ---------------------------------------------------------------------
+#define TS_FAN 8
+#define TS_SET 1088                 // u64 words of one nominee set
+#define TS_IDX (PW * PW)            // set: rows [PW][PW] (original values), row indices [PW], count, flag
+#define TS_CNT (PW * PW + PW)
+#define TS_FLAG (PW * PW + PW + 8)
+#define TS_TOP (PW * PW)            // root area: final block [PW][PW], top rows [PW][PW], plan, flag
+#define TS_PLAN (2 * PW * PW)
+#define TS_NPLAN (2 * PW * PW + 2 * PW)
+#define TS_FIN (2 * PW * PW + 2 * PW + 8)
+#define TS_ROOT 2176
+
+__host__ __device__ inline int ts_levels(long G)
+{
+  int l = 0;
+  for (long c = G; c > 1; c = (c + TS_FAN - 1) / TS_FAN) l++;
+  return l;
+}
+
+// r - l c with fused multiply-adds (complex64)
+__device__ inline float2 cfms(float2 r, float2 l, float2 c)
+{
+  float2 o;
+  o.x = fmaf(-l.x, c.x, fmaf(l.y, c.y, r.x));
+  o.y = fmaf(-l.x, c.y, fmaf(-l.y, c.x, r.y));
+  return o;
+}
+
+// maximum of a 32-bit key over the wave, in every lane: within each row
+// of 16 lanes by DPP (quad swaps, half-row and row mirrors), then over the
+// four rows by lane reads (the result is wave-uniform).  All lanes active.
+__device__ inline unsigned wave_max_key(unsigned k)
+{
+  k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+  k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+  k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x141, 0xF, 0xF, false));   // row_half_mirror
+  k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x140, 0xF, 0xF, false));   // row_mirror
+  const unsigned a = __builtin_amdgcn_readlane(k, 0), b = __builtin_amdgcn_readlane(k, 16);
+  const unsigned c = __builtin_amdgcn_readlane(k, 32), d = __builtin_amdgcn_readlane(k, 48);
+  return max(max(a, b), max(c, d));
+}
+
+// partial-pivoting LU (largest |re|+|im|, lowest row on ties) of the rows
+// [0, nr) the threads of the workgroup hold in registers (thread t: row t,
+// r[0..w-1]), rows left in place: spiv[j] = row of pivot j, srank[t] = j
+// on pivot rows (else -1); each row ends as its row of the packed factors
+// (multipliers left of its pivot step, or its U row).  Returns min(w, nr);
+// *zero = first step with a zero pivot or -1.  The pivot search compares
+// keys bits(|re|+|im|) + 1 (0: row already a pivot or absent), which order
+// as the non-negative magnitudes do.  One barrier per column: the winning
+// row of each wave goes through LDS (two buffers: a wave writes buffer
+// j & 1 again only after every wave has passed step j + 1's barrier, i.e.
+// has read it).
+__device__ __attribute__((always_inline)) inline int ts_lu(float2 (&r)[PW], int nr, int w, int *spiv, int *srank,
+                                                           unsigned (*wk)[4], int (*wi)[4], float2 (*cand)[4][PW],
+                                                           int *zero)
+{
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  bool used = t >= nr;
+  srank[t] = -1;
+  const int np = min(w, nr);
+  int z = -1;
+#pragma unroll
+  for (int j = 0; j < PW; j++) {
+    if (j < np) {
+      const unsigned key = used ? 0u : __float_as_uint(fabsf(r[j].x) + fabsf(r[j].y)) + 1u;
+      const unsigned wm = wave_max_key(key);
+      const unsigned long long hit = __ballot(wm != 0 && key == wm);
+      const int wl = hit ? __ffsll(hit) - 1 : -1;
+      if (lane == wl) {
+#pragma unroll
+        for (int q = j; q < PW; q++) cand[j & 1][wave][q] = r[q];
+        wi[j & 1][wave] = t;
+      }
+      if (lane == 0) {
+        wk[j & 1][wave] = wm;
+        if (wl < 0) wi[j & 1][wave] = -1;
+      }
+      __syncthreads();
+      unsigned b0 = wk[j & 1][0];
+      int p = wi[j & 1][0];
+      for (int q = 1; q < 4; q++) {
+        const unsigned ob = wk[j & 1][q];
+        if (ob > b0) { b0 = ob; p = wi[j & 1][q]; }
+      }
+      const int pw = p >> 6;
+      if (t == p) { used = true; srank[t] = j; }
+      if (t == 0) spiv[j] = p;
+      if (b0 == 1u && z < 0) z = j;
+      if (!used) {
+        float2 l = r[j];
+        if (b0 != 1u) { l = cmul(l, crecip(cand[j & 1][pw][j])); r[j] = l; }
+#pragma unroll
+        for (int q = j + 1; q < PW; q++) r[q] = cfms(r[q], l, cand[j & 1][pw][q]);
+      }
+    }
+  }
+  __syncthreads();
+  *zero = z;
+  return np;
+}
+
+// nominees in the level-l set of workgroup g: min(w, rows of workgroups g
+// .. g + TS_FAN^l - 1), by induction over the merges
+__device__ inline int ts_count(long g, long span, long m, int w)
+{
+  const long rows = min(span * 256, m - g * 256);
+  return (int)min((long)w, rows);
+}
+
+// wave 0 waits until the n <= 64 words f[k * stride] hold seq (bounded as
+// in k_panel), then the workgroup synchronizes
+__device__ void ts_wait(const unsigned long long *f, long stride, int n, unsigned long long seq, unsigned *err)
+{
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    unsigned spins = 0;
+    for (;;) {
+      bool ok = true;
+      if (lane < n) ok = ld_coh(f + lane * stride) == seq;
+      if (__ballot(!ok) == 0) break;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (++spins > GBAR_SPIN) { __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  }
+  __syncthreads();
+}
+
+// panel rows [r0, r0 + m) x columns [r0, r0 + w), w <= PW; grid: one
+// workgroup per 256 rows, all co-resident; outputs as k_panel (factors in
+// place, ipiv, info, the net row permutation in S.plan, the inverse of the
+// unit-lower diagonal block in S.linv)
+// (at most 168 VGPRs, so that a workgroup fits on SIMDs that run three
+// waves of the trailing GEMM instead of waiting for them to retire)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax, unsigned seq0)
+{
+  constexpr int NR = 256;
+  __shared__ float2 sF[PW][PW + 1];     // final block, row-major
+  __shared__ float2 cand[2][NR / 64][PW];
+  __shared__ float2 sRd[PW];
+  __shared__ unsigned wk[2][NR / 64];
+  __shared__ int wi[2][NR / 64];
+  __shared__ int spiv[PW], srank[NR], ssrc[NR], sfrom[NR];
+  __shared__ long sidx[NR];
+  // the panel is the critical path and the trailing GEMM shares its CUs:
+  // its waves win instruction issue on their SIMDs
+  __builtin_amdgcn_s_setprio(3);
+  const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x;
+  const long lo = r0 + (long)g * NR;
+  const int nrow = (int)min((long)NR, r0 + m - lo);
+  const long i = lo + t;
+  const unsigned long long seq = seq0;
+  unsigned *err = S.bar + 64;
+  unsigned long long *ts = S.ts;
+  unsigned long long *root = ts + (long)(ts_levels(Gmax) + 1) * Gmax * TS_SET;
+  const int nl = ts_levels(G);
+
+  // own rows into registers (A keeps the original values until the last
+  // phase, which re-reads them); workgroup 0 also publishes rows r0 .. r0 +
+  // w - 1 (those the interchanges displace)
+  float2 r[PW];
+#pragma unroll
+  for (int q = 0; q < PW; q++) {
+    r[q].x = 0; r[q].y = 0;
+    if (t < nrow && q < w) r[q] = A[i + (r0 + q) * lda];
+  }
+  if (g == 0 && t < w) {
+#pragma unroll
+    for (int q = 0; q < PW; q++)
+      if (q < w) st_coh((float2 *)(root + TS_TOP + t * PW + q), r[q]);
+  }
+  sidx[t] = lo + t;
+  // stage 0: this workgroup's rows; stage l >= 1: workgroups g % TS_FAN^l
+  // == 0 stack the nominee sets of g, g + sp, .., g + (TS_FAN - 1) sp (sp =
+  // TS_FAN^(l-1)) from stage l - 1.  The last stage runs on workgroup 0.
+  int zero = -1;
+  long sp = 1;
+  for (int l = 0; l <= nl; l++) {
+    int nr = nrow;
+    const unsigned long long *prev = ts + (long)(l > 0 ? l - 1 : 0) * Gmax * TS_SET;
+    if (l > 0) {
+      if (g % (sp * TS_FAN)) break;
+      const int nch = (int)min((long)TS_FAN, (G - g + sp - 1) / sp);
+      ts_wait(prev + (long)g * TS_SET + TS_FLAG, sp * TS_SET, nch, seq, err);
+      // the stack: child k's nominees at rows sbase[k] ..
+      int sbase[TS_FAN + 1];
+      sbase[0] = 0;
+#pragma unroll
+      for (int k = 0; k < TS_FAN; k++)
+        sbase[k + 1] = sbase[k] + (k < nch ? ts_count(g + k * sp, sp, m, w) : 0);
+      nr = sbase[TS_FAN];
+      if (t < nr) {
+        int k = 0;
+#pragma unroll
+        for (int q = 1; q < TS_FAN; q++)
+          if (t >= sbase[q]) k = q;
+        const int rr = t - sbase[k];
+        const unsigned long long *cs = prev + (g + k * sp) * TS_SET;
+#pragma unroll
+        for (int q = 0; q < PW; q++)
+          if (q < w) r[q] = ld_coh((const float2 *)(cs + rr * PW + q));
+        sidx[t] = (long)ld_coh(cs + TS_IDX + rr);
+        ssrc[t] = k * PW + rr;
+      }
+    }
+    const int np = ts_lu(r, nr, w, spiv, srank, wk, wi, cand, &zero);
+    if (l == nl) {
+      // workgroup 0: the final block in pivot order
+      const int k = t < nr ? srank[t] : -1;
+      if (k >= 0) {
+#pragma unroll
+        for (int q = 0; q < PW; q++) sF[k][q] = r[q];
+      }
+      break;
+    }
+    // nominees: original values of the pivot rows
+    unsigned long long *set = ts + ((long)l * Gmax + g) * TS_SET;
+    const int k = t < nr ? srank[t] : -1;
+    if (k >= 0) {
+      if (l == 0) {
+#pragma unroll
+        for (int q = 0; q < PW; q++)
+          if (q < w) st_coh((float2 *)(set + k * PW + q), A[i + (r0 + q) * lda]);
+      } else {
+        const unsigned long long *cs = prev + (g + (ssrc[t] / PW) * sp) * TS_SET;
+        const int rr = ssrc[t] % PW;
+#pragma unroll
+        for (int q = 0; q < PW; q++)
+          if (q < w) st_coh((float2 *)(set + k * PW + q), ld_coh((const float2 *)(cs + rr * PW + q)));
+      }
+      st_coh(set + TS_IDX + k, (unsigned long long)sidx[t]);
+    }
+    (void)np;
+    drain();
+    __syncthreads();
+    if (t == 0) st_coh(set + TS_FLAG, seq);
+    if (l > 0) sp *= TS_FAN;
+  }
+
+  if (g == 0) {
+    // publish the final block; ipiv, the net permutation and info
+    __syncthreads();
+    for (int e = t; e < PW * PW; e += NR) {
+      const int rr = e / PW, q = e % PW;
+      if (rr < w && q < w) st_coh((float2 *)(root + rr * PW + q), sF[rr][q]);
+    }
+    if (t < 64) {
+      // lanes hold (position, original row now there) for every touched
+      // position: lanes 0..w-1 the diagonal block's, pivot rows from
+      // below appended (lane indices are wave-uniform: lane reads)
+      int row = (t < w) ? (int)(r0 + t) : -1, content = row;
+      int cnt = w;
+      for (int j = 0; j < w; j++) {
+        const int R = (int)sidx[spiv[j]];
+        const unsigned long long hit = __ballot(content == R);
+        int ip;
+        if (hit) ip = __ffsll(hit) - 1;
+        else { ip = cnt; if (t == cnt) { row = R; content = R; } cnt++; }
+        const int pos = __builtin_amdgcn_readlane(row, ip);
+        if (t == 0) ipiv[r0 + j] = pos + 1;
+        const int cj = __builtin_amdgcn_readlane(content, j), cp = __builtin_amdgcn_readlane(content, ip);
+        if (t == j) content = cp;
+        if (t == ip) content = cj;
+      }
+      if (t < cnt) {
+        S.plan[t] = (int)row;
+        S.plan[PW * 2 + t] = (int)content;
+        st_coh(root + TS_PLAN + t, ((unsigned long long)row << 32) | (unsigned long long)content);
+      }
+      if (t == 0) {
+        S.plan[PW * 4] = cnt;
+        st_coh(root + TS_NPLAN, (unsigned long long)cnt);
+        if (zero >= 0 && *info == 0) *info = (int)(r0 + zero + 1);
+      }
+    }
+    // the inverse of the unit-lower w x w block, column t by substitution
+    if (S.linv && t < PW) {
+      float2 x[PW];
+#pragma unroll
+      for (int rr = 0; rr < PW; rr++) {
+        float2 val;
+        val.x = (rr == t) ? 1 : 0; val.y = 0;
+        if (rr < w && t < w) {
+#pragma unroll
+          for (int k = 0; k < rr; k++) val = cfms(val, sF[rr][k], x[k]);
+        }
+        x[rr] = val;
+      }
+      float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)t * PW;
+#pragma unroll
+      for (int rr = 0; rr < PW; rr++) li[rr] = x[rr];
+    }
+    drain();
+    __syncthreads();
+    if (t == 0) st_coh(root + TS_FIN, seq);
+  }
+
+  // every workgroup: its rows of the permuted panel
+  ts_wait(root + TS_FIN, 0, 1, seq, err);
+  for (int e = t; e < PW * PW; e += NR) {
+    const int rr = e / PW, q = e % PW;
+    if (rr < w && q < w) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+  }
+  // displaced rows of this workgroup: sfrom[row - lo] = original row
+  sfrom[t] = -1;
+  __syncthreads();
+  if (t < 2 * PW) {
+    const int np = (int)ld_coh(root + TS_NPLAN);
+    const unsigned long long pe = ld_coh(root + TS_PLAN + t);
+    const long dst = (long)(pe >> 32);
+    if (t < np && dst >= lo && dst < lo + nrow) sfrom[dst - lo] = (int)(pe & 0xFFFFFFFFull);
+  }
+  __syncthreads();
+  if (t < w) {
+    const float2 d = sF[t][t];
+    sRd[t] = (d.x != 0 || d.y != 0) ? crecip(d) : d;
+  }
+  __syncthreads();
+  if (t < nrow) {
+    if (i < r0 + w) {
+#pragma unroll
+      for (int q = 0; q < PW; q++)
+        if (q < w) A[i + (r0 + q) * lda] = sF[i - r0][q];
+    } else {
+      const long c = sfrom[t] >= 0 ? sfrom[t] : i;
+      float2 v[PW];
+#pragma unroll
+      for (int q = 0; q < PW; q++) { v[q].x = 0; v[q].y = 0; }
+      if (c == i) {
+#pragma unroll
+        for (int q = 0; q < PW; q++)
+          if (q < w) v[q] = A[i + (r0 + q) * lda];
+      } else if (c >= r0 && c < r0 + w) {
+        // a row displaced from the diagonal block (no other source exists;
+        // anything else is reported as a failed panel)
+#pragma unroll
+        for (int q = 0; q < PW; q++)
+          if (q < w) v[q] = ld_coh((const float2 *)(root + TS_TOP + (c - r0) * PW + q));
+      } else {
+        __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // x U11 = (original row); a zero pivot leaves its column unscaled
+#pragma unroll
+      for (int q = 0; q < PW; q++) {
+        if (q < w) {
+          float2 x = v[q];
+#pragma unroll
+          for (int k = 0; k < q; k++) x = cfms(x, v[k], sF[k][q]);
+          const float2 rd = sRd[q];
+          if (rd.x != 0 || rd.y != 0) x = cmul(x, rd);
+          v[q] = x;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PW; q++)
+        if (q < w) A[i + (r0 + q) * lda] = v[q];
+    }
   }
 }
 
@@ -729,7 +1110,14 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   const int ib = PW;
   constexpr int NR = PanelRows<T>::n;
   const long Gmax = (n + NR - 1) / NR;
-  const void *panel_fn = (const void *)k_panel<T>;
+  // the mixed solver's complex64 factorization pivots by tournament
+  // (SCUFFX_MIXED_PIVOTING=partial: row by row, as the complex128 path)
+  static const bool partial_env = [] {
+    const char *e = getenv("SCUFFX_MIXED_PIVOTING");
+    return e && !strcmp(e, "partial");
+  }();
+  const bool tourn = fast && std::is_same<T, float2>::value && !partial_env;
+  const void *panel_fn = tourn ? (const void *)k_panel_ts : (const void *)k_panel<T>;
   int dev = 0, ncu = 0, perCU = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -747,10 +1135,13 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   size_t off_cr = off_ci + 256 * ((sizeof(unsigned long long) * 2 * Gmax + 255) / 256);
   size_t off_rc = off_cr + 256 * ((sizeof(T) * 2 * Gmax * PW + 255) / 256);
   size_t off_li = off_rc + 256 * ((sizeof(T) * 2 * PW + 255) / 256);
-  size_t total = off_li + (fast ? sizeof(T) * PW * PW * npanels : 0);
+  size_t off_ts = off_li + 256 * (((fast ? sizeof(T) * PW * PW * npanels : 0) + 255) / 256);
+  const size_t ts_words = tourn ? (size_t)(ts_levels(Gmax) + 1) * Gmax * TS_SET + TS_ROOT : 0;
+  size_t total = off_ts + sizeof(unsigned long long) * ts_words;
   char *wsb = (char *)ws_get(WS_PANEL, total);
   if (!wsb) return -2;
   S.linv = fast ? (void *)(wsb + off_li) : nullptr;
+  S.ts = tourn ? (unsigned long long *)(wsb + off_ts) : nullptr;
   S.bar = (unsigned *)wsb;
   int *dinfo = (int *)(wsb + off_info);
   int *plans = (int *)(wsb + off_plan);
@@ -766,6 +1157,8 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   hipMemsetAsync(S.bar, 0, sizeof(unsigned) * 96, stream);
   // candidate keys start at sequence 0, which no column uses
   hipMemsetAsync(S.ckey, 0, sizeof(unsigned long long) * 2 * Gmax, stream);
+  // tournament sets and flags likewise
+  if (tourn) hipMemsetAsync(S.ts, 0, sizeof(unsigned long long) * ts_words, stream);
   // the side stream starts after the caller's stream has the matrix ready
   hipEventRecord(evR, stream);
   hipStreamWaitEvent(side, evR, 0);
@@ -849,7 +1242,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   hipEventDestroy(evP);
   hipEventDestroy(evR);
   if (prof_enabled()) prof_collect();
-  if (tmo && rc == 0) rc = -6;
+  if (tmo && rc == 0) rc = tmo == 1 ? -6 : -3;
   if (rc == 0 && hipGetLastError() != hipSuccess) rc = -3;
   return rc;
 }

@@ -157,13 +157,67 @@ def test_mixed_refinement(sx, n):
     assert np.array_equal(M.numpy(), A)
 
 
+def tournament_lu(A, w=32, rows=256, fan=8):
+    """The complex64 factorization's pivoting, restated in complex128:
+    32-column panels; every 256-row block nominates the rows a
+    partial-pivoting LU of it pivots on (largest |re|+|im|, first on
+    ties); nominee sets are stacked 8 at a time and factored again until
+    one stack is left, whose pivot rows, in order, are the panel's.
+    Returns ipiv (1-based, LAPACK interchange sequence)."""
+    def pivots(S):
+        S = S.copy()
+        used = np.zeros(S.shape[0], bool)
+        out = []
+        for j in range(min(S.shape[1], S.shape[0])):
+            a = np.where(used, -1.0, np.abs(S[:, j].real) + np.abs(S[:, j].imag))
+            p = int(np.argmax(a))
+            used[p] = True
+            out.append(p)
+            r = ~used
+            l = S[r, j] / S[p, j] if a[p] != 0 else S[r, j]
+            S[np.ix_(r, range(j + 1, S.shape[1]))] -= np.outer(l, S[p, j + 1:])
+        return out
+
+    A = A.copy()
+    n = A.shape[0]
+    ipiv = np.zeros(n, dtype=np.int64)
+    for r0 in range(0, n, w):
+        ww = min(w, n - r0)
+        P = A[r0:, r0:r0 + ww]
+        sets = [[r0 + b + p for p in pivots(P[b:b + rows])] for b in range(0, n - r0, rows)]
+        while len(sets) > 1:
+            sets = [[s[p] for p in pivots(A[s, r0:r0 + ww])]
+                    for s in (sum(sets[k:k + fan], []) for k in range(0, len(sets), fan))]
+        chosen = [sets[0][p] for p in pivots(A[sets[0], r0:r0 + ww])] if len(sets[0]) > ww else sets[0]
+        order = np.arange(n)
+        for j, R in enumerate(chosen):
+            pos = int(np.where(order == R)[0][0])
+            ipiv[r0 + j] = pos + 1
+            order[[r0 + j, pos]] = order[[pos, r0 + j]]
+            A[[r0 + j, pos]] = A[[pos, r0 + j]]
+        # LU of the diagonal block without further interchanges
+        B = A[r0:r0 + ww, r0:r0 + ww].copy()
+        for j in range(ww):
+            B[j + 1:, j] /= B[j, j]
+            B[j + 1:, j + 1:] -= np.outer(B[j + 1:, j], B[j, j + 1:])
+        L11 = np.tril(B, -1) + np.eye(ww)
+        U11 = np.triu(B)
+        A[r0:r0 + ww, r0:r0 + ww] = B
+        A[r0 + ww:, r0:r0 + ww] = np.linalg.solve(U11.T, A[r0 + ww:, r0:r0 + ww].T).T
+        A[r0:r0 + ww, r0 + ww:] = np.linalg.solve(L11, A[r0:r0 + ww, r0 + ww:])
+        A[r0 + ww:, r0 + ww:] -= A[r0 + ww:, r0:r0 + ww] @ A[r0:r0 + ww, r0 + ww:]
+    return ipiv
+
+
 @pytest.mark.parametrize("n", [40, 260, 700, 2600])
 def test_lu32_mixed_factors(sx, O, n):
-    """complex64 factorization of the mixed solver (partial pivoting,
+    """complex64 factorization of the mixed solver (tournament pivoting,
     block-row solves through the panel blocks' inverses): ipiv is a valid
-    interchange sequence, P A = L U to complex64 accuracy, |L| <= sqrt 2, and
-    the pivots agree with the oracle's partial pivoting on the rounded
-    matrix except near ties.  n = 260 leaves a 4-row workgroup."""
+    interchange sequence, P A = L U to complex64 accuracy, |L| stays small
+    (tournament pivoting bounds it less tightly than the sqrt(2) of
+    partial pivoting), and the pivots agree with the restatement above on
+    the rounded matrix except near ties.  n = 260 leaves a 4-row
+    workgroup; n = 2600 takes two merge levels."""
     import ctypes as C
     import torch
     print(f"lu32 mixed n={n}", flush=True)
@@ -186,8 +240,9 @@ def test_lu32_mixed_factors(sx, O, n):
     berr = np.linalg.norm(A32[perm] - L @ U) / np.linalg.norm(A32)
     print(f"  backward error {berr:.2e}, max |L| {np.abs(L).max():.2f}", flush=True)
     assert berr <= 2e-5 * np.sqrt(n / 40)
-    # pivots maximize |re|+|im| (izamax), so |l| <= sqrt(2) in modulus
-    assert np.abs(L).max() <= np.sqrt(2.0) + 1e-5
+    assert np.abs(L).max() <= 4.0
     if n <= 700:
-        _, ipo, _ = O.zgetrf(A32)
-        assert np.mean(piv == ipo) > 0.95
+        ipt = tournament_lu(A32)
+        agree = np.mean(piv == ipt)
+        print(f"  pivots agreeing with the restatement: {agree:.3f}", flush=True)
+        assert agree > 0.95
