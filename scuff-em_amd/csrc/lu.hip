@@ -158,22 +158,24 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       }
 }
 
+// lds_pad: extra (unused) LDS per workgroup, which caps how many of its
+// workgroups share a CU
 template <typename T, bool ACC>
 static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
-                 hipStream_t s)
+                 hipStream_t s, unsigned lds_pad = 0)
 {
   if (m <= 0 || n <= 0 || k <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
   dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
-  hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc);
   prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
 }
 
 template <typename T>
 static void gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
-                     hipStream_t s)
+                     hipStream_t s, unsigned lds_pad = 0)
 {
-  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s);
+  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s, lds_pad);
 }
 
 // ---------------------------------------------------------------------
@@ -482,7 +484,9 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
 #define TS_PLAN (2 * PW * PW)
 #define TS_NPLAN (2 * PW * PW + 2 * PW)
 #define TS_FIN (2 * PW * PW + 2 * PW + 8)
-#define TS_ROOT 2176
+#define TS_UREP 2176                // first replica (workgroup g reads replica g % 8)
+#define TS_REP 1056                 // u64 words between replicas of the final block
+#define TS_ROOT (2176 + 8 * TS_REP) // root area + 8 replicas of the final block
 
 __host__ __device__ inline int ts_levels(long G)
 {
@@ -666,6 +670,8 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
       for (int k = 0; k < TS_FAN; k++)
         sbase[k + 1] = sbase[k] + (k < nch ? ts_count(g + k * sp, sp, m, w) : 0);
       nr = sbase[TS_FAN];
+#pragma unroll
+      for (int q = 0; q < PW; q++) { r[q].x = 0; r[q].y = 0; }
       if (t < nr) {
         int k = 0;
 #pragma unroll
@@ -694,16 +700,35 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     unsigned long long *set = ts + ((long)l * Gmax + g) * TS_SET;
     const int k = t < nr ? srank[t] : -1;
     if (k >= 0) {
+      // the rows' original values, loads in batches of 8 before their stores
       if (l == 0) {
+        // the row's address, opaque here: column addresses are formed at
+        // the reads (hoisted out of the stage loop they would stay live)
+        float2 *Ai = A + i + r0 * lda;
+        asm volatile("" : "+v"(Ai));
 #pragma unroll
-        for (int q = 0; q < PW; q++)
-          if (q < w) st_coh((float2 *)(set + k * PW + q), A[i + (r0 + q) * lda]);
+        for (int q0 = 0; q0 < PW; q0 += 8) {
+          float2 o[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (q0 + q < w) o[q] = ld_coh(Ai + (q0 + q) * lda);
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (q0 + q < w) st_coh((float2 *)(set + k * PW + q0 + q), o[q]);
+        }
       } else {
         const unsigned long long *cs = prev + (g + (ssrc[t] / PW) * sp) * TS_SET;
         const int rr = ssrc[t] % PW;
 #pragma unroll
-        for (int q = 0; q < PW; q++)
-          if (q < w) st_coh((float2 *)(set + k * PW + q), ld_coh((const float2 *)(cs + rr * PW + q)));
+        for (int q0 = 0; q0 < PW; q0 += 8) {
+          float2 o[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (q0 + q < w) o[q] = ld_coh((const float2 *)(cs + rr * PW + q0 + q));
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (q0 + q < w) st_coh((float2 *)(set + k * PW + q0 + q), o[q]);
+        }
       }
       st_coh(set + TS_IDX + k, (unsigned long long)sidx[t]);
     }
@@ -715,20 +740,27 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   }
 
   if (g == 0) {
-    // publish the final block; ipiv, the net permutation and info
+    // publish the final block, and its upper triangle U11 in eight
+    // replicas (read by all workgroups at once); ipiv, the net
+    // permutation and info
     __syncthreads();
     for (int e = t; e < PW * PW; e += NR) {
       const int rr = e / PW, q = e % PW;
-      if (rr < w && q < w) st_coh((float2 *)(root + rr * PW + q), sF[rr][q]);
+      if (rr < w && q < w) {
+        st_coh((float2 *)(root + rr * PW + q), sF[rr][q]);
+        if (q >= rr)
+          for (int c = 0; c < 8; c++) st_coh((float2 *)(root + TS_UREP + c * TS_REP + rr * PW + q), sF[rr][q]);
+      }
     }
     if (t < 64) {
       // lanes hold (position, original row now there) for every touched
       // position: lanes 0..w-1 the diagonal block's, pivot rows from
       // below appended (lane indices are wave-uniform: lane reads)
       int row = (t < w) ? (int)(r0 + t) : -1, content = row;
+      const int Rl = (t < w) ? (int)sidx[spiv[t]] : -1;    // original row of pivot t
       int cnt = w;
       for (int j = 0; j < w; j++) {
-        const int R = (int)sidx[spiv[j]];
+        const int R = __builtin_amdgcn_readlane(Rl, j);
         const unsigned long long hit = __ballot(content == R);
         int ip;
         if (hit) ip = __ffsll(hit) - 1;
@@ -750,20 +782,22 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         if (zero >= 0 && *info == 0) *info = (int)(r0 + zero + 1);
       }
     }
-    // the inverse of the unit-lower w x w block, column t by substitution
-    if (S.linv && t < PW) {
+    // the inverse of the unit-lower w x w block, column c by substitution
+    // (wave 1, beside the permutation on wave 0)
+    if (S.linv && t >= 64 && t < 64 + PW) {
+      const int c = t - 64;
       float2 x[PW];
 #pragma unroll
       for (int rr = 0; rr < PW; rr++) {
         float2 val;
-        val.x = (rr == t) ? 1 : 0; val.y = 0;
-        if (rr < w && t < w) {
+        val.x = (rr == c) ? 1 : 0; val.y = 0;
+        if (rr < w && c < w) {
 #pragma unroll
           for (int k = 0; k < rr; k++) val = cfms(val, sF[rr][k], x[k]);
         }
         x[rr] = val;
       }
-      float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)t * PW;
+      float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)c * PW;
 #pragma unroll
       for (int rr = 0; rr < PW; rr++) li[rr] = x[rr];
     }
@@ -774,9 +808,14 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
 
   // every workgroup: its rows of the permuted panel
   ts_wait(root + TS_FIN, 0, 1, seq, err);
+  // workgroup 0 (the diagonal block's rows) reads the whole final
+  // block, the others U11 from replica g % 8
   for (int e = t; e < PW * PW; e += NR) {
     const int rr = e / PW, q = e % PW;
-    if (rr < w && q < w) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+    if (rr < w && q < w) {
+      if (g == 0) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+      else if (q >= rr) sF[rr][q] = ld_coh((const float2 *)(root + TS_UREP + (g & 7) * TS_REP + rr * PW + q));
+    }
   }
   // displaced rows of this workgroup: sfrom[row - lo] = original row
   sfrom[t] = -1;
@@ -804,9 +843,11 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
 #pragma unroll
       for (int q = 0; q < PW; q++) { v[q].x = 0; v[q].y = 0; }
       if (c == i) {
+        float2 *Ai = A + i + r0 * lda;
+        asm volatile("" : "+v"(Ai));
 #pragma unroll
         for (int q = 0; q < PW; q++)
-          if (q < w) v[q] = A[i + (r0 + q) * lda];
+          if (q < w) v[q] = ld_coh(Ai + q * lda);
       } else if (c >= r0 && c < r0 + w) {
         // a row displaced from the diagonal block (no other source exists;
         // anything else is reported as a failed panel)
@@ -1178,7 +1219,12 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
                       (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0};
       int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
-      hipError_t e = hipLaunchCooperativeKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream);
+      // the tournament panel has no grid barrier, only bounded waits along
+      // its merge tree: an ordinary launch keeps it on the stream's own
+      // hardware queue (a cooperative launch goes through a queue of its
+      // own, with a cross-queue handoff before and after every panel)
+      hipError_t e = tourn ? hipLaunchKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream)
+                           : hipLaunchCooperativeKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream);
       // algorithmic flops of the m x w panel: per column, one complex
       // reciprocal-multiply per row below it and a complex rank-1 update
       double pflops = 0;
@@ -1221,7 +1267,12 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
         trsm_block_inv(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c2 * lda, lda, n - c2, side);
       else
         trsm_block(A, lda, K, (int)jb, A + K + c2 * lda, lda, n - c2, side);
-      gemm_sub(n - c1, n - c2, jb, A + c1 + K * lda, lda, A + K + c2 * lda, lda, A + c1 + c2 * lda, lda, side);
+      // beside tournament panels: at most three trailing-GEMM workgroups
+      // per CU (44 KB of LDS each), so that a panel workgroup (168 VGPRs
+      // per wave, 12 KB) finds room on every CU instead of waiting for
+      // GEMM workgroups to retire
+      const unsigned pad = tourn ? 44u * 1024u - (unsigned)(4 * GBK * (GBM + 4) * sizeof(float)) : 0u;
+      gemm_sub(n - c1, n - c2, jb, A + c1 + K * lda, lda, A + K + c2 * lda, lda, A + c1 + c2 * lda, lda, side, pad);
       hipEventRecord(evR, side);
     }
   }
