@@ -1103,6 +1103,63 @@ static void trmm_strip(const T *Linv, int w, T *B, long ldb, long ncol, hipStrea
   prof_end(s, tok, 4.0 * (double)w * (double)w * (double)ncol);
 }
 
+// One panel's interchanges on the columns [c0, c1) outside the panel
+// [s0, s1) and, on the columns right of the panel, the product of the
+// panel block's unit-lower inverse with its w rows from r0 (what
+// laswp_plans and trmm_strip do, in one launch): one wave per column.
+// Lane e holds plan entry e (dst <- src, all sources read before any
+// write); entries 0..w-1 are the rows r0 .. r0 + w - 1, so after the
+// interchange lanes 0..w-1 hold the panel rows of the column.
+__device__ inline float readlane_f(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+__device__ inline double readlane_f(double v, int k)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFF), k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan, const T *Linv, int w, long r0)
+{
+  const int lane = threadIdx.x & 63;
+  const long ci = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long nleft = s0 - c0;
+  if (ci >= (c1 - c0) - (s1 - s0)) return;
+  const long c = ci < nleft ? c0 + ci : s1 + (ci - nleft);
+  const int cnt = plan[PW * 4];
+  T v;
+  v.x = 0; v.y = 0;
+  long dst = -1;
+  if (lane < cnt) {
+    dst = plan[lane];
+    v = A[plan[PW * 2 + lane] + c * lda];
+  }
+  if (c >= s1) {
+    T acc;
+    acc.x = 0; acc.y = 0;
+    for (int k = 0; k < w; k++) {
+      T vk;
+      vk.x = readlane_f(v.x, k);
+      vk.y = readlane_f(v.y, k);
+      if (lane < w) acc = cadd(acc, cmul(Linv[lane + k * PW], vk));
+    }
+    if (lane < w) v = acc;
+  }
+  if (lane < cnt) A[dst + c * lda] = v;
+}
+
+template <typename T>
+static void swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan, const T *Linv, int w,
+                      long r0, hipStream_t s)
+{
+  const long ncol = (c1 - c0) - (s1 - s0);
+  if (ncol <= 0) return;
+  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
+  hipLaunchKernelGGL(k_swap_trmm<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s, A, lda, c0, c1, s0, s1, plan,
+                     Linv, w, r0);
+  prof_end(s, tok, 4.0 * (double)w * (double)w * (double)(c1 - s1));
+}
+
 // trsm_block with the panels' block inverses: per 32-row step a product
 // with the step's inverse instead of substitution, then the GEMM update
 template <typename T>
@@ -1231,14 +1288,15 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       for (int j = 0; j < w; j++) pflops += (double)(m - j - 1) * (6.0 + 8.0 * (double)(w - j - 1));
       prof_end(stream, tok, pflops);
       if (e != hipSuccess) { rc = -3; break; }
-      // this panel's interchanges on the block's other columns
-      laswp_plans(A, lda, K, K + jb, j0, j0 + w, S.plan, 1, stream);
+      // this panel's interchanges on the block's other columns (fast: with
+      // the product by the panel block's inverse on the columns right of it)
       long nr = K + jb - (j0 + w);
+      if (fast)
+        swap_trmm(A, lda, K, K + jb, j0, j0 + w, S.plan, (const T *)S.linv + (j0 / PW) * PW * PW, w, j0, stream);
+      else
+        laswp_plans(A, lda, K, K + jb, j0, j0 + w, S.plan, 1, stream);
       if (nr > 0) {
-        if (fast)
-          trmm_strip((const T *)S.linv + (j0 / PW) * PW * PW, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
-        else
-          trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
+        if (!fast) trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
         gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
                  A + (j0 + w) + (j0 + w) * lda, lda, stream);
       }

@@ -21,8 +21,10 @@ const char *g_names[PK_COUNT] = {"gemm_c128", "gemm_c64", "panel_c128", "panel_c
 hipEvent_t take()
 {
   if (!g_pool.empty()) { hipEvent_t e = g_pool.back(); g_pool.pop_back(); return e; }
+  // timing only: no system-scope fence (cache write-back and invalidation)
+  // when the event is recorded, which would slow the work it brackets
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
+  hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 }  // namespace
