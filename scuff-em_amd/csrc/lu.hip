@@ -476,10 +476,11 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
 // waits per panel (two at most 64 workgroups) instead of one per column.
 // ---------------------------------------------------------------------
 #define TS_FAN 8
-#define TS_SET 1088                 // u64 words of one nominee set
-#define TS_IDX (PW * PW)            // set: rows [PW][PW] (original values), row indices [PW], count, flag
-#define TS_CNT (PW * PW + PW)
-#define TS_FLAG (PW * PW + PW + 8)
+#define TS_SET 1152                 // u64 words of one nominee set
+#define TS_IDX (PW * PW)            // set: rows [PW][PW] (original values, level 0 only), row indices [PW],
+#define TS_CNT (PW * PW + PW)       // count, flag, references [PW] (level-0 slot * PW + rank of each
+#define TS_FLAG (PW * PW + PW + 8)  // nominee's original values)
+#define TS_REF (PW * PW + PW + 16)
 #define TS_TOP (PW * PW)            // root area: final block [PW][PW], top rows [PW][PW], plan, flag
 #define TS_PLAN (2 * PW * PW)
 #define TS_NPLAN (2 * PW * PW + 2 * PW)
@@ -679,11 +680,13 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
           if (t >= sbase[q]) k = q;
         const int rr = t - sbase[k];
         const unsigned long long *cs = prev + (g + k * sp) * TS_SET;
+        const int ref = (int)ld_coh(cs + TS_REF + rr);
+        const unsigned long long *os = ts + (long)(ref / PW) * TS_SET + (ref % PW) * PW;   // level-0 copy
 #pragma unroll
         for (int q = 0; q < PW; q++)
-          if (q < w) r[q] = ld_coh((const float2 *)(cs + rr * PW + q));
+          if (q < w) r[q] = ld_coh((const float2 *)(os + q));
         sidx[t] = (long)ld_coh(cs + TS_IDX + rr);
-        ssrc[t] = k * PW + rr;
+        ssrc[t] = ref;
       }
     }
     const int np = ts_lu(r, nr, w, spiv, srank, wk, wi, cand, &zero);
@@ -700,35 +703,26 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     unsigned long long *set = ts + ((long)l * Gmax + g) * TS_SET;
     const int k = t < nr ? srank[t] : -1;
     if (k >= 0) {
-      // the rows' original values, loads in batches of 8 before their stores
       if (l == 0) {
-        // the row's address, opaque here: column addresses are formed at
-        // the reads (hoisted out of the stage loop they would stay live)
+        // the row's original values from A (unchanged until the last phase;
+        // plain loads, L2-resident): the address is opaque here so that the
+        // reads are not merged with the first ones (whose 32 hoisted
+        // addresses would stay live through the stages)
         float2 *Ai = A + i + r0 * lda;
         asm volatile("" : "+v"(Ai));
 #pragma unroll
-        for (int q0 = 0; q0 < PW; q0 += 8) {
-          float2 o[8];
+        for (int q0 = 0; q0 < PW; q0 += 16) {
+          float2 o[16];
 #pragma unroll
-          for (int q = 0; q < 8; q++)
-            if (q0 + q < w) o[q] = ld_coh(Ai + (q0 + q) * lda);
+          for (int q = 0; q < 16; q++)
+            if (q0 + q < w) o[q] = Ai[(q0 + q) * lda];
 #pragma unroll
-          for (int q = 0; q < 8; q++)
+          for (int q = 0; q < 16; q++)
             if (q0 + q < w) st_coh((float2 *)(set + k * PW + q0 + q), o[q]);
         }
+        st_coh(set + TS_REF + k, (unsigned long long)(g * PW + k));
       } else {
-        const unsigned long long *cs = prev + (g + (ssrc[t] / PW) * sp) * TS_SET;
-        const int rr = ssrc[t] % PW;
-#pragma unroll
-        for (int q0 = 0; q0 < PW; q0 += 8) {
-          float2 o[8];
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            if (q0 + q < w) o[q] = ld_coh((const float2 *)(cs + rr * PW + q0 + q));
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            if (q0 + q < w) st_coh((float2 *)(set + k * PW + q0 + q), o[q]);
-        }
+        st_coh(set + TS_REF + k, (unsigned long long)ssrc[t]);   // the level-0 copy stays where it is
       }
       st_coh(set + TS_IDX + k, (unsigned long long)sidx[t]);
     }
@@ -782,25 +776,6 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         if (zero >= 0 && *info == 0) *info = (int)(r0 + zero + 1);
       }
     }
-    // the inverse of the unit-lower w x w block, column c by substitution
-    // (wave 1, beside the permutation on wave 0)
-    if (S.linv && t >= 64 && t < 64 + PW) {
-      const int c = t - 64;
-      float2 x[PW];
-#pragma unroll
-      for (int rr = 0; rr < PW; rr++) {
-        float2 val;
-        val.x = (rr == c) ? 1 : 0; val.y = 0;
-        if (rr < w && c < w) {
-#pragma unroll
-          for (int k = 0; k < rr; k++) val = cfms(val, sF[rr][k], x[k]);
-        }
-        x[rr] = val;
-      }
-      float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)c * PW;
-#pragma unroll
-      for (int rr = 0; rr < PW; rr++) li[rr] = x[rr];
-    }
     drain();
     __syncthreads();
     if (t == 0) st_coh(root + TS_FIN, seq);
@@ -808,12 +783,13 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
 
   // every workgroup: its rows of the permuted panel
   ts_wait(root + TS_FIN, 0, 1, seq, err);
-  // workgroup 0 (the diagonal block's rows) reads the whole final
-  // block, the others U11 from replica g % 8
+  // workgroup 0 (the diagonal block's rows) and the last one (which
+  // inverts L11) read the whole final block, the others U11 from replica
+  // g % 8
   for (int e = t; e < PW * PW; e += NR) {
     const int rr = e / PW, q = e % PW;
     if (rr < w && q < w) {
-      if (g == 0) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+      if (g == 0 || g == G - 1) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
       else if (q >= rr) sF[rr][q] = ld_coh((const float2 *)(root + TS_UREP + (g & 7) * TS_REP + rr * PW + q));
     }
   }
@@ -832,6 +808,26 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     sRd[t] = (d.x != 0 || d.y != 0) ? crecip(d) : d;
   }
   __syncthreads();
+  // the inverse of the unit-lower w x w block, column c by substitution, on
+  // the last workgroup (usually the one with the fewest rows), off the
+  // path of the others
+  if (S.linv && g == G - 1 && t >= NR - PW) {
+    const int c = t - (NR - PW);
+    float2 x[PW];
+#pragma unroll
+    for (int rr = 0; rr < PW; rr++) {
+      float2 val;
+      val.x = (rr == c) ? 1 : 0; val.y = 0;
+      if (rr < w && c < w) {
+#pragma unroll
+        for (int k = 0; k < rr; k++) val = cfms(val, sF[rr][k], x[k]);
+      }
+      x[rr] = val;
+    }
+    float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)c * PW;
+#pragma unroll
+    for (int rr = 0; rr < PW; rr++) li[rr] = x[rr];
+  }
   if (t < nrow) {
     if (i < r0 + w) {
 #pragma unroll
@@ -843,11 +839,12 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
 #pragma unroll
       for (int q = 0; q < PW; q++) { v[q].x = 0; v[q].y = 0; }
       if (c == i) {
+        // plain loads of the unchanged row (opaque address, as above)
         float2 *Ai = A + i + r0 * lda;
         asm volatile("" : "+v"(Ai));
 #pragma unroll
         for (int q = 0; q < PW; q++)
-          if (q < w) v[q] = ld_coh(Ai + q * lda);
+          if (q < w) v[q] = Ai[q * lda];
       } else if (c >= r0 && c < r0 + w) {
         // a row displaced from the diagonal block (no other source exists;
         // anything else is reported as a failed panel)
