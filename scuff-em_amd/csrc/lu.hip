@@ -476,18 +476,22 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
 // waits per panel (two at most 64 workgroups) instead of one per column.
 // ---------------------------------------------------------------------
 #define TS_FAN 8
-#define TS_SET 1152                 // u64 words of one nominee set
-#define TS_IDX (PW * PW)            // set: rows [PW][PW] (original values, level 0 only), row indices [PW],
-#define TS_CNT (PW * PW + PW)       // count, flag, references [PW] (level-0 slot * PW + rank of each
-#define TS_FLAG (PW * PW + PW + 8)  // nominee's original values)
-#define TS_REF (PW * PW + PW + 16)
-#define TS_TOP (PW * PW)            // root area: final block [PW][PW], top rows [PW][PW], plan, flag
+#define TS_SET 128                  // u64 words of one nominee set: row indices [PW], flag,
+#define TS_IDX 0                    // references [PW] (row of each nominee's original values in
+#define TS_FLAG (PW + 8)            // the copy area: workgroup * 256 + row)
+#define TS_REF (PW + 16)
+#define TS_TOP (PW * PW)            // root area: final block [PW][PW], (spare) [PW][PW], plan, flag
 #define TS_PLAN (2 * PW * PW)
 #define TS_NPLAN (2 * PW * PW + 2 * PW)
 #define TS_FIN (2 * PW * PW + 2 * PW + 8)
 #define TS_UREP 2176                // first replica (workgroup g reads replica g % 8)
 #define TS_REP 1056                 // u64 words between replicas of the final block
 #define TS_ROOT (2176 + 8 * TS_REP) // root area + 8 replicas of the final block
+
+// the tournament's scratch: sets of all levels [levels + 1][Gmax], the
+// copy area [Gmax][256][PW] (each workgroup's rows as loaded), the root area
+__host__ __device__ inline long ts_copy_off(int levels, long Gmax) { return (long)(levels + 1) * Gmax * TS_SET; }
+__host__ __device__ inline long ts_root_off(int levels, long Gmax) { return ts_copy_off(levels, Gmax) + Gmax * 256 * PW; }
 
 __host__ __device__ inline int ts_levels(long G)
 {
@@ -627,15 +631,44 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   // the panel is the critical path and the trailing GEMM shares its CUs:
   // its waves win instruction issue on their SIMDs
   __builtin_amdgcn_s_setprio(3);
-  const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x;
+  // workgroups 0 .. G - 1 hold the panel's rows; workgroup G only inverts
+  // the final block's unit-lower triangle, beside the others' last phase
+  const int t = threadIdx.x, G = gridDim.x - 1, g = blockIdx.x;
   const long lo = r0 + (long)g * NR;
-  const int nrow = (int)min((long)NR, r0 + m - lo);
+  const int nrow = (int)max(0L, min((long)NR, r0 + m - lo));
   const long i = lo + t;
   const unsigned long long seq = seq0;
   unsigned *err = S.bar + 64;
   unsigned long long *ts = S.ts;
-  unsigned long long *root = ts + (long)(ts_levels(Gmax) + 1) * Gmax * TS_SET;
+  unsigned long long *copy = ts + ts_copy_off(ts_levels(Gmax), Gmax);
+  unsigned long long *root = ts + ts_root_off(ts_levels(Gmax), Gmax);
   const int nl = ts_levels(G);
+  if (g == G) {
+    ts_wait(root + TS_FIN, 0, 1, seq, err);
+    for (int e = t; e < PW * PW; e += NR) {
+      const int rr = e / PW, q = e % PW;
+      if (rr < w && q < w) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+    }
+    __syncthreads();
+    // the inverse of the unit-lower w x w block, column t by substitution
+    if (S.linv && t < PW) {
+      float2 x[PW];
+#pragma unroll
+      for (int rr = 0; rr < PW; rr++) {
+        float2 val;
+        val.x = (rr == t) ? 1 : 0; val.y = 0;
+        if (rr < w && t < w) {
+#pragma unroll
+          for (int k = 0; k < rr; k++) val = cfms(val, sF[rr][k], x[k]);
+        }
+        x[rr] = val;
+      }
+      float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)t * PW;
+#pragma unroll
+      for (int rr = 0; rr < PW; rr++) li[rr] = x[rr];
+    }
+    return;
+  }
 
   // own rows into registers (A keeps the original values until the last
   // phase, which re-reads them); workgroup 0 also publishes rows r0 .. r0 +
@@ -646,10 +679,13 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     r[q].x = 0; r[q].y = 0;
     if (t < nrow && q < w) r[q] = A[i + (r0 + q) * lda];
   }
-  if (g == 0 && t < w) {
+  // every row's original values to the copy area (read by the merges and,
+  // for the rows the interchanges displace from the diagonal block, by the
+  // last phase); drained before this workgroup's first flag
+  if (t < nrow) {
 #pragma unroll
     for (int q = 0; q < PW; q++)
-      if (q < w) st_coh((float2 *)(root + TS_TOP + t * PW + q), r[q]);
+      if (q < w) st_coh((float2 *)(copy + ((long)g * NR + t) * PW + q), r[q]);
   }
   sidx[t] = lo + t;
   // stage 0: this workgroup's rows; stage l >= 1: workgroups g % TS_FAN^l
@@ -681,7 +717,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         const int rr = t - sbase[k];
         const unsigned long long *cs = prev + (g + k * sp) * TS_SET;
         const int ref = (int)ld_coh(cs + TS_REF + rr);
-        const unsigned long long *os = ts + (long)(ref / PW) * TS_SET + (ref % PW) * PW;   // level-0 copy
+        const unsigned long long *os = copy + (long)ref * PW;   // the row's original values
 #pragma unroll
         for (int q = 0; q < PW; q++)
           if (q < w) r[q] = ld_coh((const float2 *)(os + q));
@@ -703,27 +739,8 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     unsigned long long *set = ts + ((long)l * Gmax + g) * TS_SET;
     const int k = t < nr ? srank[t] : -1;
     if (k >= 0) {
-      if (l == 0) {
-        // the row's original values from A (unchanged until the last phase;
-        // plain loads, L2-resident): the address is opaque here so that the
-        // reads are not merged with the first ones (whose 32 hoisted
-        // addresses would stay live through the stages)
-        float2 *Ai = A + i + r0 * lda;
-        asm volatile("" : "+v"(Ai));
-#pragma unroll
-        for (int q0 = 0; q0 < PW; q0 += 16) {
-          float2 o[16];
-#pragma unroll
-          for (int q = 0; q < 16; q++)
-            if (q0 + q < w) o[q] = Ai[(q0 + q) * lda];
-#pragma unroll
-          for (int q = 0; q < 16; q++)
-            if (q0 + q < w) st_coh((float2 *)(set + k * PW + q0 + q), o[q]);
-        }
-        st_coh(set + TS_REF + k, (unsigned long long)(g * PW + k));
-      } else {
-        st_coh(set + TS_REF + k, (unsigned long long)ssrc[t]);   // the level-0 copy stays where it is
-      }
+      // references to the original values in the copy area
+      st_coh(set + TS_REF + k, (unsigned long long)(l == 0 ? g * NR + t : ssrc[t]));
       st_coh(set + TS_IDX + k, (unsigned long long)sidx[t]);
     }
     (void)np;
@@ -783,13 +800,12 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
 
   // every workgroup: its rows of the permuted panel
   ts_wait(root + TS_FIN, 0, 1, seq, err);
-  // workgroup 0 (the diagonal block's rows) and the last one (which
-  // inverts L11) read the whole final block, the others U11 from replica
-  // g % 8
+  // workgroup 0 (the diagonal block's rows) reads the whole final
+  // block, the others U11 from replica g % 8
   for (int e = t; e < PW * PW; e += NR) {
     const int rr = e / PW, q = e % PW;
     if (rr < w && q < w) {
-      if (g == 0 || g == G - 1) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+      if (g == 0) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
       else if (q >= rr) sF[rr][q] = ld_coh((const float2 *)(root + TS_UREP + (g & 7) * TS_REP + rr * PW + q));
     }
   }
@@ -808,26 +824,6 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     sRd[t] = (d.x != 0 || d.y != 0) ? crecip(d) : d;
   }
   __syncthreads();
-  // the inverse of the unit-lower w x w block, column c by substitution, on
-  // the last workgroup (usually the one with the fewest rows), off the
-  // path of the others
-  if (S.linv && g == G - 1 && t >= NR - PW) {
-    const int c = t - (NR - PW);
-    float2 x[PW];
-#pragma unroll
-    for (int rr = 0; rr < PW; rr++) {
-      float2 val;
-      val.x = (rr == c) ? 1 : 0; val.y = 0;
-      if (rr < w && c < w) {
-#pragma unroll
-        for (int k = 0; k < rr; k++) val = cfms(val, sF[rr][k], x[k]);
-      }
-      x[rr] = val;
-    }
-    float2 *li = (float2 *)S.linv + (r0 / PW) * PW * PW + (long)c * PW;
-#pragma unroll
-    for (int rr = 0; rr < PW; rr++) li[rr] = x[rr];
-  }
   if (t < nrow) {
     if (i < r0 + w) {
 #pragma unroll
@@ -850,7 +846,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         // anything else is reported as a failed panel)
 #pragma unroll
         for (int q = 0; q < PW; q++)
-          if (q < w) v[q] = ld_coh((const float2 *)(root + TS_TOP + (c - r0) * PW + q));
+          if (q < w) v[q] = ld_coh((const float2 *)(copy + (c - r0) * PW + q));
       } else {
         __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
@@ -1231,7 +1227,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   size_t off_rc = off_cr + 256 * ((sizeof(T) * 2 * Gmax * PW + 255) / 256);
   size_t off_li = off_rc + 256 * ((sizeof(T) * 2 * PW + 255) / 256);
   size_t off_ts = off_li + 256 * (((fast ? sizeof(T) * PW * PW * npanels : 0) + 255) / 256);
-  const size_t ts_words = tourn ? (size_t)(ts_levels(Gmax) + 1) * Gmax * TS_SET + TS_ROOT : 0;
+  const size_t ts_words = tourn ? (size_t)ts_root_off(ts_levels(Gmax), Gmax) + TS_ROOT : 0;
   size_t total = off_ts + sizeof(unsigned long long) * ts_words;
   char *wsb = (char *)ws_get(WS_PANEL, total);
   if (!wsb) return -2;
@@ -1277,7 +1273,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       // its merge tree: an ordinary launch keeps it on the stream's own
       // hardware queue (a cooperative launch goes through a queue of its
       // own, with a cross-queue handoff before and after every panel)
-      hipError_t e = tourn ? hipLaunchKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream)
+      hipError_t e = tourn ? hipLaunchKernel(panel_fn, dim3((unsigned)G + 1), dim3(NR), args, 0, stream)
                            : hipLaunchCooperativeKernel(panel_fn, dim3((unsigned)G), dim3(NR), args, 0, stream);
       // algorithmic flops of the m x w panel: per column, one complex
       // reciprocal-multiply per row below it and a complex rank-1 update
