@@ -500,6 +500,20 @@ __host__ __device__ inline int ts_levels(long G)
   return l;
 }
 
+// 1/a for complex64 with hardware reciprocals (about 1 ulp each; the
+// tournament panel's pivots), scaled by max(|re|, |im|) against overflow
+// and underflow of |a|^2
+__device__ inline float2 crecip_fast(float2 a)
+{
+  const float s = fmaxf(fabsf(a.x), fabsf(a.y)), is = __builtin_amdgcn_rcpf(s);
+  const float xs = a.x * is, ys = a.y * is;
+  const float d = __builtin_amdgcn_rcpf(fmaf(xs, xs, ys * ys)) * is;
+  float2 r;
+  r.x = xs * d;
+  r.y = -ys * d;
+  return r;
+}
+
 // r - l c with fused multiply-adds (complex64)
 __device__ inline float2 cfms(float2 r, float2 l, float2 c)
 {
@@ -572,7 +586,7 @@ __device__ __attribute__((always_inline)) inline int ts_lu(float2 (&r)[PW], int 
       if (b0 == 1u && z < 0) z = j;
       if (!used) {
         float2 l = r[j];
-        if (b0 != 1u) { l = cmul(l, crecip(cand[j & 1][pw][j])); r[j] = l; }
+        if (b0 != 1u) { l = cmul(l, crecip_fast(cand[j & 1][pw][j])); r[j] = l; }
 #pragma unroll
         for (int q = j + 1; q < PW; q++) r[q] = cfms(r[q], l, cand[j & 1][pw][q]);
       }
