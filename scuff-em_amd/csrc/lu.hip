@@ -1182,6 +1182,83 @@ static void trsm_block_inv(const T *A, long lda, long r0, int nb, const T *linv,
   }
 }
 
+// B[jb x ncol] <- L^-1 B for the unit-lower jb x jb diagonal block of a
+// 256-column block (jb <= 256) in one launch, with the panels' inverses:
+// per 32-row panel i, B_i <- Linv_i B_i, then the rows below minus
+// L(below, panel i) B_i (what trsm_block_inv does in 15 launches for the
+// next block's columns).  One wave per column; lane l holds rows l, l + 64,
+// l + 128, l + 192; the panel rows are read across lanes by lane reads.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol)
+{
+  const int lane = threadIdx.x & 63;
+  const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncol) return;
+  T b[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    b[s].x = 0; b[s].y = 0;
+    if (lane + 64 * s < jb) b[s] = B[lane + 64 * s + c * ldb];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int i0 = PW * i;
+    if (i0 < jb) {
+      const int w = min(PW, jb - i0), si = i / 2, l0 = PW * (i & 1);
+      // x = Linv_i b_i on the lanes holding the panel's rows
+      const T *Li = linv + (long)i * PW * PW;
+      const bool mine = lane >= l0 && lane < l0 + w;
+      T acc;
+      acc.x = 0; acc.y = 0;
+#pragma unroll
+      for (int k = 0; k < PW; k++) {
+        if (k < w) {
+          T bk;
+          bk.x = readlane_f(b[si].x, l0 + k);
+          bk.y = readlane_f(b[si].y, l0 + k);
+          if (mine) acc = cadd(acc, cmul(Li[(lane - l0) + k * PW], bk));
+        }
+      }
+      if (mine) b[si] = acc;
+      // rows below the panel: minus L(rows, panel) x
+      if (i0 + w < jb) {
+        T x[PW];
+#pragma unroll
+        for (int k = 0; k < PW; k++) {
+          x[k].x = readlane_f(b[si].x, l0 + (k < w ? k : 0));
+          x[k].y = readlane_f(b[si].y, l0 + (k < w ? k : 0));
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const int row = lane + 64 * s;
+          if (row >= i0 + w && row < jb) {
+            const T *Lr = A + (r0 + row) + (r0 + i0) * lda;
+            T a = b[s];
+#pragma unroll
+            for (int k = 0; k < PW; k++)
+              if (k < w) a = csub(a, cmul(Lr[k * lda], x[k]));
+            b[s] = a;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+    if (lane + 64 * s < jb) B[lane + 64 * s + c * ldb] = b[s];
+}
+
+template <typename T>
+static void trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol, hipStream_t s)
+{
+  if (ncol <= 0 || jb <= 0) return;
+  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
+  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s, A, lda, r0, jb, linv, B, ldb,
+                     ncol);
+  prof_end(s, tok, 4.0 * (double)jb * (double)jb * (double)ncol);
+}
+
 // second stream of the lookahead (default priority, non-blocking), one per
 // device, created on first use
 // second stream of the lookahead (default priority, non-blocking), one per
@@ -1319,7 +1396,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       hipStreamWaitEvent(stream, evR, 0);
       laswp_plans(A, lda, c1, c2, 0, 0, plans + p0 * PLAN_LEN, bplans, stream);
       if (fast)
-        trsm_block_inv(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c1 * lda, lda, nnext, stream);
+        trsm_blk(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c1 * lda, lda, nnext, stream);
       else
         trsm_block(A, lda, K, (int)jb, A + K + c1 * lda, lda, nnext, stream);
       gemm_sub(n - c1, nnext, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, stream);
