@@ -1209,6 +1209,11 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
       // x = Linv_i b_i on the lanes holding the panel's rows
       const T *Li = linv + (long)i * PW * PW;
       const bool mine = lane >= l0 && lane < l0 + w;
+      // this lane's row of Linv_i, all loads in flight before the products
+      T lr[PW];
+      const int q = mine ? lane - l0 : 0;
+#pragma unroll
+      for (int k = 0; k < PW; k++) lr[k] = Li[q + k * PW];
       T acc;
       acc.x = 0; acc.y = 0;
 #pragma unroll
@@ -1217,7 +1222,7 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
           T bk;
           bk.x = readlane_f(b[si].x, l0 + k);
           bk.y = readlane_f(b[si].y, l0 + k);
-          if (mine) acc = cadd(acc, cmul(Li[(lane - l0) + k * PW], bk));
+          acc = cadd(acc, cmul(lr[k], bk));
         }
       }
       if (mine) b[si] = acc;
@@ -1234,10 +1239,13 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
           const int row = lane + 64 * s;
           if (row >= i0 + w && row < jb) {
             const T *Lr = A + (r0 + row) + (r0 + i0) * lda;
+            T lv[PW];
+#pragma unroll
+            for (int k = 0; k < PW; k++) lv[k] = Lr[(k < w ? k : 0) * lda];
             T a = b[s];
 #pragma unroll
             for (int k = 0; k < PW; k++)
-              if (k < w) a = csub(a, cmul(Lr[k * lda], x[k]));
+              if (k < w) a = csub(a, cmul(lv[k], x[k]));
             b[s] = a;
           }
         }
