@@ -1142,13 +1142,21 @@ k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan,
     v = A[plan[PW * 2 + lane] + c * lda];
   }
   if (c >= s1) {
+    // this lane's row of Linv, all loads in flight before the products
+    T lr[PW];
+    const int q = lane < w ? lane : 0;
+#pragma unroll
+    for (int k = 0; k < PW; k++) lr[k] = Linv[q + k * PW];
     T acc;
     acc.x = 0; acc.y = 0;
-    for (int k = 0; k < w; k++) {
-      T vk;
-      vk.x = readlane_f(v.x, k);
-      vk.y = readlane_f(v.y, k);
-      if (lane < w) acc = cadd(acc, cmul(Linv[lane + k * PW], vk));
+#pragma unroll
+    for (int k = 0; k < PW; k++) {
+      if (k < w) {
+        T vk;
+        vk.x = readlane_f(v.x, k);
+        vk.y = readlane_f(v.y, k);
+        acc = cadd(acc, cmul(lr[k], vk));
+      }
     }
     if (lane < w) v = acc;
   }
