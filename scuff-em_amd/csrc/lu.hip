@@ -1309,11 +1309,10 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   constexpr int NR = PanelRows<T>::n;
   const long Gmax = (n + NR - 1) / NR;
   // the mixed solver's complex64 factorization pivots by tournament
-  // (SCUFFX_MIXED_PIVOTING=partial: row by row, as the complex128 path)
-  static const bool partial_env = [] {
-    const char *e = getenv("SCUFFX_MIXED_PIVOTING");
-    return e && !strcmp(e, "partial");
-  }();
+  // (SCUFFX_MIXED_PIVOTING=partial, read at every factorization: row by
+  // row, as the complex128 path)
+  const char *piv_env = getenv("SCUFFX_MIXED_PIVOTING");
+  const bool partial_env = piv_env && !strcmp(piv_env, "partial");
   const bool tourn = fast && std::is_same<T, float2>::value && !partial_env;
   const void *panel_fn = tourn ? (const void *)k_panel_ts : (const void *)k_panel<T>;
   int dev = 0, ncu = 0, perCU = 0;

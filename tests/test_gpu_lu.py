@@ -246,3 +246,41 @@ def test_lu32_mixed_factors(sx, O, n):
         agree = np.mean(piv == ipt)
         print(f"  pivots agreeing with the restatement: {agree:.3f}", flush=True)
         assert agree > 0.95
+
+
+def test_lu32_mixed_partial_pivoting_switch(sx, O, monkeypatch):
+    """SCUFFX_MIXED_PIVOTING=partial (read at every factorization): the
+    mixed solver's complex64 factorization pivots row by row (k_panel)
+    through the same block-row kernels; P A = L U to complex64 accuracy,
+    |L| <= sqrt 2, pivots equal the oracle's partial pivoting on the rounded
+    matrix except near ties, and a mixed solve refines to 1e-12."""
+    import torch
+    n = 700
+    print("mixed factorization with partial pivoting", flush=True)
+    monkeypatch.setenv("SCUFFX_MIXED_PIVOTING", "partial")
+    rng = np.random.default_rng(SEED + 41)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    M = sx.HMatrix.from_numpy(A)
+    LU32 = torch.zeros((n, n), dtype=torch.complex64, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    rc = sx.lib().scuffx_lu32_from_c128(sx._ptr(M.t), n, n, sx._ptr(LU32), n, sx._ptr(ipiv))
+    assert rc == 0, (rc, sx.lib().scuffx_last_error())
+    F = LU32.cpu().numpy().T.astype(np.complex128)
+    piv = ipiv.cpu().numpy()
+    L = np.tril(F, -1) + np.eye(n)
+    U = np.triu(F)
+    perm = np.arange(n)
+    for i in range(n):
+        p = piv[i] - 1
+        perm[[i, p]] = perm[[p, i]]
+    A32 = A.astype(np.complex64).astype(np.complex128)
+    berr = np.linalg.norm(A32[perm] - L @ U) / np.linalg.norm(A32)
+    assert berr <= 2e-5 * np.sqrt(n / 40)
+    assert np.abs(L).max() <= np.sqrt(2.0) + 1e-5
+    _, ipo, _ = O.zgetrf(A32)
+    assert np.mean(piv == ipo) > 0.95
+    x_true = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    B = A + n * 0.2 * np.eye(n)
+    x, _, _ = sx.solve_mixed(sx.HMatrix.from_numpy(B), sx.HVector(n, data=torch.from_numpy(B @ x_true).cuda()),
+                             tol=1e-14, maxiter=20)
+    assert np.linalg.norm(x.numpy() - x_true) <= 1e-12 * np.linalg.norm(x_true)
