@@ -95,9 +95,9 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
   const long row0 = (long)blockIdx.x * GBM, col0 = (long)blockIdx.y * GBN;
   V4 accr[2][2], acci[2][2];
   for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); }
-  for (long k0 = 0; k0 < k; k0 += GBK) {
-    // all eight global loads in flight before any LDS store
-    T v[4], u[4];
+  // tiles of chunk k0 into registers: all eight global loads in flight
+  // before any LDS store
+  auto load = [&](long k0, T (&v)[4], T (&u)[4]) {
     if (row0 + GBM <= m && col0 + GBN <= n && k0 + GBK <= k) {
 #pragma unroll
       for (int s = 0; s < 4; s++) {
@@ -115,6 +115,15 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
         if (gkb < k && gj < n) u[s] = B[gkb + gj * ldb]; else { u[s].x = 0; u[s].y = 0; }
       }
     }
+  };
+  // complex64: the next chunk's loads are issued before this chunk's
+  // products (16 more registers, still four waves per SIMD); complex128
+  // loads each chunk in turn (its registers are the limit)
+  constexpr bool PF = std::is_same<T, float2>::value;
+  T v[4], u[4];
+  if (PF && k > 0) load(0, v, u);
+  for (long k0 = 0; k0 < k; k0 += GBK) {
+    if (!PF) load(k0, v, u);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       int i = tid & 63, kk = (tid >> 6) + 4 * s;
@@ -123,6 +132,7 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       sBr[kb][j] = u[s].x; sBi[kb][j] = u[s].y;
     }
     __syncthreads();
+    if (PF && k0 + GBK < k) load(k0 + GBK, v, u);
     for (int kk0 = 0; kk0 < GBK; kk0 += 4) {
       int kk = kk0 + (lane >> 4);
       R ar[2], ai[2], br[2], bi[2];
