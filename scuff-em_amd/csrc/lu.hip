@@ -116,10 +116,9 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       }
     }
   };
-  // complex64: the next chunk's loads are issued before this chunk's
-  // products (16 more registers, still four waves per SIMD); complex128
-  // loads each chunk in turn (its registers are the limit)
-  constexpr bool PF = std::is_same<T, float2>::value;
+  // the next chunk's loads are issued before this chunk's products
+  // (complex64: 98 registers, four waves per SIMD; complex128: two)
+  constexpr bool PF = true;
   T v[4], u[4];
   if (PF && k > 0) load(0, v, u);
   for (long k0 = 0; k0 < k; k0 += GBK) {
