@@ -36,12 +36,28 @@ extern "C" {
 #define SCUFFX_ERR_NUMERIC     -4
 #define SCUFFX_ERR_UNSUPPORTED -5
 #define SCUFFX_ERR_IO          -6
+/* positive status of the refinements: the result is the best iterate found,
+   but its residual is above the requested tolerance */
+#define SCUFFX_NOT_CONVERGED    1
+
+/* element types (the `prec` arguments): complex128 (LHM_COMPLEX), complex64,
+   and real double (LHM_REAL: the matrices of imaginary frequencies,
+   AllocateBEMMatrix(PureImagFreq=true), AssembleBEMMatrix.cc:778-783) */
+#define SCUFFX_C128 64
+#define SCUFFX_C64  32
+#define SCUFFX_F64  8
 
 typedef struct scuffx_geom scuffx_geom;
 
 /* message of the last failed call on this thread */
 const char *scuffx_last_error(void);
 int scuffx_version(void);
+/* releases every device buffer the library holds (workspaces, the device
+   data of live geometries) while the HIP runtime is still up; afterwards
+   the library makes no HIP call (destructors skip theirs) and compute calls
+   fail.  Meant for process exit (the Python module registers it with
+   atexit); no counterpart in the reference. */
+int scuffx_shutdown(void);
 /* HIP stream (hipStream_t) used by subsequent calls on this thread */
 int scuffx_set_stream(void *hip_stream);
 
@@ -173,6 +189,25 @@ int scuffx_clear_cache(void);
 /* ---- RWGGeometry::AssembleBEMMatrix(Omega, M) (AssembleBEMMatrix.cc:684) ----
    fills all N x N entries of the complex128 matrix at M_dev */
 int scuffx_assemble(scuffx_geom *G, double omega_re, double omega_im, void *M_dev, long ldm);
+/* ---- AllocateBEMMatrix(PureImagFreq=true, Packed) + AssembleBEMMatrix
+        (AssembleBEMMatrix.cc:778-783, HMatrix.cc:419-435) ----
+   the real matrix of the reference (LHM_REAL): the real part of every entry,
+   as AddEntry(cdouble) on a real HMatrix keeps; at an imaginary frequency of
+   a compact geometry the imaginary parts vanish.  packed = 0: column-major
+   M[r + c*ldm]; packed = 1: LHM_SYMMETRIC packed upper storage
+   M[r + c*(c+1)/2] for r <= c (N(N+1)/2 doubles, ldm unused). */
+int scuffx_assemble_real(scuffx_geom *G, double omega_re, double omega_im, double *M_dev, long ldm, int packed);
+/* the real counterpart of scuffx_assemble_block (the T, U and dU blocks of
+   the Casimir caller, CreateSC3Data.cc:100-157): real parts of the block and
+   of its derivatives; packed = 1 (diagonal blocks without derivatives only)
+   stores the block in packed upper storage, ldb unused */
+int scuffx_assemble_block_real(scuffx_geom *G, int nsa, int nsb, double omega_re, double omega_im, double *B_dev,
+                               long ldb, int packed, double *GradB_dev, long ldg, long gstride, int ntorque,
+                               const double *gamma, double *dBdT_dev);
+/* full symmetric n x n matrix A (column-major, lda) from packed upper
+   storage P (HMatrix LHM_SYMMETRIC layout, HMatrix.cc:330-333); prec 64, 32, 8 */
+int scuffx_unpack_symmetric(const void *P_dev, long n, void *A_dev, long lda, int prec);
+
 /* ---- RWGGeometry::AssembleBEMMatrixBlock(nsa, nsb, Omega, 0, M, GradM)
         (libscuff.h:500-505, AssembleBEMMatrix.cc:349-391) ----
    block of surfaces nsa (rows) and nsb (columns): B (NBF_a x NBF_b,
@@ -193,12 +228,21 @@ int scuffx_assemble_block(scuffx_geom *G, int nsa, int nsb, double omega_re, dou
 int scuffx_assemble_stats(const scuffx_geom *G, double *ms6, long *counts4);
 
 /* ---- HMatrix::LUFactorize (LBWrappers.cc:151-170) ----
-   prec 64: complex128 (zgetrf), prec 32: complex64 (cgetrf).
-   ipiv_dev: int[n], 1-based row interchanges.  Returns LAPACK info. */
+   prec 64: complex128 (zgetrf), prec 32: complex64 (cgetrf), prec 8: real
+   double (dgetrf; pivots by |a|, the first maximum).  ipiv_dev: int[n],
+   1-based row interchanges.  Returns LAPACK info.  (Packed symmetric
+   matrices are factored by the caller as a full copy, scuffx_unpack_symmetric,
+   with partial pivoting -- not with dsptrf/zsptrf's Bunch-Kaufman pivoting.) */
 int scuffx_lu_factor(void *A_dev, long n, long lda, int *ipiv_dev, int prec);
-/* ---- HMatrix::LUSolve (LBWrappers.cc:175-245); trans 'N', 'T' or 'C' ---- */
+/* ---- HMatrix::LUSolve (LBWrappers.cc:175-245); trans 'N', 'T' or 'C';
+        prec 64, 32 or 8 ---- */
 int scuffx_lu_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev, char trans,
                     long nrhs, void *B_dev, long ldb, int prec);
+
+/* ---- HMatrix::LUInvert (LBWrappers.cc:251-335): A <- A^{-1} from the
+        factors of scuffx_lu_factor (zgetri / dgetri); prec 64, 32 or 8.
+   Returns the first zero diagonal entry of U (1-based; A unchanged) or 0. */
+int scuffx_lu_invert(void *A_dev, long n, long lda, const int *ipiv_dev, int prec);
 
 /* ---- quantities the Casimir caller reads from the factors
         (applications/scuff-cas3D/CasimirIntegrand.cc:51-146) ----
@@ -215,17 +259,36 @@ int scuffx_trace_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev
 /* ---- mixed precision: complex64 LU of a complex128 matrix plus
         complex128 iterative refinement (no counterpart in the reference;
         the plain path above is the drop-in) ---- */
-/* LU32 = complex64 copy of M, factored in place with partial pivoting
-   (ipiv 1-based, LAPACK layout); the block-row triangular solves use the
+/* LU32 = complex64 copy of M, factored in place: P M = L U with L unit
+   lower, ipiv 1-based row interchanges in LAPACK layout.  The pivots come
+   from tournament pivoting (communication-avoiding LU: each 32-column panel
+   picks its 32 pivot rows by a tree of partial-pivoting eliminations over
+   256-row groups), which differs from row-by-row partial pivoting; the
+   environment variable SCUFFX_MIXED_PIVOTING=partial selects row-by-row
+   partial pivoting instead.  The block-row triangular solves use the
    explicit inverses of the 32x32 unit-lower panel blocks, so the factors
    carry slightly larger rounding than scuffx_lu_factor(prec=32).  Returns
    the first zero pivot or 0. */
 int scuffx_lu32_from_c128(const void *M_dev, long n, long ldm, void *LU32_dev, long ld32,
                           int *ipiv_dev);
-/* x = M^{-1} b refined until |b - M x|_inf <= tol |b|_inf or maxiter steps */
+/* x = M^{-1} b (complex128) from the complex64 factors: x0 = LU32^{-1} b,
+   then x += LU32^{-1}(b - M x) with the residual in complex128, at most
+   maxiter corrections, stopping when |b - M x|_inf <= tol |b|_inf or when a
+   correction does not lower the residual (that correction is undone).  If
+   the tolerance is still not met, GMRES-IR (below) continues from the best
+   iterate.  Returns 0 when the tolerance is met, SCUFFX_NOT_CONVERGED when
+   it is not (x is the best iterate, *relres its residual), or a negative
+   error. */
 int scuffx_refine(const void *M_dev, long n, long ldm, const void *LU32_dev, long ld32,
                   const int *ipiv_dev, const void *b_dev, void *x_dev, double tol, int maxiter,
                   int *iters, double *relres);
+/* GMRES-IR: x0 = LU32^{-1} b, then corrections d from restarted flexible
+   GMRES(20) on M d = b - M x in complex128, right-preconditioned by the
+   complex64 factors; maxiter bounds the total GMRES steps (*iters).  Same
+   tolerance and returns as scuffx_refine. */
+int scuffx_gmres_ir(const void *M_dev, long n, long ldm, const void *LU32_dev, long ld32,
+                    const int *ipiv_dev, const void *b_dev, void *x_dev, double tol, int maxiter,
+                    int *iters, double *relres);
 
 /* ---- callers either side of the path ---- */
 /* AssembleRHSVector for one plane wave in the exterior medium

@@ -13,6 +13,7 @@
 #include "assemble.h"
 #include "fippi_cache.h"
 #include "prof.h"
+#include "ws.h"
 
 #include <algorithm>
 #include <chrono>
@@ -168,6 +169,7 @@ Assembler::Assembler(const Geometry &G) : G_(G)
 
 Assembler::~Assembler()
 {
+  if (hip_down()) return;   // after scuffx_shutdown: left to the process exit
   for (auto &D : ds_) { hipFree(D.V); hipFree(D.PVI); hipFree(D.PC); hipFree(D.EI); hipFree(D.EL); }
   for (auto &B : blocks_) { hipFree(B.d_rowPtr); hipFree(B.d_col); hipFree(B.d_idx); hipFree(B.d_pairs); }
   hipFree(d_jobs_); hipFree(d_records_);
@@ -526,10 +528,11 @@ int Assembler::run_sw(int sa, int sb, int bi, const RegionK *rk, int nreg, const
   return 0;
 }
 
-int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
+int Assembler::assemble(cdouble omega, void *Mp, long ldm, int kind, hipStream_t stream)
 {
   if (!err_.empty()) return -3;
-  if (ldm < G_.TotalBFs) { err_ = "leading dimension smaller than the matrix"; return -1; }
+  if (kind != MAT_F64_PACKED && ldm < G_.TotalBFs) { err_ = "leading dimension smaller than the matrix"; return -1; }
+  const MatOut M{Mp, ldm, kind};
   int rc = build_fippi(stream);
   if (rc) return rc;
   const int NS = (int)G_.Surfaces.size();
@@ -551,8 +554,7 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
         // (AssembleBEMMatrix.cc:733-740); the lower triangle comes from the
         // final mirror as for every diagonal block
         const long om = G_.BFIndexOffset[mates_[sa]], os = G_.BFIndexOffset[sa], n = Sa.NumBFs;
-        HIPCHK(hipMemcpy2DAsync(M + os + os * ldm, sizeof(double2) * ldm, M + om + om * ldm, sizeof(double2) * ldm,
-                                sizeof(double2) * n, n, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(launch_copy_block(stream, M, om, os, n));
         continue;
       }
       RegionK rk[2];
@@ -612,7 +614,7 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
         int tok = prof_begin(stream, PK_STAMP);
         HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same,
                             Sa.IsPEC, Sb.IsPEC, nreg, dreg, needH1, d_HP_, d_HN_, d_HS_,
-                            B.d_rowPtr, B.d_col, B.d_idx, M, ldm, G_.BFIndexOffset[sa], G_.BFIndexOffset[sb]));
+                            B.d_rowPtr, B.d_col, B.d_idx, M, G_.BFIndexOffset[sa], G_.BFIndexOffset[sb]));
         prof_end(stream, tok, 16.0 * (same ? 0.5 * nbf : nbf));
       }
       hipEventRecord(ev[4], stream);
@@ -626,7 +628,7 @@ int Assembler::assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream)
       stats_.nnear += (long)B.pairs.size();
     }
   hipEventRecord(ev[0], stream);
-  HIPCHK(launch_mirror(stream, M, ldm, G_.TotalBFs));
+  HIPCHK(launch_mirror(stream, M, G_.TotalBFs));
   hipEventRecord(ev[1], stream);
   HIPCHK(hipEventSynchronize(ev[1]));
   float ms;
@@ -723,8 +725,9 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
     if (rc2) return rc2;
   }
   HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same, Sa.IsPEC, Sb.IsPEC,
-                      nreg, d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx, target, ldt, 0, 0));
-  if (same) HIPCHK(launch_mirror(stream, B, ldb, Sa.NumBFs));
+                      nreg, d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx,
+                      MatOut{target, ldt, MAT_C128}, 0, 0));
+  if (same) HIPCHK(launch_mirror(stream, MatOut{B, ldb, MAT_C128}, Sa.NumBFs));
   if (stage) {
     dim3 grid((Sa.NumBFs + 31) / 32, (Sb.NumBFs + 31) / 32);
     hipLaunchKernelGGL(k_transpose_into, grid, dim3(256), 0, stream, (const double2 *)stage, ldt, Sa.NumBFs,

@@ -39,8 +39,10 @@ class Assembler {
  public:
   explicit Assembler(const Geometry &G);
   ~Assembler();
-  // fills the full N x N matrix (column-major complex128, device memory)
-  int assemble(cdouble omega, double2 *M, long ldm, hipStream_t stream);
+  // fills the full N x N matrix in device memory: kind MAT_C128 (column-
+  // major complex128), MAT_F64 (real parts, column-major) or
+  // MAT_F64_PACKED (real parts, packed upper triangle; ldm unused)
+  int assemble(cdouble omega, void *M, long ldm, int kind, hipStream_t stream);
   int build_fippi(hipStream_t stream);
   // block of surfaces sa (rows) and sb (columns); for sa != sb optionally
   // its derivatives with respect to a rigid motion of surface sa

@@ -47,6 +47,23 @@ struct NearPair {
   int pad;
 };
 
+// destination of the stamped matrix entries: complex128 (M[r + c ld]),
+// real part only (M[r + c ld], the LHM_REAL matrix of an imaginary
+// frequency) or real part in packed upper storage (M[r + c (c+1)/2] for
+// r <= c, LHM_SYMMETRIC; entries below the diagonal are not stored)
+enum { MAT_C128 = 0, MAT_F64 = 1, MAT_F64_PACKED = 2 };
+struct MatOut {
+  void *p;
+  long ld;
+  int kind;
+  __device__ inline void put(long r, long c, cplx v) const
+  {
+    if (kind == MAT_C128) ((double2 *)p)[r + c * ld] = make_double2(v.r, v.i);
+    else if (kind == MAT_F64) ((double *)p)[r + c * ld] = v.r;
+    else if (r <= c) ((double *)p)[r + c * (c + 1) / 2] = v.r;
+  }
+};
+
 struct RegionK {
   cplx k;             // wavenumber in the region
   cplx pf1, pf2, pf3; // i*mu*w*s, -i*k*s, -i*eps*w*s
@@ -69,8 +86,9 @@ hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const doubl
                         int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
                         const double *HP, const double *HN, const double *HS,
                         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
-                        double2 *M, long ldm, int rowOff, int colOff);
-hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n);
+                        MatOut M, int rowOff, int colOff);
+hipError_t launch_mirror(hipStream_t stream, MatOut M, int n);
+hipError_t launch_copy_block(hipStream_t stream, MatOut M, long from, long to, long n);
 // short-wavelength panel pairs (kern_sw.hip): kind 0 = near pair without a
 // common vertex (all 9 combinations, 78-point rules), 1 = common edge /
 // vertex near pair (Taylor-Duffy, combination q); pair = near-pair index

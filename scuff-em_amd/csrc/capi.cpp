@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -17,6 +19,7 @@
 #include "adjoint.h"
 #include "lu.h"
 #include "prof.h"
+#include "refine.h"
 #include "rhs.h"
 #include "ws.h"
 
@@ -26,6 +29,14 @@ static thread_local std::string g_err;
 static thread_local hipStream_t g_stream = nullptr;
 
 static int fail(int code, const std::string &msg) { g_err = msg; return code; }
+static int refine_status(int rc, double rel, double tol);
+
+// for entry points defined in other files of the library
+namespace scuffx {
+int capi_fail(int code, const char *msg) { return fail(code, msg); }
+int capi_refine_status(int rc, double rel, double tol) { return refine_status(rc, rel, tol); }
+hipStream_t capi_stream() { return g_stream; }
+}  // namespace scuffx
 
 struct DevRhs {
   double *PA = nullptr, *PZ = nullptr, *work = nullptr;
@@ -36,6 +47,15 @@ struct scuffx_trans {
   std::vector<GTComplex> C;
 };
 
+struct scuffx_geom;
+// live geometries, so that scuffx_shutdown can release their device memory
+static std::mutex g_geom_mu;
+static std::set<scuffx_geom *> &live_geoms()
+{
+  static std::set<scuffx_geom *> *s = new std::set<scuffx_geom *>();   // never destroyed
+  return *s;
+}
+
 struct scuffx_geom {
   Geometry G;
   std::unique_ptr<Assembler> A;
@@ -43,14 +63,32 @@ struct scuffx_geom {
   bool transformed = false;     // a surface moved since the last UnTransform
   std::vector<DevRhs> rhs;
   double *d_out = nullptr;
+  scuffx_geom()
+  {
+    std::lock_guard<std::mutex> lk(g_geom_mu);
+    live_geoms().insert(this);
+  }
   ~scuffx_geom()
   {
+    {
+      std::lock_guard<std::mutex> lk(g_geom_mu);
+      live_geoms().erase(this);
+    }
+    release_device();
+  }
+  // frees every device buffer (no HIP call after shutdown)
+  void release_device()
+  {
+    if (hip_down()) return;
+    A.reset();
     drop_rhs();
     hipFree(d_out);
+    d_out = nullptr;
   }
   void drop_rhs()
   {
-    for (auto &r : rhs) { hipFree(r.PA); hipFree(r.PZ); hipFree(r.work); hipFree(r.PEI); hipFree(r.EV); }
+    if (!hip_down())
+      for (auto &r : rhs) { hipFree(r.PA); hipFree(r.PZ); hipFree(r.work); hipFree(r.PEI); hipFree(r.EV); }
     rhs.clear();
   }
   // the geometry moved: device copies are rebuilt on their next use
@@ -71,7 +109,20 @@ struct scuffx_geom {
 extern "C" {
 
 const char *scuffx_last_error(void) { return g_err.c_str(); }
-int scuffx_version(void) { return 100; }
+int scuffx_version(void) { return 200; }
+
+int scuffx_shutdown(void)
+{
+  if (hip_down()) return 0;
+  hipDeviceSynchronize();
+  {
+    std::lock_guard<std::mutex> lk(g_geom_mu);
+    for (scuffx_geom *g : live_geoms()) g->release_device();
+  }
+  ws_release_all();
+  set_hip_down();
+  return 0;
+}
 int scuffx_set_stream(void *s) { g_stream = (hipStream_t)s; return 0; }
 
 int scuffx_geom_new(scuffx_geom **G)
@@ -331,6 +382,7 @@ int scuffx_icosphere(int level, double radius, const double *center, double *V, 
 
 int scuffx_fippi_build(scuffx_geom *G)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!G || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "empty geometry");
   Assembler *A = G->assembler();
   int rc = A->build_fippi(g_stream);
@@ -381,17 +433,73 @@ int scuffx_fippi_records(scuffx_geom *G, double *out)
 
 int scuffx_assemble(scuffx_geom *G, double wr, double wi, void *M, long ldm)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!G || !M || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "bad arguments");
   if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
   Assembler *A = G->assembler();
-  int rc = A->assemble(cdouble(wr, wi), (double2 *)M, ldm, g_stream);
+  int rc = A->assemble(cdouble(wr, wi), M, ldm, MAT_C128, g_stream);
   if (rc) return fail(rc, A->error());
+  return 0;
+}
+
+int scuffx_assemble_real(scuffx_geom *G, double wr, double wi, double *M, long ldm, int packed)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!G || !M || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
+  Assembler *A = G->assembler();
+  int rc = A->assemble(cdouble(wr, wi), M, ldm, packed ? MAT_F64_PACKED : MAT_F64, g_stream);
+  if (rc) return fail(rc, A->error());
+  return 0;
+}
+
+int scuffx_assemble_block_real(scuffx_geom *G, int nsa, int nsb, double wr, double wi, double *B, long ldb,
+                               int packed, double *GradB, long ldg, long gstride, int ntorque, const double *gamma,
+                               double *dBdT)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!G || !B || G->G.Surfaces.empty() || nsa < 0 || nsb < 0 || nsa >= (int)G->G.Surfaces.size() ||
+      nsb >= (int)G->G.Surfaces.size() || ntorque < 0 || ntorque > 3)
+    return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (packed && (nsa != nsb || GradB)) return fail(SCUFFX_ERR_ARG, "packed storage is for diagonal blocks only");
+  const long na = G->G.Surfaces[nsa].NumBFs, nb = G->G.Surfaces[nsb].NumBFs;
+  if ((!packed && ldb < na) || (GradB && (ldg < na || gstride < ldg * nb)))
+    return fail(SCUFFX_ERR_ARG, "leading dimension smaller than the block");
+  // the complex block (and its derivatives) are staged, then their real
+  // parts stored: at an imaginary frequency the imaginary parts vanish
+  const long nmat = 1 + (GradB ? 3 + ntorque : 0);
+  double2 *st = (double2 *)ws_get(WS_STAGE, sizeof(double2) * na * nb * nmat);
+  if (!st) return fail(SCUFFX_ERR_ALLOC, "staging allocation failed");
+  const long bs = na * nb;
+  int rc = scuffx_assemble_block(G, nsa, nsb, wr, wi, st, na, GradB ? st + bs : nullptr, na, bs, ntorque, gamma,
+                                 ntorque ? st + 4 * bs : nullptr);
+  if (rc) return rc;
+  if (real_part_c128(st, na, B, ldb, na, nb, packed, g_stream)) return fail(SCUFFX_ERR_HIP, "conversion failed");
+  for (long m = 1; m < nmat; m++) {
+    double *dst = m <= 3 ? GradB + (m - 1) * gstride : dBdT + (m - 4) * gstride;
+    if (real_part_c128(st + m * bs, na, dst, ldg, na, nb, 0, g_stream)) return fail(SCUFFX_ERR_HIP, "conversion failed");
+  }
+  if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "conversion failed");
+  return 0;
+}
+
+int scuffx_unpack_symmetric(const void *P, long n, void *A, long lda, int prec)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!P || !A || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  int rc;
+  if (prec == 8) rc = unpack_symmetric<double>((const double *)P, (double *)A, lda, n, g_stream);
+  else if (prec == 64) rc = unpack_symmetric<double2>((const double2 *)P, (double2 *)A, lda, n, g_stream);
+  else if (prec == 32) rc = unpack_symmetric<float2>((const float2 *)P, (float2 *)A, lda, n, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64, 32 or 8");
+  if (rc || hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "unpack failed");
   return 0;
 }
 
 int scuffx_assemble_block(scuffx_geom *G, int nsa, int nsb, double wr, double wi, void *B, long ldb, void *GradB,
                           long ldg, long gstride, int ntorque, const double *gamma, void *dBdT)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!G || !B || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "bad arguments");
   if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
   Assembler *A = G->assembler();
@@ -421,21 +529,25 @@ static int lu_status(int rc, int info)
 
 int scuffx_lu_factor(void *A, long n, long lda, int *ipiv, int prec)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!A || !ipiv || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad LU arguments");
   int info = 0, rc;
   if (prec == 64) rc = lu_factor<double2>((double2 *)A, n, lda, ipiv, &info, g_stream);
   else if (prec == 32) rc = lu_factor<float2>((float2 *)A, n, lda, ipiv, &info, g_stream);
-  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  else if (prec == 8) rc = lu_factor<double>((double *)A, n, lda, ipiv, &info, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64, 32 or 8");
   return lu_status(rc, info);
 }
 
 int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char trans, long nrhs, void *B, long ldb, int prec)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!LU || !ipiv || !B) return fail(SCUFFX_ERR_ARG, "null argument");
   int rc;
   if (prec == 64) rc = lu_solve<double2>((const double2 *)LU, n, lda, ipiv, trans, nrhs, (double2 *)B, ldb, g_stream);
   else if (prec == 32) rc = lu_solve<float2>((const float2 *)LU, n, lda, ipiv, trans, nrhs, (float2 *)B, ldb, g_stream);
-  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  else if (prec == 8) rc = lu_solve<double>((const double *)LU, n, lda, ipiv, trans, nrhs, (double *)B, ldb, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64, 32 or 8");
   if (rc == -1) return fail(SCUFFX_ERR_ARG, "bad LU solve arguments");
   if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "solve workspace allocation failed");
   if (rc) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
@@ -445,12 +557,14 @@ int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char tran
 
 int scuffx_lu_logdet(const void *LU, long n, long lda, const int *ipiv, int prec, double *logabs, double *phase)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!LU || !ipiv || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad arguments");
   double out[2] = {0, 0};
   int rc;
   if (prec == 64) rc = lu_logdet<double2>((const double2 *)LU, n, lda, ipiv, out, g_stream);
   else if (prec == 32) rc = lu_logdet<float2>((const float2 *)LU, n, lda, ipiv, out, g_stream);
-  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  else if (prec == 8) rc = lu_logdet<double>((const double *)LU, n, lda, ipiv, out, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64, 32 or 8");
   if (rc) return fail(SCUFFX_ERR_HIP, "log det failed on the device");
   if (logabs) *logabs = out[0];
   if (phase) *phase = out[1];
@@ -465,6 +579,7 @@ int scuffx_trace_solve(const void *LU, long n, long lda, const int *ipiv, void *
   if (rc) return rc;
   double out[2] = {0, 0};
   if (prec == 64) rc = mat_trace<double2>((const double2 *)D, ncols, ldd, out, g_stream);
+  else if (prec == 8) rc = mat_trace<double>((const double *)D, ncols, ldd, out, g_stream);
   else rc = mat_trace<float2>((const float2 *)D, ncols, ldd, out, g_stream);
   if (rc) return fail(SCUFFX_ERR_HIP, "trace failed on the device");
   if (tr_re) *tr_re = out[0];
@@ -472,8 +587,26 @@ int scuffx_trace_solve(const void *LU, long n, long lda, const int *ipiv, void *
   return 0;
 }
 
+int scuffx_lu_invert(void *A, long n, long lda, const int *ipiv, int prec)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!A || !ipiv || n < 0 || lda < (n > 1 ? n : 1)) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  size_t es = prec == 64 ? sizeof(double2) : prec == 32 ? sizeof(float2) : prec == 8 ? sizeof(double) : 0;
+  if (!es) return fail(SCUFFX_ERR_ARG, "prec must be 64, 32 or 8");
+  void *W = ws_get(WS_STAGE, es * (size_t)n * n + 256);
+  if (!W) return fail(SCUFFX_ERR_ALLOC, "workspace allocation failed");
+  int info = 0, rc;
+  if (prec == 64) rc = lu_invert<double2>((double2 *)A, n, lda, ipiv, (double2 *)W, &info, g_stream);
+  else if (prec == 32) rc = lu_invert<float2>((float2 *)A, n, lda, ipiv, (float2 *)W, &info, g_stream);
+  else rc = lu_invert<double>((double *)A, n, lda, ipiv, (double *)W, &info, g_stream);
+  if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "solve workspace allocation failed");
+  if (rc) return fail(SCUFFX_ERR_HIP, "LU inverse failed on the device");
+  return info;
+}
+
 int scuffx_lu32_from_c128(const void *M, long n, long ldm, void *LU32, long ld32, int *ipiv)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!M || !LU32 || !ipiv || ldm < n || ld32 < n) return fail(SCUFFX_ERR_ARG, "bad arguments");
   if (convert_c128_to_c64((const double2 *)M, (float2 *)LU32, n, n, ldm, ld32, g_stream)) return fail(SCUFFX_ERR_HIP, "conversion failed");
   int info = 0;
@@ -481,45 +614,53 @@ int scuffx_lu32_from_c128(const void *M, long n, long ldm, void *LU32, long ld32
   return lu_status(rc, info);
 }
 
-int scuffx_refine(const void *M, long n, long ldm, const void *LU32, long ld32, const int *ipiv,
-                  const void *b, void *x, double tol, int maxiter, int *iters, double *relres)
+static int refine_status(int rc, double rel, double tol)
 {
-  if (!M || !LU32 || !ipiv || !b || !x || n <= 0) return fail(SCUFFX_ERR_ARG, "bad arguments");
-  int nch = (int)((n + 511) / 512);
-  size_t nr = 256 * ((sizeof(double2) * n + 255) / 256), nw = 256 * ((sizeof(double2) * n * nch + 255) / 256);
-  size_t n32 = 256 * ((sizeof(float2) * n + 255) / 256);
-  char *wsb = (char *)ws_get(WS_REFINE, nr + nw + n32 + 256);
-  if (!wsb) return fail(SCUFFX_ERR_ALLOC, "workspace allocation failed");
-  double2 *r = (double2 *)wsb, *work = (double2 *)(wsb + nr);
-  float2 *r32 = (float2 *)(wsb + nr + nw);
-  double *scal = (double *)(wsb + nr + nw + n32);
-  hipStream_t s = g_stream;
-  const double2 *bb = (const double2 *)b;
-  double2 *xx = (double2 *)x;
-  double bn = absmax_c128(bb, n, scal, s);
-  // x0 from the complex64 factors
-  convert_c128_to_c64(bb, r32, n, 1, n, n, s);
-  lu_solve<float2>((const float2 *)LU32, n, ld32, ipiv, 'N', 1, r32, n, s);
-  convert_c64_to_c128(r32, xx, n, s);
-  int it = 0;
-  double rel = 0, prev = 1e300;
-  for (;;) {
-    int tok = prof_begin(s, PK_RESID);
-    residual_c128((const double2 *)M, n, ldm, xx, bb, r, work, s);
-    prof_end(s, tok, 16.0 * (double)n * (double)n);
-    rel = absmax_c128(r, n, scal, s) / (bn > 0 ? bn : 1.0);
-    if (rel <= tol || it >= maxiter || rel >= prev) break;
-    prev = rel;
-    convert_c128_to_c64(r, r32, n, 1, n, n, s);
-    lu_solve<float2>((const float2 *)LU32, n, ld32, ipiv, 'N', 1, r32, n, s);
-    axpy_c64_into_c128(r32, xx, n, s);
-    it++;
+  if (rc == 1) {
+    char b[160];
+    std::snprintf(b, sizeof b, "refinement stopped at relative residual %.3e above the tolerance %.3e", rel, tol);
+    g_err = b;
+    return SCUFFX_NOT_CONVERGED;
   }
-  hipStreamSynchronize(s);
+  if (rc == -1) return fail(SCUFFX_ERR_ARG, "bad refinement arguments");
+  if (rc == -2) return fail(SCUFFX_ERR_ALLOC, "refinement workspace allocation failed");
+  if (rc == -4) return fail(SCUFFX_ERR_NUMERIC, "non-finite residual in the refinement");
+  if (rc == -6) return fail(SCUFFX_ERR_HIP, "a triangular solve's bounded wait timed out");
+  if (rc) return fail(SCUFFX_ERR_HIP, "refinement failed on the device");
+  return 0;
+}
+
+static int refine_common(const void *M, long n, long ldm, const void *LU32, long ld32, const int *ipiv,
+                         const void *b, void *x, double tol, int maxiter, int *iters, double *relres, int mode)
+{
+  if (!M || !LU32 || !ipiv || !b || !x || n <= 0 || ldm < n || ld32 < n || maxiter < 0)
+    return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  char *wsb = (char *)ws_get(WS_REFINE, refine_ws_bytes(n, REFINE_GMRES_M));
+  if (!wsb) return fail(SCUFFX_ERR_ALLOC, "workspace allocation failed");
+  RefineWs w = refine_ws_carve(wsb, n, REFINE_GMRES_M);
+  int it = 0;
+  double rel = 0;
+  int rc = mode == 0 ? refine_ir((const double2 *)M, n, ldm, (const float2 *)LU32, ld32, ipiv, (const double2 *)b,
+                                 (double2 *)x, tol, maxiter, 1, &it, &rel, w, g_stream)
+                     : gmres_ir((const double2 *)M, n, ldm, (const float2 *)LU32, ld32, ipiv, (const double2 *)b,
+                                (double2 *)x, tol, maxiter, false, &it, &rel, w, g_stream);
   if (prof_enabled()) prof_collect();
   if (iters) *iters = it;
   if (relres) *relres = rel;
-  return 0;
+  return refine_status(rc, rel, tol);
+}
+
+int scuffx_refine(const void *M, long n, long ldm, const void *LU32, long ld32, const int *ipiv,
+                  const void *b, void *x, double tol, int maxiter, int *iters, double *relres)
+{
+  return refine_common(M, n, ldm, LU32, ld32, ipiv, b, x, tol, maxiter, iters, relres, 0);
+}
+
+int scuffx_gmres_ir(const void *M, long n, long ldm, const void *LU32, long ld32, const int *ipiv,
+                    const void *b, void *x, double tol, int maxiter, int *iters, double *relres)
+{
+  return refine_common(M, n, ldm, LU32, ld32, ipiv, b, x, tol, maxiter, iters, relres, 1);
 }
 
 static int ensure_rhs_data(scuffx_geom *G)
@@ -553,6 +694,7 @@ static int ensure_rhs_data(scuffx_geom *G)
 
 int scuffx_rhs_planewave(scuffx_geom *G, double wr, double wi, const double *E0, const double *nhat, void *rhs)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!G || !E0 || !nhat || !rhs) return fail(SCUFFX_ERR_ARG, "null argument");
   Assembler *A = G->assembler();
   if (!A->error().empty()) return fail(SCUFFX_ERR_HIP, A->error());
@@ -580,6 +722,7 @@ int scuffx_rhs_planewave(scuffx_geom *G, double wr, double wi, const double *E0,
 
 int scuffx_power(scuffx_geom *G, double wr, double wi, const void *KN, const void *rhs, double *PAbs, double *PExt)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   (void)wr; (void)wi;
   if (!G || !KN || !rhs) return fail(SCUFFX_ERR_ARG, "null argument");
   Assembler *A = G->assembler();
@@ -606,6 +749,7 @@ int scuffx_power(scuffx_geom *G, double wr, double wi, const void *KN, const voi
 int scuffx_adjoint(scuffx_geom *G, int ns, double wr, double wi, const void *x, const void *y, int ntorque,
                    const double *gamma, double *out)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!G || !x || !y || !out || ns < 0 || ns >= (int)G->G.Surfaces.size() || ntorque < 0 || ntorque > 3 ||
       (ntorque > 0 && !gamma))
     return fail(SCUFFX_ERR_ARG, "bad arguments");
@@ -659,6 +803,7 @@ int scuffx_adjoint(scuffx_geom *G, int ns, double wr, double wi, const void *x, 
 int scuffx_batch_sweep(scuffx_geom *G, const double *omegas, int count, const double *E0, const double *nhat,
                        int prec, double *out)
 {
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
   if (!G || !omegas || count < 0 || !E0 || !nhat || !out || G->G.Surfaces.empty())
     return fail(SCUFFX_ERR_ARG, "bad arguments");
   if (prec != 64 && prec != 32) return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
@@ -683,7 +828,10 @@ int scuffx_batch_sweep(scuffx_geom *G, const double *omegas, int count, const do
       if ((rc = scuffx_lu32_from_c128(M, n, n, lu32, n, ipiv)) != 0) break;
       int it = 0;
       double rel = 0;
-      if ((rc = scuffx_refine(M, n, n, lu32, n, ipiv, rhs, kn, 1e-13, 30, &it, &rel)) != 0) break;
+      if ((rc = scuffx_refine(M, n, n, lu32, n, ipiv, rhs, kn, 1e-13, 30, &it, &rel)) != 0) {
+        if (rc == SCUFFX_NOT_CONVERGED) rc = fail(SCUFFX_ERR_NUMERIC, g_err);
+        break;
+      }
     }
     double pa = 0, pe = 0;
     if ((rc = scuffx_power(G, wr, wi, kn, rhs, &pa, &pe)) != 0) break;

@@ -345,7 +345,7 @@ k_stamp(int nEa, const int *EIa, const double *ELa, int nPa, int nEb, const int 
         int nPb, int same, int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
         const double *HP, const double *HN, const double *HS,
         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
-        double2 *M, long ldm, int rowOff, int colOff)
+        MatOut M, int rowOff, int colOff)
 {
   int ea = blockIdx.x * blockDim.x + threadIdx.x;
   int eb = blockIdx.y;
@@ -375,19 +375,19 @@ k_stamp(int nEa, const int *EIa, const double *ELa, int nPa, int nEb, const int 
   }
   long R = rowOff, Cc = colOff;
   if (pecA && pecB) {
-    M[(R + ea) + (Cc + eb) * ldm] = make_double2(m00.r, m00.i);
+    M.put(R + ea, Cc + eb, m00);
   } else if (pecA) {
-    M[(R + ea) + (Cc + 2 * eb) * ldm] = make_double2(m00.r, m00.i);
-    M[(R + ea) + (Cc + 2 * eb + 1) * ldm] = make_double2(m01.r, m01.i);
+    M.put(R + ea, Cc + 2 * eb, m00);
+    M.put(R + ea, Cc + 2 * eb + 1, m01);
   } else if (pecB) {
-    M[(R + 2 * ea) + (Cc + eb) * ldm] = make_double2(m00.r, m00.i);
-    M[(R + 2 * ea + 1) + (Cc + eb) * ldm] = make_double2(m01.r, m01.i);
+    M.put(R + 2 * ea, Cc + eb, m00);
+    M.put(R + 2 * ea + 1, Cc + eb, m01);
   } else {
     long x = R + 2 * ea, y = Cc + 2 * eb;
-    M[x + y * ldm] = make_double2(m00.r, m00.i);
-    if (!same || ea != eb) M[(x + 1) + y * ldm] = make_double2(m01.r, m01.i);
-    M[x + (y + 1) * ldm] = make_double2(m01.r, m01.i);
-    M[(x + 1) + (y + 1) * ldm] = make_double2(m11.r, m11.i);
+    M.put(x, y, m00);
+    if (!same || ea != eb) M.put(x + 1, y, m01);
+    M.put(x, y + 1, m01);
+    M.put(x + 1, y + 1, m11);
   }
 }
 
@@ -396,13 +396,13 @@ hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const doubl
                         int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
                         const double *HP, const double *HN, const double *HS,
                         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
-                        double2 *M, long ldm, int rowOff, int colOff)
+                        MatOut M, int rowOff, int colOff)
 {
   if (nEa <= 0 || nEb <= 0) return hipSuccess;
   dim3 grid((nEa + 255) / 256, nEb);
   hipLaunchKernelGGL(k_stamp, grid, dim3(256), 0, stream, nEa, EIa, ELa, nPa, nEb, EIb, ELb, nPb,
                      same, pecA, pecB, nreg, regs, needH1, HP, HN, HS, nearRowPtr, nearCol, nearIdx,
-                     M, ldm, rowOff, colOff);
+                     M, rowOff, colOff);
   return hipGetLastError();
 }
 
@@ -410,9 +410,10 @@ hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const doubl
 // lower triangle <- transpose of the upper triangle
 // (AssembleBEMMatrix.cc:753-758), 32x32 tiles staged through LDS
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_mirror(double2 *M, long ldm, int n)
+template <typename E>
+__global__ void __launch_bounds__(256) k_mirror(E *M, long ldm, int n)
 {
-  __shared__ double2 tile[32][33];
+  __shared__ E tile[32][33];
   int bi = blockIdx.y, bj = blockIdx.x;   // destination tile (rows bi, cols bj), bi >= bj
   if (bj > bi) return;
   int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
@@ -428,12 +429,39 @@ __global__ void __launch_bounds__(256) k_mirror(double2 *M, long ldm, int n)
   }
 }
 
-hipError_t launch_mirror(hipStream_t stream, double2 *M, long ldm, int n)
+hipError_t launch_mirror(hipStream_t stream, MatOut M, int n)
 {
-  if (n <= 1) return hipSuccess;
+  if (n <= 1 || M.kind == MAT_F64_PACKED) return hipSuccess;   // packed storage holds the upper triangle only
   int nt = (n + 31) / 32;
-  hipLaunchKernelGGL(k_mirror, dim3(nt, nt), dim3(256), 0, stream, M, ldm, n);
+  if (M.kind == MAT_C128)
+    hipLaunchKernelGGL(k_mirror<double2>, dim3(nt, nt), dim3(256), 0, stream, (double2 *)M.p, M.ld, n);
+  else
+    hipLaunchKernelGGL(k_mirror<double>, dim3(nt, nt), dim3(256), 0, stream, (double *)M.p, M.ld, n);
   return hipGetLastError();
+}
+
+// diagonal block of surface size n copied from offset `from` to offset `to`
+// of a packed upper-triangular matrix (the Mate copy of packed storage)
+__global__ void k_copy_packed_block(double *M, long from, long to, long n)
+{
+  const long c = blockIdx.y, r = (long)blockIdx.x * 256 + threadIdx.x;
+  if (r > c || c >= n) return;
+  const long cf = from + c, ct = to + c;
+  M[(to + r) + ct * (ct + 1) / 2] = M[(from + r) + cf * (cf + 1) / 2];
+}
+
+hipError_t launch_copy_block(hipStream_t stream, MatOut M, long from, long to, long n)
+{
+  if (n <= 0) return hipSuccess;
+  if (M.kind == MAT_F64_PACKED) {
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+    hipLaunchKernelGGL(k_copy_packed_block, grid, dim3(256), 0, stream, (double *)M.p, from, to, n);
+    return hipGetLastError();
+  }
+  const size_t es = M.kind == MAT_C128 ? sizeof(double2) : sizeof(double);
+  char *base = (char *)M.p;
+  return hipMemcpy2DAsync(base + es * (to + to * M.ld), es * M.ld, base + es * (from + from * M.ld), es * M.ld,
+                          es * n, n, hipMemcpyDeviceToDevice, stream);
 }
 
 }  // namespace scuffx

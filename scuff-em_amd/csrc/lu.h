@@ -11,7 +11,8 @@
 
 namespace scuffx {
 
-// T = double2 (complex128) or float2 (complex64).  fast = 1 solves the
+// T = double2 (complex128), float2 (complex64) or double (real, the LHM_REAL
+// matrices of imaginary frequencies: dgetrf/dgetrs semantics).  fast = 1 solves the
 // block rows with the explicit inverses of the 32x32 unit-lower panel
 // blocks (products instead of substitution): same pivots, slightly larger
 // rounding; meant for preconditioner factorizations.
@@ -19,10 +20,31 @@ template <typename T>
 int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb = 256,
               int fast = 0);
 
-// solves op(A) X = B with the factors of lu_factor; trans 'N', 'T', 'C'
+// Caller-provided buffers of a one-right-hand-side solve (trans 'N'):
+// the intermediate vector W (n), the inverses of the 64x64 diagonal blocks
+// of L and U (nblk * 64 * 64 each, nblk = ceil(n / 64)), the composed row
+// permutation (n ints) and the one-launch solve's words (2 nblk + 8).
+// inv_ready: invL / invU already hold the inverses for these factors (set
+// by the first solve; clear it for new factors).
+template <typename T>
+struct SolveWs {
+  T *W = nullptr, *invL = nullptr, *invU = nullptr;
+  int *perm = nullptr;
+  unsigned long long *chain = nullptr;
+  bool inv_ready = false;
+};
+// bytes of the buffers above for n rows, each part rounded to 256 bytes
+template <typename T>
+size_t solve_ws_bytes(long n);
+// carves the buffers from `base` (solve_ws_bytes(n) bytes)
+template <typename T>
+SolveWs<T> solve_ws_carve(char *base, long n);
+
+// solves op(A) X = B with the factors of lu_factor; trans 'N', 'T', 'C'.
+// With ws (trans 'N', nrhs 1 only) it allocates nothing.
 template <typename T>
 int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B,
-             long ldb, hipStream_t stream);
+             long ldb, hipStream_t stream, SolveWs<T> *ws = nullptr);
 
 // log |det| and arg det of an LU-factored matrix (out[0], out[1]; host)
 template <typename T>
@@ -31,15 +53,28 @@ int lu_logdet(const T *LU, long n, long lda, const int *ipiv, double out[2], hip
 template <typename T>
 int mat_trace(const T *A, long n, long lda, double out[2], hipStream_t stream);
 
-// r = b - A x (complex128, A column-major n x n); all device pointers
+// r = b - A x (complex128, A column-major n x n); all device pointers;
+// b == nullptr gives r = A x.  work: n * residual_chunks(n) elements
 int residual_c128(const double2 *A, long n, long lda, const double2 *x, const double2 *b,
                   double2 *r, double2 *work, hipStream_t stream);
+long residual_chunks(long n);
 // y = (complex64) x and back
 int convert_c128_to_c64(const double2 *x, float2 *y, long rows, long cols, long ldx, long ldy,
                         hipStream_t stream);
 int convert_c64_to_c128(const float2 *x, double2 *y, long n, hipStream_t stream);
 // x += (complex128) d
 int axpy_c64_into_c128(const float2 *d, double2 *x, long n, hipStream_t stream);
+// real parts of a complex128 block into a real matrix (packed: upper
+// storage dst[r + c (c+1)/2] of a square block, r <= c)
+int real_part_c128(const double2 *x, long ldx, double *y, long ldy, long rows, long cols, int packed,
+                   hipStream_t stream);
+// full symmetric n x n matrix from packed upper storage
+template <typename T>
+int unpack_symmetric(const T *P, T *A, long lda, long n, hipStream_t stream);
+// A <- A^{-1} from its LU factors (LUInvert); W: n*n elements + one int;
+// *info = first zero diagonal entry of U (A then unchanged), else 0
+template <typename T>
+int lu_invert(T *A, long n, long lda, const int *ipiv, T *W, int *info, hipStream_t stream);
 // max_i |re|+|im| of a vector (host result)
 double absmax_c128(const double2 *x, long n, double *work, hipStream_t stream);
 

@@ -45,6 +45,13 @@ template <> struct CT<double2> {
   __device__ static inline V4 zero() { return V4{0, 0, 0, 0}; }
   __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
+template <> struct CT<double> {
+  typedef double R;
+  typedef d4 V4;
+  __device__ static inline V4 mfma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static inline V4 zero() { return V4{0, 0, 0, 0}; }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
 template <> struct CT<float2> {
   typedef float R;
   typedef f4 V4;
@@ -65,6 +72,44 @@ template <typename T> __device__ inline T crecip(T a)
   else { R t = a.x / a.y, d = a.x * t + a.y; r.x = t / d; r.y = -1 / d; }
   return r;
 }
+// real (double) counterparts, so that the panel and the solves also
+// instantiate for real matrices (LHM_REAL, dgetrf/dgetrs semantics)
+__device__ inline double cmul(double a, double b) { return a * b; }
+__device__ inline double csub(double a, double b) { return a - b; }
+__device__ inline double cadd(double a, double b) { return a + b; }
+__device__ inline double crecip(double a) { return 1.0 / a; }
+__device__ inline double cdiv(double a, double b) { return a / b; }
+// pivot magnitude |re| + |im| (izamax / idamax)
+__device__ inline double mag1(double a) { return fabs(a); }
+template <typename T> __device__ inline auto mag1(T a) -> decltype(a.x) { return fabs(a.x) + fabs(a.y); }
+__device__ inline bool nonzero(double a) { return a != 0; }
+template <typename T> __device__ inline bool nonzero(T a) { return a.x != 0 || a.y != 0; }
+template <typename T> __device__ inline T zero_v() { T r = zero_v<T>(); return r; }
+template <> __device__ inline double zero_v<double>() { return 0; }
+template <typename T> __device__ inline T unit_v() { T r; r.x = 1; r.y = 0; return r; }
+template <> __device__ inline double unit_v<double>() { return 1; }
+// acc += a b, real and imaginary parts each as one expression
+template <typename T> __device__ inline void mac(T &acc, T a, T b)
+{
+  mac(acc, a, b);
+}
+__device__ inline void mac(double &acc, double a, double b) { acc += a * b; }
+// acc += conj(a) b (conj = identity for real)
+template <typename T> __device__ inline void macc(T &acc, T a, T b, bool cj)
+{
+  if (cj) a.y = -a.y;
+  mac(acc, a, b);
+}
+__device__ inline void macc(double &acc, double a, double b, bool) { acc += a * b; }
+template <typename T> __device__ inline T conj_if(T a, bool cj) { if (cj) a.y = -a.y; return a; }
+__device__ inline double conj_if(double a, bool) { return a; }
+template <typename T> __device__ inline double re_of(T a) { return a.x; }
+template <typename T> __device__ inline double im_of(T a) { return a.y; }
+__device__ inline double re_of(double a) { return a; }
+__device__ inline double im_of(double) { return 0; }
+template <typename T> struct IsReal { static const bool v = false; };
+template <> struct IsReal<double> { static const bool v = true; };
+
 template <typename T> __device__ inline T cdiv(T a, T b)
 {
   typedef decltype(a.x) R;
@@ -109,10 +154,10 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       for (int s = 0; s < 4; s++) {
         int i = tid & 63, kk = (tid >> 6) + 4 * s;
         long gi = row0 + i, gk = k0 + kk;
-        if (gi < m && gk < k) v[s] = A[gi + gk * lda]; else { v[s].x = 0; v[s].y = 0; }
+        if (gi < m && gk < k) v[s] = A[gi + gk * lda]; else { v[s] = zero_v<T>(); }
         int kb = tid & 15, j = (tid >> 4) + 16 * s;
         long gkb = k0 + kb, gj = col0 + j;
-        if (gkb < k && gj < n) u[s] = B[gkb + gj * ldb]; else { u[s].x = 0; u[s].y = 0; }
+        if (gkb < k && gj < n) u[s] = B[gkb + gj * ldb]; else { u[s] = zero_v<T>(); }
       }
     }
   };
@@ -167,6 +212,75 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
       }
 }
 
+// Real (double) C -= A B or C = A B: the same 64x64 tile and k-chunk
+// staging, one v_mfma_f64_16x16x4f64 per product; each wave owns a 32x64
+// strip (2 x 4 tiles of 16x16), so one LDS read of A feeds four products.
+template <bool ACC>
+__global__ void __launch_bounds__(256)
+k_gemm_r(long m, long n, long k, const double *A, long lda, const double *B, long ldb, double *C, long ldc)
+{
+  __shared__ double sA[GBK][2 * GBM + 4], sB[GBK][GBN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long row0 = (long)blockIdx.x * 2 * GBM, col0 = (long)blockIdx.y * GBN;
+  d4 acc[2][4];
+  for (int a = 0; a < 2; a++) for (int b = 0; b < 4; b++) acc[a][b] = d4{0, 0, 0, 0};
+  // A chunk: 128 rows x 16, B chunk: 16 x 64 per k-chunk
+  auto load = [&](long k0, double (&v)[8], double (&u)[4]) {
+    if (row0 + 2 * GBM <= m && col0 + GBN <= n && k0 + GBK <= k) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = A[(row0 + (tid & 127)) + (k0 + (tid >> 7) + 2 * q) * lda];
+#pragma unroll
+      for (int q = 0; q < 4; q++) u[q] = B[(k0 + (tid & 15)) + (col0 + (tid >> 4) + 16 * q) * ldb];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        long gi = row0 + (tid & 127), gk = k0 + (tid >> 7) + 2 * q;
+        v[q] = (gi < m && gk < k) ? A[gi + gk * lda] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        long gk = k0 + (tid & 15), gj = col0 + (tid >> 4) + 16 * q;
+        u[q] = (gk < k && gj < n) ? B[gk + gj * ldb] : 0.0;
+      }
+    }
+  };
+  double v[8], u[4];
+  if (k > 0) load(0, v, u);
+  for (long k0 = 0; k0 < k; k0 += GBK) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) sA[(tid >> 7) + 2 * q][tid & 127] = v[q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) sB[tid & 15][(tid >> 4) + 16 * q] = u[q];
+    __syncthreads();
+    if (k0 + GBK < k) load(k0 + GBK, v, u);
+    for (int kk0 = 0; kk0 < GBK; kk0 += 4) {
+      const int kk = kk0 + (lane >> 4);
+      double ar[2], br[4];
+      for (int t = 0; t < 2; t++) ar[t] = sA[kk][wave * 32 + t * 16 + (lane & 15)];
+      for (int t = 0; t < 4; t++) br[t] = sB[kk][t * 16 + (lane & 15)];
+      const double sg = ACC ? -1.0 : 1.0;
+      for (int ti = 0; ti < 2; ti++)
+        for (int tj = 0; tj < 4; tj++)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(sg * ar[ti], br[tj], acc[ti][tj], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int ti = 0; ti < 2; ti++)
+    for (int tj = 0; tj < 4; tj++)
+      for (int r = 0; r < 4; r++) {
+        long gi = row0 + wave * 32 + ti * 16 + (lane >> 4) + 4 * r;
+        long gj = col0 + tj * 16 + (lane & 15);
+        if (gi < m && gj < n) {
+          if (ACC) C[gi + gj * ldc] += acc[ti][tj][r];
+          else C[gi + gj * ldc] = acc[ti][tj][r];
+        }
+      }
+}
+
+static int gemm_kind(const double2 *) { return PK_GEMM_C128; }
+static int gemm_kind(const float2 *) { return PK_GEMM_C64; }
+static int gemm_kind(const double *) { return PK_GEMM_F64; }
+
 // lds_pad: extra (unused) LDS per workgroup, which caps how many of its
 // workgroups share a CU
 template <typename T, bool ACC>
@@ -174,10 +288,16 @@ static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long 
                  hipStream_t s, unsigned lds_pad = 0)
 {
   if (m <= 0 || n <= 0 || k <= 0) return;
-  int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_GEMM_C128 : PK_GEMM_C64);
-  dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
-  hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc);
-  prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
+  int tok = prof_begin(s, gemm_kind(A));
+  if constexpr (IsReal<T>::v) {
+    dim3 grid((unsigned)((m + 2 * GBM - 1) / (2 * GBM)), (unsigned)((n + GBN - 1) / GBN));
+    hipLaunchKernelGGL((k_gemm_r<ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc);
+    prof_end(s, tok, 2.0 * (double)m * (double)n * (double)k);
+  } else {
+    dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
+    hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc);
+    prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
+  }
 }
 
 template <typename T>
@@ -203,6 +323,7 @@ static void gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, l
 template <typename T> struct PanelRows;
 template <> struct PanelRows<double2> { static const int n = 128; };
 template <> struct PanelRows<float2> { static const int n = 256; };
+template <> struct PanelRows<double> { static const int n = 256; };
 
 struct PanelScratch {
   unsigned *bar;   // grid barrier: [0] arrivals, [32] generation, [64] timeout flag (own cache lines)
@@ -248,6 +369,14 @@ __device__ inline void st_coh(double2 *p, double2 v)
   st_coh((unsigned long long *)p, (unsigned long long)__double_as_longlong(v.x));
   st_coh((unsigned long long *)p + 1, (unsigned long long)__double_as_longlong(v.y));
 }
+__device__ inline void st_coh(double *p, double v)
+{
+  st_coh((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
+}
+__device__ inline double ld_coh(const double *p)
+{
+  return __longlong_as_double((long long)ld_coh((const unsigned long long *)p));
+}
 __device__ inline double2 ld_coh(const double2 *p)
 {
   double2 v;
@@ -274,7 +403,7 @@ template <typename T>
 __global__ void __launch_bounds__(PanelRows<T>::n)
 k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax, unsigned seq0)
 {
-  typedef decltype(T().x) R;
+  typedef typename CT<T>::R R;
   constexpr int NR = PanelRows<T>::n;
   constexpr bool kPackKey = std::is_same<T, float2>::value;
   __shared__ T slab[PW][NR];           // column-major: slab[q][t] = A(lo + t, r0 + q)
@@ -310,7 +439,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
     const unsigned seq = seq0 + (unsigned)j;
     // local candidate: largest |re|+|im| among own rows >= c, first on ties
     R best = -1; int bi = -1;
-    if (t < nrow && i >= c) { T v = slab[j][t]; best = fabs(v.x) + fabs(v.y); bi = t; }
+    if (t < nrow && i >= c) { T v = slab[j][t]; best = mag1(v); bi = t; }
     for (int off = 32; off > 0; off >>= 1) {
       R ob = __shfl_xor(best, off);
       int oi = __shfl_xor(bi, off);
@@ -414,7 +543,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
     }
     __syncthreads();
     T piv = spiv[j];
-    const bool nz = (piv.x != 0 || piv.y != 0);
+    const bool nz = nonzero(piv);
     if (t < nrow && i > c) {
       T l = slab[j][t];
       if (nz) { l = cmul(l, crecip(piv)); slab[j][t] = l; }
@@ -430,7 +559,7 @@ k_panel(T *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScrat
     T x[PW];
 #pragma unroll
     for (int r = 0; r < PW; r++) {
-      T v; v.x = (r == t) ? 1 : 0; v.y = 0;
+      T v = (r == t) ? unit_v<T>() : zero_v<T>();
       if (r < w && t < w) {
 #pragma unroll
         for (int k = 0; k < r; k++) v = csub(v, cmul(slab[k][r], x[k]));
@@ -699,7 +828,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   float2 r[PW];
 #pragma unroll
   for (int q = 0; q < PW; q++) {
-    r[q].x = 0; r[q].y = 0;
+    r[q] = zero_v<float2>();
     if (t < nrow && q < w) r[q] = A[i + (r0 + q) * lda];
   }
   // every row's original values to the copy area (read by the merges and,
@@ -731,7 +860,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         sbase[k + 1] = sbase[k] + (k < nch ? ts_count(g + k * sp, sp, m, w) : 0);
       nr = sbase[TS_FAN];
 #pragma unroll
-      for (int q = 0; q < PW; q++) { r[q].x = 0; r[q].y = 0; }
+      for (int q = 0; q < PW; q++) { r[q] = zero_v<float2>(); }
       if (t < nr) {
         int k = 0;
 #pragma unroll
@@ -856,7 +985,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
       const long c = sfrom[t] >= 0 ? sfrom[t] : i;
       float2 v[PW];
 #pragma unroll
-      for (int q = 0; q < PW; q++) { v[q].x = 0; v[q].y = 0; }
+      for (int q = 0; q < PW; q++) { v[q] = zero_v<float2>(); }
       if (c == i) {
         // plain loads of the unchanged row (opaque address, as above)
         float2 *Ai = A + i + r0 * lda;
@@ -965,6 +1094,7 @@ static void laswp(T *A, long lda, long c0, long c1, long k1, long k2, const int 
 template <typename T> struct TsmCols;
 template <> struct TsmCols<double2> { static const int v = 64; };
 template <> struct TsmCols<float2> { static const int v = 128; };
+template <> struct TsmCols<double> { static const int v = 128; };
 
 template <typename T>
 __global__ void __launch_bounds__(TsmCols<T>::v)
@@ -1077,7 +1207,7 @@ k_trmm_strip(const T *Linv, int w, T *B, long ldb, long ncol)
     for (int u = 0; u < 8; u++) {
       int e = t + 256 * u, r = e & (PW - 1), c = e / PW;
       if (full || (r < w && c0 + c < ncol)) bv[u] = B[r + (c0 + c) * ldb];
-      else { bv[u].x = 0; bv[u].y = 0; }
+      else { bv[u] = zero_v<T>(); }
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -1094,7 +1224,7 @@ k_trmm_strip(const T *Linv, int w, T *B, long ldb, long ncol)
   const int r = t & (PW - 1), cg = t >> 5;
   T acc[8];
 #pragma unroll
-  for (int u = 0; u < 8; u++) { acc[u].x = 0; acc[u].y = 0; }
+  for (int u = 0; u < 8; u++) { acc[u] = zero_v<T>(); }
 #pragma unroll 8
   for (int k = 0; k < PW; k++) {
     const T a = Ls[r][k];
@@ -1143,8 +1273,7 @@ k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan,
   if (ci >= (c1 - c0) - (s1 - s0)) return;
   const long c = ci < nleft ? c0 + ci : s1 + (ci - nleft);
   const int cnt = plan[PW * 4];
-  T v;
-  v.x = 0; v.y = 0;
+  T v = zero_v<T>();
   long dst = -1;
   if (lane < cnt) {
     dst = plan[lane];
@@ -1156,8 +1285,7 @@ k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan,
     const int q = lane < w ? lane : 0;
 #pragma unroll
     for (int k = 0; k < PW; k++) lr[k] = Linv[q + k * PW];
-    T acc;
-    acc.x = 0; acc.y = 0;
+    T acc = zero_v<T>();
 #pragma unroll
     for (int k = 0; k < PW; k++) {
       if (k < w) {
@@ -1215,7 +1343,7 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
   T b[4];
 #pragma unroll
   for (int s = 0; s < 4; s++) {
-    b[s].x = 0; b[s].y = 0;
+    b[s] = zero_v<T>();
     if (lane + 64 * s < jb) b[s] = B[lane + 64 * s + c * ldb];
   }
 #pragma unroll
@@ -1231,8 +1359,7 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
       const int q = mine ? lane - l0 : 0;
 #pragma unroll
       for (int k = 0; k < PW; k++) lr[k] = Li[q + k * PW];
-      T acc;
-      acc.x = 0; acc.y = 0;
+      T acc = zero_v<T>();
 #pragma unroll
       for (int k = 0; k < PW; k++) {
         if (k < w) {
@@ -1322,13 +1449,15 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   // row, as the complex128 path)
   const char *piv_env = getenv("SCUFFX_MIXED_PIVOTING");
   const bool partial_env = piv_env && !strcmp(piv_env, "partial");
+  if (IsReal<T>::v) fast = 0;     // the explicit-inverse block rows are a complex64 option
   const bool tourn = fast && std::is_same<T, float2>::value && !partial_env;
   const void *panel_fn = tourn ? (const void *)k_panel_ts : (const void *)k_panel<T>;
   int dev = 0, ncu = 0, perCU = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, panel_fn, NR, 0);
-  if ((long)std::max(1, ncu) * std::max(1, perCU) < Gmax) return -5;   // panel would not be co-resident
+  // the tournament launches one more workgroup (it inverts the diagonal block)
+  if ((long)std::max(1, ncu) * std::max(1, perCU) < Gmax + (tourn ? 1 : 0)) return -5;   // not co-resident
   if (Gmax > 64 * KEYS_PER_LANE) return -5;                           // more workgroups than polled keys
   const long npanels = (n + PW - 1) / PW;
   // panel scratch, carved from one workspace: barrier words (3 cache
@@ -1383,7 +1512,8 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       unsigned seq0 = (unsigned)(j0 + 1);
       void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
                       (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0};
-      int tok = prof_begin(stream, std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
+      int tok = prof_begin(stream, IsReal<T>::v ? PK_PANEL_F64
+                                   : std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
       // the tournament panel has no grid barrier, only bounded waits along
       // its merge tree: an ordinary launch keeps it on the stream's own
       // hardware queue (a cooperative launch goes through a queue of its
@@ -1399,9 +1529,10 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       // this panel's interchanges on the block's other columns (fast: with
       // the product by the panel block's inverse on the columns right of it)
       long nr = K + jb - (j0 + w);
-      if (fast)
-        swap_trmm(A, lda, K, K + jb, j0, j0 + w, S.plan, (const T *)S.linv + (j0 / PW) * PW * PW, w, j0, stream);
-      else
+      if (fast) {
+        if constexpr (!IsReal<T>::v)
+          swap_trmm(A, lda, K, K + jb, j0, j0 + w, S.plan, (const T *)S.linv + (j0 / PW) * PW * PW, w, j0, stream);
+      } else
         laswp_plans(A, lda, K, K + jb, j0, j0 + w, S.plan, 1, stream);
       if (nr > 0) {
         if (!fast) trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
@@ -1419,9 +1550,10 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
     if (nnext > 0) {
       hipStreamWaitEvent(stream, evR, 0);
       laswp_plans(A, lda, c1, c2, 0, 0, plans + p0 * PLAN_LEN, bplans, stream);
-      if (fast)
-        trsm_blk(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c1 * lda, lda, nnext, stream);
-      else
+      if (fast) {
+        if constexpr (!IsReal<T>::v)
+          trsm_blk(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c1 * lda, lda, nnext, stream);
+      } else
         trsm_block(A, lda, K, (int)jb, A + K + c1 * lda, lda, nnext, stream);
       gemm_sub(n - c1, nnext, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, stream);
     }
@@ -1429,9 +1561,10 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
     if (c2 < n) {
       hipStreamWaitEvent(side, evP, 0);
       laswp_plans(A, lda, c2, n, 0, 0, plans + p0 * PLAN_LEN, bplans, side);
-      if (fast)
-        trsm_block_inv(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c2 * lda, lda, n - c2, side);
-      else
+      if (fast) {
+        if constexpr (!IsReal<T>::v)
+          trsm_block_inv(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c2 * lda, lda, n - c2, side);
+      } else
         trsm_block(A, lda, K, (int)jb, A + K + c2 * lda, lda, n - c2, side);
       // beside tournament panels: at most three trailing-GEMM workgroups
       // per CU (44 KB of LDS each), so that a panel workgroup (168 VGPRs
@@ -1510,15 +1643,15 @@ k_trsv_trans(const T *A, long lda, long n, T *B, long ldb, int upper_first, int 
     bool up = (pass == 0);
     for (long s = 0; s < n; s++) {
       long i = up ? s : n - 1 - s;
-      T acc; acc.x = 0; acc.y = 0;
-      if (up) { for (long r = tid; r < i; r += 256) { T a = A[r + i * lda]; if (conj_) a.y = -a.y; acc.x += a.x * b[r].x - a.y * b[r].y; acc.y += a.x * b[r].y + a.y * b[r].x; } }
-      else { for (long r = i + 1 + tid; r < n; r += 256) { T a = A[r + i * lda]; if (conj_) a.y = -a.y; acc.x += a.x * b[r].x - a.y * b[r].y; acc.y += a.x * b[r].y + a.y * b[r].x; } }
+      T acc = zero_v<T>();
+      if (up) { for (long r = tid; r < i; r += 256) { T a = conj_if(A[r + i * lda], conj_); mac(acc, a, b[r]); } }
+      else { for (long r = i + 1 + tid; r < n; r += 256) { T a = conj_if(A[r + i * lda], conj_); mac(acc, a, b[r]); } }
       red[tid] = acc;
       __syncthreads();
-      for (int q = 128; q > 0; q >>= 1) { if (tid < q) { red[tid].x += red[tid + q].x; red[tid].y += red[tid + q].y; } __syncthreads(); }
+      for (int q = 128; q > 0; q >>= 1) { if (tid < q) red[tid] = cadd(red[tid], red[tid + q]); __syncthreads(); }
       if (tid == 0) {
         T v = csub(b[i], red[0]);
-        if (up) { T d = A[i + i * lda]; if (conj_) d.y = -d.y; v = cdiv(v, d); }
+        if (up) { T d = conj_if(A[i + i * lda], conj_); v = cdiv(v, d); }
         b[i] = v;
       }
       __syncthreads();
@@ -1584,7 +1717,8 @@ __global__ void __launch_bounds__(256) k_perm_apply(const T *in, long ldi, T *ou
 // laswp with increment 1) or their inverse (reverse order); tmp holds n x
 // nrhs elements
 template <typename T>
-static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s);
+static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s,
+                           int *perm_buf = nullptr);
 
 // Triangular solves with few right-hand sides, in 64-row block steps.
 // The diagonal 64x64 blocks of L (unit lower) and U (upper) are inverted
@@ -1603,24 +1737,24 @@ k_diag_inv(const T *LU, long lda, long n, T *inv)
   for (int q = 0; q < TB; q++) {
     T v;
     if (t < bs && q < bs) v = LU[(kb + t) + (kb + q) * lda];
-    else { v.x = (t == q) ? 1 : 0; v.y = 0; }
+    else v = (t == q) ? unit_v<T>() : zero_v<T>();
     D[t][q] = v;
   }
   __syncthreads();
   const int j = t;
   if (LOWER) {
-    for (int r = 0; r < j; r++) { X[r][j].x = 0; X[r][j].y = 0; }
-    X[j][j].x = 1; X[j][j].y = 0;
+    for (int r = 0; r < j; r++) { X[r][j] = zero_v<T>(); }
+    X[j][j] = unit_v<T>();
     for (int r = j + 1; r < TB; r++) {
-      T acc; acc.x = 0; acc.y = 0;
+      T acc = zero_v<T>();
       for (int k = j; k < r; k++) acc = csub(acc, cmul(D[r][k], X[k][j]));
       X[r][j] = acc;
     }
   } else {
-    for (int r = j + 1; r < TB; r++) { X[r][j].x = 0; X[r][j].y = 0; }
+    for (int r = j + 1; r < TB; r++) { X[r][j] = zero_v<T>(); }
     X[j][j] = crecip(D[j][j]);
     for (int r = j - 1; r >= 0; r--) {
-      T acc; acc.x = 0; acc.y = 0;
+      T acc = zero_v<T>();
       for (int k = r + 1; k <= j; k++) acc = csub(acc, cmul(D[r][k], X[k][j]));
       X[r][j] = cdiv(acc, D[r][r]);
     }
@@ -1642,11 +1776,11 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
   T *b = Bin + (long)blockIdx.y * ldb;
   T *xo = Xout + (long)blockIdx.y * ldx;
   if (t < TB) {
-    if (t < bs) rs[t] = b[kb + t]; else { rs[t].x = 0; rs[t].y = 0; }
+    if (t < bs) rs[t] = b[kb + t]; else { rs[t] = zero_v<T>(); }
   }
   __syncthreads();
   if (t < TB) {
-    T acc; acc.x = 0; acc.y = 0;
+    T acc = zero_v<T>();
 #pragma unroll
     for (int k0 = 0; k0 < TB; k0 += 16) {
       T a[16];
@@ -1655,8 +1789,7 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
 #pragma unroll
       for (int u = 0; u < 16; u++) {
         T v = rs[k0 + u];
-        acc.x += a[u].x * v.x - a[u].y * v.y;
-        acc.y += a[u].x * v.y + a[u].y * v.x;
+        mac(acc, a[u], v);
       }
     }
     xs[t] = acc;
@@ -1666,8 +1799,7 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
   const long row = LOWER ? kb + bs + (long)blockIdx.x * 256 + t : (long)blockIdx.x * 256 + t;
   const long rend = LOWER ? n : kb;
   if (row < rend) {
-    T acc;
-    acc.x = 0; acc.y = 0;
+    T acc = zero_v<T>();
     const T *Lr = LU + row + kb * lda;
     T bo = b[row];
     if (bs == TB) {
@@ -1680,15 +1812,13 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
 #pragma unroll
         for (int u = 0; u < 16; u++) {
           T v = xs[i0 + u];
-          acc.x += a[u].x * v.x - a[u].y * v.y;
-          acc.y += a[u].x * v.y + a[u].y * v.x;
+          mac(acc, a[u], v);
         }
       }
     } else {
       for (int i = 0; i < bs; i++) {
         T a = Lr[(long)i * lda], v = xs[i];
-        acc.x += a.x * v.x - a.y * v.y;
-        acc.y += a.x * v.y + a.y * v.x;
+        mac(acc, a, v);
       }
     }
     b[row] = csub(bo, acc);
@@ -1717,8 +1847,7 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
   const long g = blockIdx.x;
   const long row = g * TB + r;
   const int bs = (int)std::min<long>(TB, n - g * TB);
-  T b;
-  b.x = 0; b.y = 0;
+  T b = zero_v<T>();
   if (q == 0 && r < bs) b = bin[row];
   // blocks k in solve order before g
   const long nk = LOWER ? g : nblk - 1 - g;
@@ -1728,7 +1857,7 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
     for (int u = 0; u < 16; u++) {
       const long col = k * TB + q * 16 + u;
       if (r < bs && col < n) a[u] = LU[row + col * lda];
-      else { a[u].x = 0; a[u].y = 0; }
+      else { a[u] = zero_v<T>(); }
     }
   };
   // the inverse diagonal block, in the same registers once no block is left
@@ -1759,14 +1888,12 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
     for (int u = 0; u < 16; u++) {
       const long col = k * TB + q * 16 + u;
       if (col < n) x[u] = ld_coh(xout + col);
-      else { x[u].x = 0; x[u].y = 0; }
+      else { x[u] = zero_v<T>(); }
     }
-    T acc;
-    acc.x = 0; acc.y = 0;
+    T acc = zero_v<T>();
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-      acc.x += a[u].x * x[u].x - a[u].y * x[u].y;
-      acc.y += a[u].x * x[u].y + a[u].y * x[u].x;
+      mac(acc, a[u], x[u]);
     }
     if (i + 1 < nk) load(blk(i + 1), a);
     else load_inv(a);
@@ -1782,13 +1909,11 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
   // own block: x_g = inv(D_g) b_g
   if (q == 0) bx[r] = b;
   __syncthreads();
-  T acc;
-  acc.x = 0; acc.y = 0;
+  T acc = zero_v<T>();
 #pragma unroll
   for (int u = 0; u < 16; u++) {
     const T m = a[u], v = bx[q * 16 + u];
-    acc.x += m.x * v.x - m.y * v.y;
-    acc.y += m.x * v.y + m.y * v.x;
+    mac(acc, m, v);
   }
   part[q][r] = acc;
   __syncthreads();
@@ -1806,7 +1931,8 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
 // two cooperative launches; returns -1 if it cannot run (grid not
 // co-resident, workspace), -6 on a time-out
 template <typename T>
-static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *invU, T *B, T *W, hipStream_t s)
+static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *invU, T *B, T *W, hipStream_t s,
+                      unsigned long long *words)
 {
   const long nblk = (n + TB - 1) / TB;
   static int ncu = -1, perCU = 0;
@@ -1817,7 +1943,7 @@ static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *inv
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_trsv_chain<T, true>, 256, 0);
   }
   if ((long)ncu * perCU < nblk) return -1;
-  char *w = (char *)ws_get(WS_CHAIN, sizeof(unsigned long long) * (2 * nblk + 8));
+  char *w = words ? (char *)words : (char *)ws_get(WS_CHAIN, sizeof(unsigned long long) * (2 * nblk + 8));
   if (!w) return -1;
   unsigned *err = (unsigned *)w;
   unsigned long long *fl = (unsigned long long *)(w + 64), *fu = fl + nblk;
@@ -1840,9 +1966,10 @@ static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *inv
 }
 
 template <typename T>
-static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s)
+static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s,
+                           int *perm_buf)
 {
-  int *perm = n <= PERM_LDS_MAX ? (int *)ws_get(WS_PERM, sizeof(int) * n) : nullptr;
+  int *perm = n > PERM_LDS_MAX ? nullptr : perm_buf ? perm_buf : (int *)ws_get(WS_PERM, sizeof(int) * n);
   if (!perm || !tmp) {
     // beyond the LDS copy: the interchanges one after another, per column
     if (fwd) laswp(B, ldb, 0, nrhs, 0, n, ipiv, s);
@@ -1857,22 +1984,49 @@ static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, b
 }
 
 template <typename T>
+size_t solve_ws_bytes(long n)
+{
+  auto r256 = [](size_t b) { return 256 * ((b + 255) / 256); };
+  const long nblk = (n + TB - 1) / TB;
+  return r256(sizeof(T) * n) + 2 * r256(sizeof(T) * nblk * TB * TB) + r256(sizeof(int) * n) +
+         r256(sizeof(unsigned long long) * (2 * nblk + 8));
+}
+
+template <typename T>
+SolveWs<T> solve_ws_carve(char *base, long n)
+{
+  auto r256 = [](size_t b) { return 256 * ((b + 255) / 256); };
+  const long nblk = (n + TB - 1) / TB;
+  SolveWs<T> w;
+  w.W = (T *)base; base += r256(sizeof(T) * n);
+  w.invL = (T *)base; base += r256(sizeof(T) * nblk * TB * TB);
+  w.invU = (T *)base; base += r256(sizeof(T) * nblk * TB * TB);
+  w.perm = (int *)base; base += r256(sizeof(int) * n);
+  w.chain = (unsigned long long *)base;
+  return w;
+}
+
+template <typename T>
 int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B, long ldb,
-             hipStream_t stream)
+             hipStream_t stream, SolveWs<T> *ws)
 {
   if (n < 0 || nrhs < 0 || lda < std::max(1L, n) || ldb < std::max(1L, n)) return -1;
   if (n == 0 || nrhs == 0) return 0;
+  if (ws && !((trans == 'N' || trans == 'n') && nrhs == 1)) return -1;
   if ((trans == 'N' || trans == 'n') && nrhs <= 16) {
     const long nblk = (n + TB - 1) / TB;
-    T *W = (T *)ws_get(WS_SOLVE_W, sizeof(T) * n * nrhs);
-    T *invL = (T *)ws_get(WS_SOLVE_INVL, sizeof(T) * nblk * TB * TB);
-    T *invU = (T *)ws_get(WS_SOLVE_INVU, sizeof(T) * nblk * TB * TB);
+    T *W = ws ? ws->W : (T *)ws_get(WS_SOLVE_W, sizeof(T) * n * nrhs);
+    T *invL = ws ? ws->invL : (T *)ws_get(WS_SOLVE_INVL, sizeof(T) * nblk * TB * TB);
+    T *invU = ws ? ws->invU : (T *)ws_get(WS_SOLVE_INVU, sizeof(T) * nblk * TB * TB);
     if (!W || !invL || !invU) return -2;
-    hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL);
-    hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU);
-    apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream);
+    if (!ws || !ws->inv_ready) {
+      hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL);
+      hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU);
+      if (ws) ws->inv_ready = true;
+    }
+    apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream, ws ? ws->perm : nullptr);
     if (nrhs == 1) {
-      int rc = trsv_chain(LU, lda, n, invL, invU, B, W, stream);
+      int rc = trsv_chain(LU, lda, n, invL, invU, B, W, stream, ws ? ws->chain : nullptr);
       if (rc != -1) return rc;
     }
     // L y = P b: y accumulates in W, B holds the running right-hand side
@@ -1925,7 +2079,7 @@ __global__ void __launch_bounds__(256) k_logdet(const T *LU, long n, long lda, c
   double a = 0, b = 0;
   for (long i = threadIdx.x; i < n; i += 256) {
     T d = LU[i + i * lda];
-    double re = d.x, im = d.y;
+    double re = re_of(d), im = im_of(d);
     a += 0.5 * log(re * re + im * im);
     b += atan2(im, re);
     if (ipiv[i] != i + 1) b += 3.14159265358979323846;
@@ -1945,7 +2099,7 @@ __global__ void __launch_bounds__(256) k_trace(const T *A, long n, long lda, dou
 {
   __shared__ double s0[256], s1[256];
   double a = 0, b = 0;
-  for (long i = threadIdx.x; i < n; i += 256) { T d = A[i + i * lda]; a += d.x; b += d.y; }
+  for (long i = threadIdx.x; i < n; i += 256) { T d = A[i + i * lda]; a += re_of(d); b += im_of(d); }
   s0[threadIdx.x] = a; s1[threadIdx.x] = b;
   __syncthreads();
   for (int q = 128; q > 0; q >>= 1) {
@@ -1958,7 +2112,7 @@ __global__ void __launch_bounds__(256) k_trace(const T *A, long n, long lda, dou
 template <typename T>
 int lu_logdet(const T *LU, long n, long lda, const int *ipiv, double out[2], hipStream_t s)
 {
-  double *d = (double *)ws_get(WS_REFINE, 256);
+  double *d = (double *)ws_get(WS_SMALL, 256);
   if (!d) return -2;
   hipLaunchKernelGGL(k_logdet<T>, dim3(1), dim3(256), 0, s, LU, n, lda, ipiv, d);
   hipMemcpyAsync(out, d, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -1968,7 +2122,7 @@ int lu_logdet(const T *LU, long n, long lda, const int *ipiv, double out[2], hip
 template <typename T>
 int mat_trace(const T *A, long n, long lda, double out[2], hipStream_t s)
 {
-  double *d = (double *)ws_get(WS_REFINE, 256);
+  double *d = (double *)ws_get(WS_SMALL, 256);
   if (!d) return -2;
   hipLaunchKernelGGL(k_trace<T>, dim3(1), dim3(256), 0, s, A, n, lda, d);
   hipMemcpyAsync(out, d, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -1976,13 +2130,24 @@ int mat_trace(const T *A, long n, long lda, double out[2], hipStream_t s)
 }
 
 template int lu_logdet<double2>(const double2 *, long, long, const int *, double *, hipStream_t);
+template int lu_logdet<double>(const double *, long, long, const int *, double *, hipStream_t);
+template int mat_trace<double>(const double *, long, long, double *, hipStream_t);
+template int lu_factor<double>(double *, long, long, int *, int *, hipStream_t, int, int);
+template int lu_solve<double>(const double *, long, long, const int *, char, long, double *, long, hipStream_t,
+                              SolveWs<double> *);
 template int lu_logdet<float2>(const float2 *, long, long, const int *, double *, hipStream_t);
 template int mat_trace<double2>(const double2 *, long, long, double *, hipStream_t);
 template int mat_trace<float2>(const float2 *, long, long, double *, hipStream_t);
 template int lu_factor<double2>(double2 *, long, long, int *, int *, hipStream_t, int, int);
 template int lu_factor<float2>(float2 *, long, long, int *, int *, hipStream_t, int, int);
-template int lu_solve<double2>(const double2 *, long, long, const int *, char, long, double2 *, long, hipStream_t);
-template int lu_solve<float2>(const float2 *, long, long, const int *, char, long, float2 *, long, hipStream_t);
+template int lu_solve<double2>(const double2 *, long, long, const int *, char, long, double2 *, long, hipStream_t,
+                               SolveWs<double2> *);
+template int lu_solve<float2>(const float2 *, long, long, const int *, char, long, float2 *, long, hipStream_t,
+                              SolveWs<float2> *);
+template size_t solve_ws_bytes<double2>(long);
+template size_t solve_ws_bytes<float2>(long);
+template SolveWs<double2> solve_ws_carve<double2>(char *, long);
+template SolveWs<float2> solve_ws_carve<float2>(char *, long);
 
 // ---------------------------------------------------------------------
 // refinement helpers
@@ -2009,8 +2174,10 @@ __global__ void k_resid_final(long n, int nch, const double2 *b, const double2 *
   if (i >= n) return;
   double sr = 0, si = 0;
   for (int q = 0; q < nch; q++) { sr += work[q * n + i].x; si += work[q * n + i].y; }
-  r[i] = make_double2(b[i].x - sr, b[i].y - si);
+  r[i] = b ? make_double2(b[i].x - sr, b[i].y - si) : make_double2(sr, si);
 }
+
+long residual_chunks(long n) { return (n + RES_COLCHUNK - 1) / RES_COLCHUNK; }
 
 int residual_c128(const double2 *A, long n, long lda, const double2 *x, const double2 *b, double2 *r,
                   double2 *work, hipStream_t stream)
@@ -2037,6 +2204,100 @@ int convert_c128_to_c64(const double2 *x, float2 *y, long rows, long cols, long 
   hipLaunchKernelGGL(k_c128_to_c64, dim3((unsigned)((rows + 255) / 256), gy), dim3(256), 0, stream, x, y, rows, cols, ldx, ldy);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
+
+// real parts of a complex128 block into a real matrix (column-major, or
+// packed upper storage of a square block: dst[r + c (c+1)/2], r <= c)
+__global__ void k_re_part(const double2 *x, long ldx, double *y, long ldy, long rows, long cols, int packed)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long c = blockIdx.y; c < cols; c += gridDim.y) {
+    if (i >= rows) return;
+    const double v = x[i + c * ldx].x;
+    if (!packed) y[i + c * ldy] = v;
+    else if (i <= c) y[i + c * (c + 1) / 2] = v;
+  }
+}
+
+int real_part_c128(const double2 *x, long ldx, double *y, long ldy, long rows, long cols, int packed,
+                   hipStream_t stream)
+{
+  if (rows <= 0 || cols <= 0) return 0;
+  unsigned gy = (unsigned)std::min<long>(cols, 65535);
+  hipLaunchKernelGGL(k_re_part, dim3((unsigned)((rows + 255) / 256), gy), dim3(256), 0, stream, x, ldx, y, ldy, rows,
+                     cols, packed);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// full n x n matrix from packed upper storage (symmetric: A(r,c) = A(c,r))
+template <typename T>
+__global__ void k_unpack_sym(const T *P, T *A, long lda, long n)
+{
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long c = blockIdx.y; c < n; c += gridDim.y) {
+    if (r >= n) return;
+    const long lo = r < c ? r : c, hi = r < c ? c : r;
+    A[r + c * lda] = P[lo + hi * (hi + 1) / 2];
+  }
+}
+
+template <typename T>
+int unpack_symmetric(const T *P, T *A, long lda, long n, hipStream_t stream)
+{
+  if (n <= 0) return 0;
+  unsigned gy = (unsigned)std::min<long>(n, 65535);
+  hipLaunchKernelGGL(k_unpack_sym<T>, dim3((unsigned)((n + 255) / 256), gy), dim3(256), 0, stream, P, A, lda, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+template int unpack_symmetric<double>(const double *, double *, long, long, hipStream_t);
+template int unpack_symmetric<double2>(const double2 *, double2 *, long, long, hipStream_t);
+template int unpack_symmetric<float2>(const float2 *, float2 *, long, long, hipStream_t);
+
+// identity matrix (n x n, column-major)
+template <typename T>
+__global__ void k_identity(T *A, long lda, long n)
+{
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long c = blockIdx.y; c < n; c += gridDim.y) {
+    if (r >= n) return;
+    A[r + c * lda] = r == c ? unit_v<T>() : zero_v<T>();
+  }
+}
+
+// first zero diagonal entry (1-based) of U, or 0
+template <typename T>
+__global__ void k_first_zero_diag(const T *LU, long lda, long n, int *out)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (!nonzero(LU[i + i * lda])) atomicMin(out, (int)(i + 1));
+}
+
+// LUInvert (zgetri / dgetri semantics): A <- A^{-1} from its LU factors
+// (unit-lower L, U, ipiv), as the solution of A X = I with nrhs = n; W holds
+// n x n elements plus one int.  info = first zero diagonal of U (A unchanged)
+template <typename T>
+int lu_invert(T *A, long n, long lda, const int *ipiv, T *W, int *info, hipStream_t s)
+{
+  *info = 0;
+  if (n <= 0) return 0;
+  int *dz = (int *)(W + n * n);
+  const int big = 0x7F7F7F7F;          // every byte 0x7F: larger than any index
+  hipMemsetAsync(dz, 0x7F, sizeof(int), s);
+  hipLaunchKernelGGL(k_first_zero_diag<T>, dim3((unsigned)std::min<long>(1024, (n + 255) / 256)), dim3(256), 0, s, A,
+                     lda, n, dz);
+  int z = 0;
+  hipMemcpyAsync(&z, dz, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return -3;
+  if (z != big) { *info = z; return 0; }
+  unsigned gy = (unsigned)std::min<long>(n, 65535);
+  hipLaunchKernelGGL(k_identity<T>, dim3((unsigned)((n + 255) / 256), gy), dim3(256), 0, s, W, n, n);
+  int rc = lu_solve<T>(A, n, lda, ipiv, 'N', n, W, n, s);
+  if (rc) return rc;
+  hipMemcpy2DAsync(A, sizeof(T) * lda, W, sizeof(T) * n, sizeof(T) * n, n, hipMemcpyDeviceToDevice, s);
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : -3;
+}
+template int lu_invert<double2>(double2 *, long, long, const int *, double2 *, int *, hipStream_t);
+template int lu_invert<float2>(float2 *, long, long, const int *, float2 *, int *, hipStream_t);
+template int lu_invert<double>(double *, long, long, const int *, double *, int *, hipStream_t);
 
 __global__ void k_c64_to_c128(const float2 *x, double2 *y, long n)
 {

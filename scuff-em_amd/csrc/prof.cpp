@@ -16,7 +16,7 @@ std::vector<hipEvent_t> g_pool;
 double g_ms[PK_COUNT], g_work[PK_COUNT];
 long g_n[PK_COUNT];
 const char *g_names[PK_COUNT] = {"gemm_c128", "gemm_c64", "panel_c128", "panel_c64", "trsm_c128", "trsm_c64",
-                                 "far", "near", "self", "stamp", "fippi", "resid", "grad"};
+                                 "far", "near", "self", "stamp", "fippi", "resid", "grad", "gemm_f64", "panel_f64"};
 
 hipEvent_t take()
 {

@@ -10,7 +10,7 @@ namespace scuffx {
 
 enum ProfKernel {
   PK_GEMM_C128 = 0, PK_GEMM_C64, PK_PANEL_C128, PK_PANEL_C64, PK_TRSM_C128, PK_TRSM_C64,
-  PK_FAR, PK_NEAR, PK_SELF, PK_STAMP, PK_FIPPI, PK_RESID, PK_GRAD, PK_COUNT
+  PK_FAR, PK_NEAR, PK_SELF, PK_STAMP, PK_FIPPI, PK_RESID, PK_GRAD, PK_GEMM_F64, PK_PANEL_F64, PK_COUNT
 };
 
 bool prof_enabled();

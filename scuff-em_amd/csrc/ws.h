@@ -18,11 +18,22 @@ enum WsSlot {
   WS_PERM,         // triangular solve: composed row permutation of ipiv
   WS_PERM_TMP,     // ... and the permuted right-hand sides (many-column solves)
   WS_CHAIN,        // one-launch triangular solves: error word and block flags
+  WS_GMRES,        // GMRES refinement: Krylov basis and vectors
+  WS_SMALL,        // reductions with a host result (log det, trace, norms)
+  WS_STAGE,        // staging of blocks (real blocks, LU inverse, unpacked storage)
   WS_COUNT
 };
 
 // device pointer of at least `bytes` bytes for `slot` on the current device
-// (nullptr if the allocation fails)
+// (nullptr if the allocation fails or the library was shut down)
 void *ws_get(int slot, size_t bytes);
+
+// frees every workspace of every thread and device (at shutdown)
+void ws_release_all();
+
+// after scuffx_shutdown the library makes no further HIP calls: device
+// memory still held by live objects is left to the process exit
+bool hip_down();
+void set_hip_down();
 
 }  // namespace scuffx

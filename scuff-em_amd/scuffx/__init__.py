@@ -24,6 +24,7 @@ _LIBPATH = os.path.join(_HERE, "libscuffx.so")
 _LIB = None
 
 ZVAC = 376.73031346177
+NOT_CONVERGED = 1      # SCUFFX_NOT_CONVERGED
 
 
 class SCUFFXError(RuntimeError):
@@ -92,6 +93,14 @@ def lib():
             "scuffx_lu32_from_c128": ([vp, C.c_long, C.c_long, vp, C.c_long, vp], C.c_int),
             "scuffx_refine": ([vp, C.c_long, C.c_long, vp, C.c_long, vp, vp, vp, C.c_double, C.c_int,
                                ip, dp], C.c_int),
+            "scuffx_gmres_ir": ([vp, C.c_long, C.c_long, vp, C.c_long, vp, vp, vp, C.c_double, C.c_int,
+                                 ip, dp], C.c_int),
+            "scuffx_shutdown": ([], C.c_int),
+            "scuffx_assemble_real": ([vp, C.c_double, C.c_double, vp, C.c_long, C.c_int], C.c_int),
+            "scuffx_assemble_block_real": ([vp, C.c_int, C.c_int, C.c_double, C.c_double, vp, C.c_long, C.c_int,
+                                            vp, C.c_long, C.c_long, C.c_int, vp, vp], C.c_int),
+            "scuffx_unpack_symmetric": ([vp, C.c_long, vp, C.c_long, C.c_int], C.c_int),
+            "scuffx_lu_invert": ([vp, C.c_long, C.c_long, vp, C.c_int], C.c_int),
             "scuffx_rhs_planewave": ([vp, C.c_double, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_power": ([vp, C.c_double, C.c_double, vp, vp, dp, dp], C.c_int),
             "scuffx_lu_logdet": ([vp, C.c_long, C.c_long, vp, C.c_int, dp, dp], C.c_int),
@@ -108,7 +117,21 @@ def lib():
             f.argtypes = args
             f.restype = res
         _LIB = L
+        # release device memory while the HIP runtime is still up: at exit,
+        # C++ destructors of the library (and __del__ of live objects) run
+        # after the runtime -- or a profiler attached to it -- is torn down.
+        # torch was imported first, so this handler runs before its own.
+        import atexit
+        atexit.register(_shutdown)
     return _LIB
+
+
+def _shutdown():
+    # every device call goes through _sync_stream, which initializes torch's
+    # HIP context first: without it the library holds no device memory
+    import torch
+    if _LIB is not None and torch.cuda.is_initialized():
+        _LIB.scuffx_shutdown()
 
 
 def _check(rc):
@@ -162,14 +185,30 @@ def CreateGammaMatrix(axis):
     return np.array([[0.0, -u[2], u[1]], [u[2], 0.0, -u[0]], [-u[1], u[0], 0.0]])
 
 
+_PREC = {"complex128": 64, "complex64": 32, "float64": 8}
+
+
+def _prec_of(dtype):
+    import torch
+    if dtype in _PREC:
+        return _PREC[dtype]
+    return {torch.complex128: 64, torch.complex64: 32, torch.float64: 8}[dtype]
+
+
+def _torch_dtype(prec):
+    import torch
+    return {64: torch.complex128, 32: torch.complex64, 8: torch.float64}[prec]
+
+
 class HVector:
-    """Dense complex vector in GPU memory (libs/libhmat/libhmat.h:71-150)."""
+    """Dense vector in GPU memory (libs/libhmat/libhmat.h:71-150):
+    complex128 by default, complex64, or float64 (LHM_REAL)."""
 
     def __init__(self, N, dtype="complex128", data=None):
         torch = _torch()
-        tdt = torch.complex128 if dtype in ("complex128", torch.complex128) else torch.complex64
         self.N = N
-        self.t = torch.zeros(N, dtype=tdt, device="cuda") if data is None else data
+        self.prec = _prec_of(data.dtype if data is not None else dtype)
+        self.t = torch.zeros(N, dtype=_torch_dtype(self.prec), device="cuda") if data is None else data
 
     def numpy(self):
         return self.t.cpu().numpy()
@@ -183,39 +222,74 @@ class HVector:
         """HVector(FileName, LHM_HDF5, Name) (HDF5IO.cc:497-540)"""
         torch = _torch()
         v = ImportFromHDF5(FileName, Name, Vector=True)
-        return cls(len(v), data=torch.from_numpy(v.astype(np.complex128)).cuda())
+        return cls(len(v), data=torch.from_numpy(v).cuda())
 
     def GetEntry(self, n):
         return complex(self.t[n].item())
+
+    def GetEntryD(self, n):
+        return complex(self.t[n].item()).real
 
     def SetEntry(self, n, v):
         self.t[n] = v
 
 
 class HMatrix:
-    """Dense column-major complex matrix in GPU memory with LU factors and
-    pivots (libs/libhmat/libhmat.h:211-394).  Storage is a torch tensor
-    of shape (NC, NR): t[c, r] is entry (r, c)."""
+    """Dense column-major matrix in GPU memory with LU factors and pivots
+    (libs/libhmat/libhmat.h:211-394).  dtype "complex128" (LHM_COMPLEX,
+    default), "complex64" or "float64" (LHM_REAL).  Storage "normal": a
+    torch tensor of shape (NC, NR), t[c, r] is entry (r, c); "symmetric"
+    (LHM_SYMMETRIC, square only): packed upper triangle, t[r + c(c+1)/2] for
+    r <= c (HMatrix.cc:325-342).  A packed matrix is factored as a full copy
+    with partial pivoting (LU), not by dsptrf/zsptrf's Bunch-Kaufman LDL^T;
+    LUSolve, LogDet and TraceSolve use that copy."""
 
-    def __init__(self, NR, NC, dtype="complex128", data=None):
+    def __init__(self, NR, NC, dtype="complex128", data=None, Storage="normal"):
         torch = _torch()
         self.NR, self.NC = NR, NC
-        self.prec = 64 if dtype in ("complex128", torch.complex128) else 32
-        tdt = torch.complex128 if self.prec == 64 else torch.complex64
-        self.t = torch.zeros((NC, NR), dtype=tdt, device="cuda") if data is None else data
+        self.prec = _prec_of(data.dtype if data is not None else dtype)
+        self.packed = (Storage == "symmetric")
+        if self.packed and NR != NC:
+            self.packed = False      # as HMatrix.cc:94-96 does for non-square matrices
+        shape = (NR * (NR + 1) // 2,) if self.packed else (NC, NR)
+        self.t = torch.zeros(shape, dtype=_torch_dtype(self.prec), device="cuda") if data is None else data
         self.ipiv = None
         self.factored = False
+        self._full = None            # full working copy of a packed matrix (its factors)
+
+    @property
+    def RealComplex(self):
+        return "LHM_REAL" if self.prec == 8 else "LHM_COMPLEX"
 
     @classmethod
-    def from_numpy(cls, A, dtype="complex128"):
+    def from_numpy(cls, A, dtype=None, Storage="normal"):
         torch = _torch()
         A = np.asarray(A)
-        M = cls(A.shape[0], A.shape[1], dtype=dtype)
-        M.t.copy_(torch.from_numpy(np.ascontiguousarray(A.T)).to(M.t.dtype))
+        if dtype is None:
+            dtype = "float64" if not np.iscomplexobj(A) else "complex128"
+        M = cls(A.shape[0], A.shape[1], dtype=dtype, Storage=Storage)
+        if M.packed:
+            iu = np.triu_indices(A.shape[0])
+            order = np.lexsort((iu[0], iu[1]))           # column-major upper triangle
+            M.t.copy_(torch.from_numpy(np.ascontiguousarray(A[iu[0][order], iu[1][order]])).to(M.t.dtype))
+        else:
+            M.t.copy_(torch.from_numpy(np.ascontiguousarray(A.T)).to(M.t.dtype))
         return M
 
+    def _index(self, r, c):
+        if r > c:
+            r, c = c, r
+        return r + c * (c + 1) // 2
+
     def numpy(self):
-        return self.t.cpu().numpy().T.copy()
+        if not self.packed:
+            return self.t.cpu().numpy().T.copy()
+        p = self.t.cpu().numpy()
+        n = self.NR
+        A = np.zeros((n, n), dtype=p.dtype)
+        for c in range(n):
+            A[:c + 1, c] = p[c * (c + 1) // 2:c * (c + 1) // 2 + c + 1]
+        return np.triu(A) + np.triu(A, 1).T
 
     def ExportToHDF5(self, dest, Name):
         """HMatrix::ExportToHDF5 (HDF5IO.cc:171-250)"""
@@ -224,14 +298,33 @@ class HMatrix:
     @classmethod
     def FromHDF5(cls, FileName, Name):
         """HMatrix(FileName, LHM_HDF5, Name) (HDF5IO.cc:252-418)"""
-        return cls.from_numpy(ImportFromHDF5(FileName, Name).astype(np.complex128))
+        return cls.from_numpy(ImportFromHDF5(FileName, Name))
 
     def GetEntry(self, r, c):
-        return complex(self.t[c, r].item())
+        v = self.t[self._index(r, c)] if self.packed else self.t[c, r]
+        return complex(v.item())
+
+    def GetEntryD(self, r, c):
+        return self.GetEntry(r, c).real
 
     def SetEntry(self, r, c, v):
-        self.t[c, r] = v
+        if self.packed:
+            self.t[self._index(r, c)] = v
+        else:
+            self.t[c, r] = v
         self.factored = False
+
+    def _work(self):
+        """the matrix the factorization works on: itself, or the full copy
+        of packed storage"""
+        if not self.packed:
+            return self.t
+        torch = _torch()
+        if self._full is None:
+            self._full = torch.empty((self.NC, self.NR), dtype=self.t.dtype, device="cuda")
+        _sync_stream()
+        _check(lib().scuffx_unpack_symmetric(_ptr(self.t), self.NR, _ptr(self._full), self.NR, self.prec))
+        return self._full
 
     def LUFactorize(self):
         """In-place LU with partial pivoting; returns LAPACK info."""
@@ -240,10 +333,16 @@ class HMatrix:
             raise SCUFFXError("LUFactorize needs a square matrix")
         if self.ipiv is None:
             self.ipiv = torch.zeros(self.NR, dtype=torch.int32, device="cuda")
+        A = self._work()
         _sync_stream()
-        info = _check(lib().scuffx_lu_factor(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), self.prec))
+        info = _check(lib().scuffx_lu_factor(_ptr(A), self.NR, self.NR, _ptr(self.ipiv), self.prec))
         self.factored = True
         return info
+
+    def _factors(self):
+        if not self.factored:
+            raise SCUFFXError("LUFactorize() must be called first")
+        return self._full if self.packed else self.t
 
     def LUSolve(self, X, Trans="N", nrhs=None):
         """Overwrites X (HVector or HMatrix) with op(A)^{-1} X."""
@@ -252,14 +351,34 @@ class HMatrix:
         if isinstance(X, HVector):
             t, n_rhs, ldb = X.t, 1, X.N
         else:
+            if X.packed:
+                raise SCUFFXError("LUSolve: the right-hand sides must be in normal storage")
             t, n_rhs, ldb = X.t, (X.NC if nrhs is None else nrhs), X.NR
         if t.dtype != self.t.dtype:
-            raise SCUFFXError("LUSolve: precision of the right-hand side differs from the factors")
+            raise SCUFFXError("LUSolve: type mismatch between the factors and the right-hand side")
+        if Trans != "N" and self.packed:
+            raise SCUFFXError("transposed LU-solves not available for packed matrices")
         _sync_stream()
-        _check(lib().scuffx_lu_solve(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), Trans.encode()[:1],
+        _check(lib().scuffx_lu_solve(_ptr(self._factors()), self.NR, self.NR, _ptr(self.ipiv), Trans.encode()[:1],
                                      n_rhs, _ptr(t), ldb, self.prec))
         return 0
 
+    def LUInvert(self):
+        """Replaces the matrix with its inverse, after LUFactorize()
+        (HMatrix::LUInvert, LBWrappers.cc:251-335); returns LAPACK info."""
+        if not self.factored:
+            raise SCUFFXError("LUFactorize() must be called before LUInvert()")
+        F = self._factors()
+        _sync_stream()
+        info = _check(lib().scuffx_lu_invert(_ptr(F), self.NR, self.NR, _ptr(self.ipiv), self.prec))
+        if info == 0 and self.packed:
+            # the inverse of a symmetric matrix is symmetric: back to packed storage
+            torch = _torch()
+            iu = torch.triu_indices(self.NR, self.NR, device="cuda")
+            order = torch.argsort(iu[1] * self.NR + iu[0])
+            self.t.copy_(F[iu[1][order], iu[0][order]])
+        self.factored = False
+        return info
 
     def LogDet(self):
         """log det of the factored matrix as a complex number
@@ -268,21 +387,21 @@ class HMatrix:
             raise SCUFFXError("LogDet needs LUFactorize() first")
         la, ph = C.c_double(), C.c_double()
         _sync_stream()
-        _check(lib().scuffx_lu_logdet(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), self.prec,
+        _check(lib().scuffx_lu_logdet(_ptr(self._factors()), self.NR, self.NR, _ptr(self.ipiv), self.prec,
                                       C.byref(la), C.byref(ph)))
         return complex(la.value, ph.value)
 
     def TraceSolve(self, D):
         """sum of the first D.NC diagonal entries of A^{-1} D with the
-        factors of A; D (NR x NC with NC <= NR, same precision) is
-        overwritten by A^{-1} D."""
+        factors of A; D (NR x NC with NC <= NR, same type) is overwritten by
+        A^{-1} D."""
         if not self.factored:
             raise SCUFFXError("TraceSolve needs LUFactorize() first")
-        if D.NR != self.NR or D.NC > self.NR or D.prec != self.prec:
-            raise SCUFFXError("TraceSolve: D must have NR rows, at most NR columns and the same precision")
+        if D.NR != self.NR or D.NC > self.NR or D.prec != self.prec or D.packed:
+            raise SCUFFXError("TraceSolve: D must have NR rows, at most NR columns and the same type")
         tr, ti = C.c_double(), C.c_double()
         _sync_stream()
-        _check(lib().scuffx_trace_solve(_ptr(self.t), self.NR, self.NR, _ptr(self.ipiv), _ptr(D.t), D.NR,
+        _check(lib().scuffx_trace_solve(_ptr(self._factors()), self.NR, self.NR, _ptr(self.ipiv), _ptr(D.t), D.NR,
                                         D.NC, self.prec, C.byref(tr), C.byref(ti)))
         return complex(tr.value, ti.value)
 
@@ -569,9 +688,13 @@ class RWGGeometry:
         _check(lib().scuffx_geom_vertices(self.h, ns, out.ctypes.data_as(C.c_void_p)))
         return out
 
-    def AllocateBEMMatrix(self):
+    def AllocateBEMMatrix(self, PureImagFreq=False, Packed=False):
+        """RWGGeometry::AllocateBEMMatrix (AssembleBEMMatrix.cc:778-783): a
+        real matrix (LHM_REAL) for pure imaginary frequencies of a compact
+        geometry, else complex; Packed gives symmetric packed storage."""
         N = self.TotalBFs
-        return HMatrix(N, N)
+        return HMatrix(N, N, dtype="float64" if PureImagFreq else "complex128",
+                       Storage="symmetric" if Packed else "normal")
 
     def BuildFIPPITable(self):
         _sync_stream()
@@ -585,16 +708,31 @@ class RWGGeometry:
         return out
 
     def AssembleBEMMatrix(self, Omega, M=None):
+        """fills M (AssembleBEMMatrix.cc:684-773).  A complex128 matrix gets
+        the entries; a real one (AllocateBEMMatrix(PureImagFreq=True)) their
+        real parts, as AddEntry on a real HMatrix keeps (HMatrix.cc:419-435);
+        packed storage keeps the upper triangle."""
         N = self.TotalBFs
         if M is None:
             M = HMatrix(N, N)
-        elif M.NR != N or M.NC != N or M.prec != 64:
+        elif M.NR != N or M.NC != N or M.prec == 32:
             # wrong-size matrix: a new one is returned, as the reference does
             # (AssembleBEMMatrix.cc:700-705)
             M = HMatrix(N, N)
         Omega = complex(Omega)
         _sync_stream()
-        _check(lib().scuffx_assemble(self.h, Omega.real, Omega.imag, _ptr(M.t), N))
+        if M.prec == 8:
+            _check(lib().scuffx_assemble_real(self.h, Omega.real, Omega.imag, _ptr(M.t), N, int(M.packed)))
+        elif M.packed:
+            # complex symmetric packed storage: assembled full, then packed
+            full = HMatrix(N, N)
+            _check(lib().scuffx_assemble(self.h, Omega.real, Omega.imag, _ptr(full.t), N))
+            torch = _torch()
+            iu = torch.triu_indices(N, N, device="cuda")
+            order = torch.argsort(iu[1] * N + iu[0])
+            M.t.copy_(full.t[iu[1][order], iu[0][order]])
+        else:
+            _check(lib().scuffx_assemble(self.h, Omega.real, Omega.imag, _ptr(M.t), N))
         M.factored = False
         return M
 
@@ -607,6 +745,35 @@ class RWGGeometry:
 
     def BlockSizes(self, nsa, nsb):
         return self.BFOffset(nsa)[1], self.BFOffset(nsb)[1]
+
+    def AssembleBEMMatrixBlockReal(self, nsa, nsb, Omega, GradM=False, TorqueAxes=None, Packed=False):
+        """the real counterpart of AssembleBEMMatrixBlock for pure imaginary
+        frequencies (the LHM_REAL T, U and dU blocks of the Casimir caller,
+        CreateSC3Data.cc:100-157): real parts of the block and of its
+        derivatives; Packed (diagonal blocks only) gives symmetric packed
+        storage.  Returns B, or (B, GradB), or (B, GradB, dBdT)."""
+        torch = _torch()
+        na, nb = self.BlockSizes(nsa, nsb)
+        B = HMatrix(na, nb, dtype="float64", Storage="symmetric" if Packed else "normal")
+        nt = 0 if TorqueAxes is None else len(TorqueAxes)
+        want = GradM or nt > 0
+        G3 = torch.zeros((3, nb, na), dtype=torch.float64, device="cuda") if want else None
+        T3 = torch.zeros((max(nt, 1), nb, na), dtype=torch.float64, device="cuda") if nt else None
+        gam = np.zeros(9 * max(nt, 1))
+        for t in range(nt):
+            gam[9 * t:9 * t + 9] = CreateGammaMatrix(TorqueAxes[t]).ravel(order="F")
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_assemble_block_real(self.h, nsa, nsb, Omega.real, Omega.imag, _ptr(B.t), na,
+                                                int(B.packed), _ptr(G3) if want else None, na, na * nb, nt,
+                                                gam.ctypes.data_as(C.c_void_p) if nt else None,
+                                                _ptr(T3) if nt else None))
+        if not want:
+            return B
+        grads = [HMatrix(na, nb, data=G3[m]) for m in range(3)]
+        if nt == 0:
+            return B, grads
+        return B, grads, [HMatrix(na, nb, data=T3[t]) for t in range(nt)]
 
     def AssembleBEMMatrixBlock(self, nsa, nsb, Omega, GradM=False, TorqueAxes=None):
         """Block (nsa rows, nsb columns) of the BEM matrix
@@ -689,7 +856,7 @@ def BatchSweep(G, Omegas, IF, prec=64):
 
 
 PROFILED_KERNELS = ("gemm_c128", "gemm_c64", "panel_c128", "panel_c64", "trsm_c128", "trsm_c64",
-                    "far", "near", "self", "stamp", "fippi", "resid", "grad")
+                    "far", "near", "self", "stamp", "fippi", "resid", "grad", "gemm_f64", "panel_f64")
 
 
 def profile(on=True):
@@ -708,9 +875,12 @@ def profile_read():
     return out
 
 
-def solve_mixed(M, b, tol=1e-13, maxiter=30):
-    """Solve M x = b with a complex64 LU of M and complex128 iterative
-    refinement; returns (x, iterations, final relative residual)."""
+def solve_mixed(M, b, tol=1e-13, maxiter=30, method="ir"):
+    """Solve M x = b with a complex64 LU of M and complex128 refinement;
+    returns (x, iterations, final relative residual |b - M x|_inf / |b|_inf).
+    method "ir": classical refinement, continued by GMRES-IR if it stalls
+    above tol; "gmres": GMRES-IR from the start (maxiter bounds its inner
+    steps).  Raises if the tolerance is not reached."""
     torch = _torch()
     N = M.NR
     LU32 = torch.empty((N, N), dtype=torch.complex64, device="cuda")
@@ -721,6 +891,9 @@ def solve_mixed(M, b, tol=1e-13, maxiter=30):
         raise SCUFFXError("singular complex64 factor (info=%d)" % info)
     x = HVector(N)
     it, rel = C.c_int(), C.c_double()
-    _check(lib().scuffx_refine(_ptr(M.t), N, N, _ptr(LU32), N, _ptr(ipiv), _ptr(b.t), _ptr(x.t),
-                               tol, maxiter, C.byref(it), C.byref(rel)))
+    fn = {"ir": lib().scuffx_refine, "gmres": lib().scuffx_gmres_ir}[method]
+    rc = _check(fn(_ptr(M.t), N, N, _ptr(LU32), N, _ptr(ipiv), _ptr(b.t), _ptr(x.t),
+                   tol, maxiter, C.byref(it), C.byref(rel)))
+    if rc == NOT_CONVERGED:
+        raise SCUFFXError(lib().scuffx_last_error().decode())
     return x, it.value, rel.value

@@ -24,14 +24,18 @@ some entry so far at this Xi (CasimirIntegrand.cc:370-494).
 
 Every block, factorization, determinant and trace runs on the GPU through
 the C ABI; this module only orchestrates them and moves device blocks
-into place.  Scope: compact (non-periodic) geometries at zero
-temperature.
+into place.  As in the reference (CreateSC3Data.cc:100-157), the matrices
+of a compact geometry at imaginary frequencies are real: T blocks in packed
+symmetric storage, U and dU blocks, M and dM as real matrices, factored by
+the real LU (dgetrf semantics).  Real=False keeps everything complex128
+(same values: the imaginary parts vanish).  Scope: compact (non-periodic)
+geometries at zero temperature.
 """
 import math
 
 import numpy as np
 
-from . import HMatrix, SCUFFXError, ReadTransFile
+from . import HMatrix, SCUFFXError, ReadTransFile, lib, _check, _ptr, _sync_stream
 
 XIMIN_DEFAULT = 0.001          # scuff-cas3D.cc:80
 RELTOL_DEFAULT = 1.0e-2        # scuff-cas3D.cc:84
@@ -49,8 +53,10 @@ class SC3Data:
     one such group per transformation in file order."""
 
     def __init__(self, G, Energy=False, XForce=False, YForce=False, ZForce=False, TorqueAxes=None,
-                 XiMin=XIMIN_DEFAULT, AbsTol=0.0, RelTol=RELTOL_DEFAULT, TransFile=None):
+                 XiMin=XIMIN_DEFAULT, AbsTol=0.0, RelTol=RELTOL_DEFAULT, TransFile=None, Real=True):
         self.G = G
+        self.Real = bool(Real)
+        self.dtype = "float64" if self.Real else "complex128"
         self.GTCs = ReadTransFile(TransFile)          # CreateSC3Data.cc:67-77
         G.CheckGTCList(self.GTCs)
         self.NumTransformations = len(self.GTCs)
@@ -71,8 +77,8 @@ class SC3Data:
         self.NTNQ = self.NumTransformations * self.NumQuantities
         self.XiMin, self.AbsTol, self.RelTol = XiMin, AbsTol, RelTol
         self.Mate = G.Mate
-        self.M = HMatrix(self.N, self.N)
-        self.dM = HMatrix(self.N, self.N1) if self.Mus else None
+        self.M = HMatrix(self.N, self.N, dtype=self.dtype)
+        self.dM = HMatrix(self.N, self.N1, dtype=self.dtype) if self.Mus else None
         self.ByXi = []                       # (Xi, EFT) of every evaluation
 
     def _offsets(self):
@@ -100,10 +106,11 @@ class SC3Data:
                     ld[ns] = ld[nm]
                     logdet_inf += ld[ns]
                 continue
-            T = G.AssembleBEMMatrixBlock(ns, ns, Omega)
+            T = self._block(ns, ns, Omega, Packed=True)
             TBlocks[ns] = T
             if self.Energy:
-                TF = HMatrix(n, n, data=T.t.clone())
+                # LU of T for the diagonal of M_inf's factors (:315-337)
+                TF = HMatrix(n, n, dtype=self.dtype, data=T.t.clone(), Storage="symmetric" if T.packed else "normal")
                 TF.LUFactorize()
                 ld[ns] = TF.LogDet().real
                 logdet_inf += ld[ns]
@@ -124,16 +131,21 @@ class SC3Data:
                     if (ns, nsp) in UBlocks and never_moved[ns] and never_moved[nsp]:
                         continue
                     if ns == 0 and self.Mus:
-                        r = G.AssembleBEMMatrixBlock(0, nsp, Omega, GradM=True,
-                                                     TorqueAxes=self.TorqueAxes if nt_q else None)
+                        r = self._block(0, nsp, Omega, GradM=True, TorqueAxes=self.TorqueAxes if nt_q else None)
                         UBlocks[(ns, nsp)] = r[0]
                         dU[nsp] = list(r[1]) + (list(r[2]) if nt_q else [])
                     else:
-                        UBlocks[(ns, nsp)] = G.AssembleBEMMatrixBlock(ns, nsp, Omega)
+                        UBlocks[(ns, nsp)] = self._block(ns, nsp, Omega)
             EFT[nt * NQ:(nt + 1) * NQ] = self._quantities(offs, TBlocks, UBlocks, dU, logdet_inf)
             G.UnTransform()
         self.ByXi.append((Xi, EFT.copy()))
         return EFT
+
+    def _block(self, nsa, nsb, Omega, GradM=False, TorqueAxes=None, Packed=False):
+        if self.Real:
+            return self.G.AssembleBEMMatrixBlockReal(nsa, nsb, Omega, GradM=GradM, TorqueAxes=TorqueAxes,
+                                                     Packed=Packed)
+        return self.G.AssembleBEMMatrixBlock(nsa, nsb, Omega, GradM=GradM, TorqueAxes=TorqueAxes)
 
     def _quantities(self, offs, TBlocks, UBlocks, dU, logdet_inf):
         """stamps the blocks into M, factorizes it and reads the energy,
@@ -143,7 +155,13 @@ class SC3Data:
         M.t.zero_()
         M.factored = False
         for ns, (o, n) in enumerate(offs):
-            M.t[o:o + n, o:o + n].copy_(TBlocks[ns].t)
+            T = TBlocks[ns]
+            if T.packed:
+                # InsertBlock of packed storage: unpacked straight into M
+                _sync_stream()
+                _check(lib().scuffx_unpack_symmetric(_ptr(T.t), n, _ptr(M.t[o:o + n, o:o + n]), self.N, T.prec))
+            else:
+                M.t[o:o + n, o:o + n].copy_(T.t)
         for (ns, nsp), U in UBlocks.items():
             (o, n), (op, nq) = offs[ns], offs[nsp]
             M.t[op:op + nq, o:o + n].copy_(U.t)                          # InsertBlock
