@@ -301,6 +301,20 @@ int scuffx_rhs_planewave(scuffx_geom *G, double omega_re, double omega_im, const
 int scuffx_power(scuffx_geom *G, double omega_re, double omega_im, const void *KN_dev,
                  const void *rhs_dev, double *PAbs, double *PExt);
 
+/* ---- energy/momentum-transfer power (GetEMTPFTMatrix, EMTPFT.cc:468-803, the
+        reference's default power method, PFTOptions.h:84) ----
+   Mext_dev: the exterior region's matrix from scuffx_assemble_region(G, w, 0,
+   ...), column-major complex128.  PScat from the quadratic form of the
+   solution KN with it (integral method EHDERIVATIVES2, EMTPFT.cc:334-362),
+   PAbs = extinction (scuffx_power) - PScat.  Every surface must live in the
+   exterior region (else SCUFFX_ERR_UNSUPPORTED). */
+int scuffx_power_emt(scuffx_geom *G, double omega_re, double omega_im, const void *KN_dev, const void *rhs_dev,
+                     const void *Mext_dev, long ldm, double *PAbs, double *PScat);
+/* the BEM matrix with the contributions of one region only (0: exterior),
+   complex128: the G and C integrals the EMT power reads
+   (GetGCMatrixElements for that region, EMTPFT.cc:336-353) */
+int scuffx_assemble_region(scuffx_geom *G, double omega_re, double omega_im, int region, void *M_dev, long ldm);
+
 /* ---- measurement (no counterpart in the reference) ----
    on != 0: every launch of the hot kernels is bracketed by HIP events on
    its stream; totals are reset.  Kernel names: gemm_c128 gemm_c64

@@ -40,8 +40,10 @@ int o_api_add_surface(void *p, const double *V, int nV, const int *T, int nT,
   if (!S) { if (err) { fprintf(stderr, "oracle: %s\n", err); free(err); } return -1; }
   OMaterial m;
   memset(&m, 0, sizeof(m));
-  m.Kind = kind == 2 ? OMAT_DRUDE : OMAT_CONST;
+  /* kind 3: CONST_EPS_er_SIGMA_wp (the conductivity travels in wp) */
+  m.Kind = kind == 2 ? OMAT_DRUDE : kind == 3 ? OMAT_SIGMA : OMAT_CONST;
   m.Eps = er + I*ei; m.Mu = 1.0; m.wp = wp; m.gamma = gamma;
+  if (kind == 3) { m.sigma = wp; m.wp = 0.0; }
   return o_geom_add_surface(h->G, S, m);
 }
 
@@ -114,6 +116,30 @@ int o_api_assemble_rows(void *p, double wr, double wi, double *M, int ldm, int e
 {
   oh *h = (oh *)p;
   return o_assemble_bem_rows(h->G, wr + I*wi, h->FC, (zdouble *)M, ldm, e0, e1);
+}
+
+int o_api_assemble_rows_compact(void *p, double wr, double wi, double *M, int ldm, int e0, int e1)
+{
+  oh *h = (oh *)p;
+  return o_assemble_bem_rows_compact(h->G, wr + I*wi, h->FC, (zdouble *)M, ldm, e0, e1);
+}
+
+/* the matrix with region r's contributions only (r < 0: all regions) */
+int o_api_assemble_region(void *p, double wr, double wi, double *M, int ldm, int r)
+{
+  oh *h = (oh *)p;
+  h->G->RegionFilter = r < 0 ? 0 : r + 1;
+  int rc = o_assemble_bem(h->G, wr + I*wi, h->FC, (zdouble *)M, ldm, NULL);
+  h->G->RegionFilter = 0;
+  return rc;
+}
+
+int o_api_power_emt(void *p, double wr, double wi, const double *KN, const double *RHS,
+                    const double *Mext, int ldm, double *out2)
+{
+  oh *h = (oh *)p;
+  return o_power_emt(h->G, wr + I*wi, (const zdouble *)KN, (const zdouble *)RHS, (const zdouble *)Mext, ldm,
+                     out2, out2 + 1);
 }
 
 size_t o_api_fippi_count(void *p) { return o_fippi_cache_size(((oh *)p)->FC); }

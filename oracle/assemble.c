@@ -90,20 +90,27 @@ static zdouble csqrt2(zdouble z)
   return cimag(r) < 0.0 ? -r : r;
 }
 
-#define MADD(r, c, v) (M[(size_t)(r) + (size_t)(c)*(size_t)ldm] += (v))
+#define MADD(r, c, v) (M[(size_t)((r) - rbase) + (size_t)(c)*(size_t)ldm] += (v))
 
 /* one surface-pair block; rows restricted to edges [e0,e1) of Sa */
+/* rows are stored from rbase on: M[(row - rbase) + col*ldm] */
 static void ssi_block(OGeom *G, int nsa, int nsb, zdouble Omega, OFIPPICache *FC,
-                      zdouble *M, int ldm, int e0, int e1, int zero, OAsmStats *st)
+                      zdouble *M, int ldm, int rbase, int e0, int e1, int zero, OAsmStats *st)
 {
   OSurface *Sa = G->Surfaces[nsa], *Sb = G->Surfaces[nsb];
   int RowOffset = G->BFIndexOffset[nsa], ColOffset = G->BFIndexOffset[nsb];
   int Symmetric = (nsa == nsb);
   if (zero)
     for (int c = 0; c < Sb->NumBFs; c++)
-      for (int r = 0; r < Sa->NumBFs; r++) M[(size_t)(RowOffset + r) + (size_t)(ColOffset + c)*ldm] = 0.0;
+      for (int r = 0; r < Sa->NumBFs; r++) M[(size_t)(RowOffset + r - rbase) + (size_t)(ColOffset + c)*ldm] = 0.0;
   int CR[2]; double Signs[2];
   int ncr = count_common_regions(Sa, Sb, CR, Signs);
+  if (G->RegionFilter > 0) {
+    int kept = 0;
+    for (int r = 0; r < ncr; r++)
+      if (CR[r] == G->RegionFilter - 1) { CR[kept] = CR[r]; Signs[kept] = Signs[r]; kept++; }
+    ncr = kept;
+  }
   if (ncr == 0) return;
   zdouble EpsA = G->EpsTF[CR[0]], MuA = G->MuTF[CR[0]], EpsB = 0, MuB = 0;
   double SignA = Signs[0], SignB = 0;
@@ -165,7 +172,7 @@ int o_assemble_bem(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M, int ldm
             M[(size_t)(o + r) + (size_t)(o + c)*ldm] = M[(size_t)(om + r) + (size_t)(om + c)*ldm];
         continue;
       }
-      ssi_block(G, ns, nsp, Omega, FC, M, ldm, 0, G->Surfaces[ns]->NumEdges, 1, st);
+      ssi_block(G, ns, nsp, Omega, FC, M, ldm, 0, 0, G->Surfaces[ns]->NumEdges, 1, st);
     }
   int N = G->TotalBFs;
   for (int nr = 1; nr < N; nr++)
@@ -177,7 +184,17 @@ int o_assemble_bem(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M, int ldm
 int o_assemble_bem_rows(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M, int ldm, int e0, int e1)
 {
   o_update_epsmu(G, Omega);
-  ssi_block(G, 0, 0, Omega, FC, M, ldm, e0, e1, 0, NULL);
+  ssi_block(G, 0, 0, Omega, FC, M, ldm, 0, e0, e1, 0, NULL);
+  return 0;
+}
+
+/* the same rows (upper triangle) into a compact buffer: M[(r - r0) + c*ldm]
+   with r0 the first basis function of edge e0 on surface 0 */
+int o_assemble_bem_rows_compact(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M, int ldm, int e0, int e1)
+{
+  o_update_epsmu(G, Omega);
+  int r0 = G->Surfaces[0]->IsPEC ? e0 : 2*e0;
+  ssi_block(G, 0, 0, Omega, FC, M, ldm, r0, e0, e1, 0, NULL);
   return 0;
 }
 

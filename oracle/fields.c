@@ -7,6 +7,8 @@
  *                      (libs/libscuff/AssembleRHSVector.cc:147-427): degree-20
  *                      triangle rule on each panel, sources in the exterior
  *                      region enter with a minus sign, E rows divided by ZVAC
+ *  o_power_emt         scattered power by energy/momentum transfer from the
+ *                      exterior-region matrix (EMTPFT.cc:468-803)
  *  o_power             absorbed power from the crossed overlap integrals
  *                      (OPFT.cc:82-146, :310-364); extinction from the
  *                      inner products of the solution with the RHS
@@ -148,6 +150,54 @@ void o_power(OGeom *G, zdouble Omega, const zdouble *KN, const zdouble *RHS,
   }
   if (PAbs) *PAbs = pabs;
   if (PExt) *PExt = pext;
+}
+
+/* EMT scattered power (EMTPFT.cc:629-803 with the integrals of the default
+   method EHDERIVATIVES2, :334-362) and absorbed power = extinction -
+   scattered (:818-830).  Every surface lives in the exterior region 0
+   (Sign = +1 for every pair, :640-650).  The integrals come from the
+   exterior-region matrix Mext: for edges a, b with basis functions
+   K (and N on a dielectric surface),
+     Mext(Ka,Kb) = i w mu G_ab, Mext(Na,Nb) = -i w eps G_ab, Mext(Ka,Nb) = -i k C_ab,
+   so QKK = mu w G = -i Mext(Ka,Kb), QNN = eps w G = i Mext(Na,Nb) and
+   QKNmNK = i k C = -Mext(Ka,Nb).  KN bilinears (GetPFT.cc:95-132):
+   k = KN(K), n = -ZVAC KN(N), and only KK when either surface is PEC.
+   Self pairs use the symmetry-reduced form (:695-705); a pair of distinct
+   surfaces adds the same value with both orders summed, so every ordered
+   pair contributes 0.5 (Re u0KK Im QKK + Re e0NN Im QNN + Im KNmNK Im QKNmNK). */
+int o_power_emt(OGeom *G, zdouble Omega, const zdouble *KN, const zdouble *RHS,
+                const zdouble *Mext, int ldm, double *PAbs, double *PScat)
+{
+  for (int ns = 0; ns < G->NumSurfaces; ns++)
+    if (G->Surfaces[ns]->RegionIndices[0] != 0) return -1;
+  double pscat = 0.0;
+  for (int nsa = 0; nsa < G->NumSurfaces; nsa++)
+    for (int nsb = 0; nsb < G->NumSurfaces; nsb++) {
+      OSurface *SA = G->Surfaces[nsa], *SB = G->Surfaces[nsb];
+      int OA = G->BFIndexOffset[nsa], OB = G->BFIndexOffset[nsb];
+      int both = !SA->IsPEC && !SB->IsPEC;
+      for (int a = 0; a < SA->NumEdges; a++)
+        for (int b = 0; b < SB->NumEdges; b++) {
+          int Ka = SA->IsPEC ? OA + a : OA + 2*a, Kb = SB->IsPEC ? OB + b : OB + 2*b;
+          zdouble ka = KN[Ka], kb = KN[Kb];
+          zdouble u0KK = conj(ka)*kb*O_ZVAC;
+          zdouble QKK = -I*Mext[(size_t)Ka + (size_t)Kb*ldm];
+          double d = creal(u0KK)*cimag(QKK);
+          if (both) {
+            zdouble na = -O_ZVAC*KN[Ka + 1], nb = -O_ZVAC*KN[Kb + 1];
+            zdouble e0NN = conj(na)*nb/O_ZVAC, KNmNK = conj(ka)*nb - conj(na)*kb;
+            zdouble QNN = I*Mext[(size_t)(Ka + 1) + (size_t)(Kb + 1)*ldm];
+            zdouble QKNmNK = -Mext[(size_t)Ka + (size_t)(Kb + 1)*ldm];
+            d += creal(e0NN)*cimag(QNN) + cimag(KNmNK)*cimag(QKNmNK);
+          }
+          pscat += 0.5*d;
+        }
+    }
+  double pabs = 0.0, pext = 0.0;
+  o_power(G, Omega, KN, RHS, &pabs, &pext);
+  if (PScat) *PScat = pscat;
+  if (PAbs) *PAbs = pext - pscat;
+  return 0;
 }
 
 /* Mie efficiencies; m_rel with creal(m_rel) < 0 selects the PEC sphere */

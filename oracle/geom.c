@@ -194,7 +194,7 @@ static void init_panel(OPanel *P, const double *Vt)
   P->ZHat[1] = A[2]*B[0] - A[0]*B[2];
   P->ZHat[2] = A[0]*B[1] - A[1]*B[0];
   double n = sqrt(P->ZHat[0]*P->ZHat[0] + P->ZHat[1]*P->ZHat[1] + P->ZHat[2]*P->ZHat[2]);
-  double s = 1.0/n;
+  double s = (P->ZHatFlipped ? -1.0 : 1.0)/n;
   for (int i = 0; i < 3; i++) P->ZHat[i] *= s;
   P->Area = 0.5*n;
 }
@@ -320,7 +320,62 @@ int o_geom_add_surface(OGeom *G, OSurface *S, OMaterial interior)
   G->BFIndexOffset[ns] = (ns == 0) ? 0 : G->BFIndexOffset[ns-1] + G->Surfaces[ns-1]->NumBFs;
   G->TotalBFs = G->BFIndexOffset[ns] + S->NumBFs;
   G->StoredOmega = 0.0;
+  o_geom_orient_normals(G);
   return ns;
+}
+
+/* number of panels of S crossed by the ray from X toward z = -infinity
+   (kdtri_surface_piercings, PointInObject.cc, without the kd-tree) */
+static int surface_piercings(const OSurface *S, const double X[3])
+{
+  int count = 0;
+  for (int p = 0; p < S->NumPanels; p++) {
+    const OPanel *P = S->Panels + p;
+    const double *A = S->Vertices + 3*P->VI[0], *B = S->Vertices + 3*P->VI[1], *C = S->Vertices + 3*P->VI[2];
+    double d = (B[0] - A[0])*(C[1] - A[1]) - (C[0] - A[0])*(B[1] - A[1]);
+    if (d == 0.0) continue;
+    double u = ((X[0] - A[0])*(C[1] - A[1]) - (C[0] - A[0])*(X[1] - A[1]))/d;
+    double v = ((B[0] - A[0])*(X[1] - A[1]) - (X[0] - A[0])*(B[1] - A[1]))/d;
+    if (u < 0.0 || v < 0.0 || u + v > 1.0) continue;
+    if (A[2] + u*(B[2] - A[2]) + v*(C[2] - A[2]) < X[2]) count++;
+  }
+  return count;
+}
+
+/* PointInRegion (PointInObject.cc:620-657): piercings of every non-PEC
+   surface bounding region r; region 0 for an even count, an interior
+   region for an odd one */
+static int point_in_region(const OGeom *G, int r, const double X[3])
+{
+  int total = 0;
+  for (int s = 0; s < G->NumSurfaces; s++) {
+    const OSurface *S = G->Surfaces[s];
+    if (S->IsPEC) continue;
+    if (S->RegionIndices[0] == r || S->RegionIndices[1] == r) total += surface_piercings(S, X);
+  }
+  return r == 0 ? (total % 2 == 0) : (total % 2 == 1);
+}
+
+/* panel normals into each surface's exterior region (RWGGeometry.cc:653-690):
+   the point half a panel radius along the normal must lie in the exterior
+   region; if it lies in the interior region the normal is flipped */
+void o_geom_orient_normals(OGeom *G)
+{
+  for (int s = 0; s < G->NumSurfaces; s++) {
+    OSurface *S = G->Surfaces[s];
+    for (int p = 0; p < S->NumPanels; p++) {
+      OPanel *P = S->Panels + p;
+      P->ZHatFlipped = 0;
+      init_panel(P, S->Vertices);
+      double X[3];
+      for (int i = 0; i < 3; i++) X[i] = P->Centroid[i] + 0.5*P->Radius*P->ZHat[i];
+      if (point_in_region(G, S->RegionIndices[0], X)) continue;
+      if (point_in_region(G, S->RegionIndices[1], X)) {
+        P->ZHatFlipped = 1;
+        for (int i = 0; i < 3; i++) P->ZHat[i] = -P->ZHat[i];
+      }
+    }
+  }
 }
 
 void o_geom_free(OGeom *G)
@@ -337,6 +392,7 @@ void o_update_epsmu(OGeom *G, zdouble Omega)
     OMaterial *m = G->Materials + r;
     if (m->Kind == OMAT_VACUUM) { G->EpsTF[r] = 1.0; G->MuTF[r] = 1.0; }
     else if (m->Kind == OMAT_CONST) { G->EpsTF[r] = m->Eps; G->MuTF[r] = m->Mu; }
+    else if (m->Kind == OMAT_SIGMA) { G->EpsTF[r] = creal(m->Eps) + I*m->sigma/Omega; G->MuTF[r] = 1.0; }
     else { G->EpsTF[r] = 1.0 - m->wp*m->wp / (Omega*(Omega + I*m->gamma)); G->MuTF[r] = 1.0; }
   }
   G->StoredOmega = Omega;

@@ -29,6 +29,7 @@ typedef struct {
   double Radius;
   double Area;
   int Index;
+  int ZHatFlipped;        /* normal reversed to point into the exterior region */
 } OPanel;
 
 typedef struct OEdge_s {
@@ -56,10 +57,12 @@ typedef struct {
 #define OMAT_VACUUM 0
 #define OMAT_CONST  1
 #define OMAT_DRUDE  2   /* eps = 1 - wp^2/(w(w + i gamma)) */
+#define OMAT_SIGMA  3   /* eps = Re(Eps) + i sigma/w (CONST_EPS_x_SIGMA_y, libMatProp.cc:273-280) */
 typedef struct {
   int Kind;
-  zdouble Eps, Mu;        /* OMAT_CONST */
+  zdouble Eps, Mu;        /* OMAT_CONST; Re(Eps) for OMAT_SIGMA */
   double wp, gamma;       /* OMAT_DRUDE */
+  double sigma;           /* OMAT_SIGMA */
 } OMaterial;
 
 #define OMAXSURF 8
@@ -74,6 +77,7 @@ typedef struct {
   zdouble EpsTF[OMAXREG], MuTF[OMAXREG];
   zdouble StoredOmega;
   int Mate[OMAXSURF];     /* RWGGeometry::Mate (RWGGeometry.cc:733-770), set by the caller; -1 = none */
+  int RegionFilter;       /* 0: every common region; r + 1: contributions of region r only */
 } OGeom;
 
 /* mesh construction */
@@ -86,6 +90,7 @@ OSurface *o_surface_new(const double *V, int nV, const int *T, int nT,
 void o_surface_free(OSurface *S);
 OGeom *o_geom_new(void);
 int  o_geom_add_surface(OGeom *G, OSurface *S, OMaterial interior);
+void o_geom_orient_normals(OGeom *G);   /* RWGGeometry.cc:653-690 */
 void o_geom_free(OGeom *G);
 void o_update_epsmu(OGeom *G, zdouble Omega);
 
@@ -221,6 +226,8 @@ int o_assemble_block_grad(OGeom *G, zdouble Omega, OFIPPICache *FC, int nsa, int
 /* rows [r0,r1) of the upper triangle only (bounded CPU timing sample) */
 int o_assemble_bem_rows(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M,
                         int ldm, int e0, int e1);
+int o_assemble_bem_rows_compact(OGeom *G, zdouble Omega, OFIPPICache *FC, zdouble *M,
+                                int ldm, int e0, int e1);
 
 /* ---------------------------------------------------------------------
  * LU / solve (LAPACK zgetrf/zgetrs semantics; LBWrappers.cc:151-245)
@@ -238,6 +245,12 @@ int  o_assemble_rhs_pw(OGeom *G, zdouble Omega, const zdouble E0[3],
                        const double nHat[3], zdouble *RHS);
 void o_power(OGeom *G, zdouble Omega, const zdouble *KN, const zdouble *RHS,
              double *PAbs, double *PExt);
+/* energy/momentum-transfer scattered and absorbed power (EMTPFT.cc:468-803,
+   default integral method EHDERIVATIVES2 :334-362) for surfaces that all
+   live in the exterior region 0: Mext = the matrix assembled with the
+   exterior region's contributions only */
+int o_power_emt(OGeom *G, zdouble Omega, const zdouble *KN, const zdouble *RHS,
+                const zdouble *Mext, int ldm, double *PAbs, double *PScat);
 void o_mie_sphere(double x_size, zdouble m_rel, int nmax,
                   double *Qsca, double *Qext);
 

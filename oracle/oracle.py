@@ -51,6 +51,10 @@ def lib():
     L.o_api_assemble.argtypes = [C.c_void_p, C.c_double, C.c_double, _zp, C.c_int]
     L.o_api_assemble_rows.argtypes = [C.c_void_p, C.c_double, C.c_double, _zp, C.c_int,
                                       C.c_int, C.c_int]
+    L.o_api_assemble_rows_compact.argtypes = [C.c_void_p, C.c_double, C.c_double, _zp, C.c_int,
+                                              C.c_int, C.c_int]
+    L.o_api_assemble_region.argtypes = [C.c_void_p, C.c_double, C.c_double, _zp, C.c_int, C.c_int]
+    L.o_api_power_emt.argtypes = [C.c_void_p, C.c_double, C.c_double, _zp, _zp, _zp, C.c_int, _dp]
     L.o_api_fippi_count.argtypes = [C.c_void_p]
     L.o_api_fippi_count.restype = C.c_size_t
     L.o_api_qifippi.argtypes = [_dp, _dp, C.c_int, _dp, C.POINTER(C.c_size_t)]
@@ -215,6 +219,37 @@ class Geometry:
         omega = complex(omega)
         self.L.o_api_assemble_rows(self.h, omega.real, omega.imag, M, N, e0, e1)
         return M
+
+    def rows(self, omega, e0, e1):
+        """upper-triangle entries of the rows of edges e0..e1-1 of surface 0
+        (single-surface geometries), as a (rows x N) array"""
+        N = self.N
+        nr = (e1 - e0) * (1 if self.num_bfs(0) == self.L.o_api_num_edges(self.h, 0) else 2)
+        R = np.zeros((nr, N), dtype=np.complex128, order="F")
+        omega = complex(omega)
+        self.L.o_api_assemble_rows_compact(self.h, omega.real, omega.imag, R, nr, e0, e1)
+        return R
+
+    def assemble_region(self, omega, region):
+        """the matrix with one region's contributions only (region 0: the
+        exterior)"""
+        N = self.N
+        M = np.zeros((N, N), dtype=np.complex128, order="F")
+        omega = complex(omega)
+        if self.L.o_api_assemble_region(self.h, omega.real, omega.imag, M, N, region) != 0:
+            raise RuntimeError("oracle: region assembly failed")
+        return M
+
+    def power_emt(self, omega, KN, RHS, Mext):
+        """{'PAbs', 'PScat', 'PExt'} by energy/momentum transfer"""
+        out = np.zeros(2)
+        omega = complex(omega)
+        KN = np.asfortranarray(KN, dtype=np.complex128)
+        RHS = np.asfortranarray(RHS, dtype=np.complex128)
+        if self.L.o_api_power_emt(self.h, omega.real, omega.imag, KN, RHS, np.asfortranarray(Mext),
+                                  Mext.shape[0], out) != 0:
+            raise RuntimeError("oracle: EMT power needs every surface in the exterior region")
+        return {"PAbs": out[0], "PScat": out[1], "PExt": out[0] + out[1]}
 
     def num_bfs(self, ns):
         return self.L.o_api_num_bfs(self.h, ns)

@@ -171,7 +171,11 @@ Assembler::~Assembler()
 {
   if (hip_down()) return;   // after scuffx_shutdown: left to the process exit
   for (auto &D : ds_) { hipFree(D.V); hipFree(D.PVI); hipFree(D.PC); hipFree(D.EI); hipFree(D.EL); }
-  for (auto &B : blocks_) { hipFree(B.d_rowPtr); hipFree(B.d_col); hipFree(B.d_idx); hipFree(B.d_pairs); }
+  for (auto &B : blocks_) {
+    hipFree(B.d_rowPtr); hipFree(B.d_col); hipFree(B.d_idx); hipFree(B.d_pairs);
+    hipFree(B.bands.d_cols); hipFree(B.bands.d_colSlots); hipFree(B.bands.d_slots);
+  }
+  for (auto &e : evpool_) hipEventDestroy(e);
   hipFree(d_jobs_); hipFree(d_records_);
   hipFree(d_HP_); hipFree(d_HN_); hipFree(d_HS_); hipFree(d_scratch_); hipFree(d_stage_);
   hipFree(d_regs_);
@@ -341,8 +345,157 @@ void Assembler::build_near()
       hipMemcpy(B.d_idx, B.idx.data(), sizeof(int) * B.idx.size(), hipMemcpyHostToDevice);
       hipMemcpy(B.d_pairs, B.pairs.data(), sizeof(NearPair) * B.pairs.size(), hipMemcpyHostToDevice);
     }
+    build_bands(B);
   }
   stats_.nfippi = (long)jobs_.size();
+}
+
+// Column bands of a block.  The key of a column edge is its higher panel
+// index; edges are numbered at the second visit of a panel scan
+// (InitEdgeList.cc:163-167), so on one surface the key grows with the edge
+// index and a band is a run of consecutive columns.  The band width `ch`
+// keeps one band's far records near SCUFFX_FAR_BAND_MB (default 1024 MB):
+// records of 2 regions x 9 combinations x {H0, H1}, and about 1.3 slots per
+// panel of the key range (the lower panels of the band's edges).
+void Assembler::build_bands(NearBlock &B)
+{
+  FarBands &F = B.bands;
+  const Surface &Sa = G_.Surfaces[B.sa], &Sb = G_.Surfaces[B.sb];
+  const int nPa = Sa.NumPanels(), nPb = Sb.NumPanels(), nEa = Sa.NumEdges(), nEb = Sb.NumEdges();
+  const bool same = (B.sa == B.sb);
+  double budget = 1024.0 * 1048576.0;
+  if (const char *e = getenv("SCUFFX_FAR_BAND_MB")) budget = std::max(1.0, atof(e)) * 1048576.0;
+  const double rec_bytes = 2 * 9 * 4 * sizeof(double);
+  long ch = (long)(budget / (1.3 * std::max(nPa, 1) * rec_bytes));
+  ch = std::max(32L, std::min(ch, (long)std::max(nPb, 1)));
+  F.ch = (int)ch;
+  F.rowsCap = std::max(nPa, 1);
+  auto key = [&](int e) { return std::max(Sb.Edges[e].iPPanel, Sb.Edges[e].iMPanel); };
+  const int nbands = (int)((nPb + ch - 1) / ch);
+  std::vector<std::vector<int>> bcols(std::max(nbands, 1));
+  for (int e = 0; e < nEb; e++) {
+    int k = key(e);
+    if (k < 0) continue;
+    bcols[k / ch].push_back(e);
+  }
+  // row panels of each row-edge prefix (diagonal blocks): 1 + the highest
+  // panel index among edges 0..ea
+  std::vector<int> prefRow(nEa + 1, 0);
+  for (int e = 0; e < nEa; e++)
+    prefRow[e + 1] = std::max(prefRow[e], 1 + std::max(Sa.Edges[e].iPPanel, Sa.Edges[e].iMPanel));
+  // near pairs by column panel (pa values of the CSR entries (pa, pb))
+  std::vector<std::vector<int>> nearByCol(nPb);
+  for (int pa = 0; pa < nPa; pa++)
+    for (int q = B.rowPtr[pa]; q < B.rowPtr[pa + 1]; q++) nearByCol[B.col[q]].push_back(pa);
+  F.uniqueFar = same ? (long)nPa * (nPa - 1) / 2 - (long)B.col.size() / 2 : (long)nPa * nPb - (long)B.col.size();
+  F.colStart.assign(1, 0);
+  F.slotStart.assign(1, 0);
+  for (int k = 0; k < nbands; k++) {
+    const std::vector<int> &cl = bcols[k];
+    if (cl.empty()) continue;
+    std::vector<int> sl;
+    for (int e : cl) {
+      if (Sb.Edges[e].iPPanel >= 0) sl.push_back(Sb.Edges[e].iPPanel);
+      if (Sb.Edges[e].iMPanel >= 0) sl.push_back(Sb.Edges[e].iMPanel);
+    }
+    std::sort(sl.begin(), sl.end());
+    sl.erase(std::unique(sl.begin(), sl.end()), sl.end());
+    int maxe = 0;
+    for (int e : cl) {
+      F.cols.push_back(e);
+      const int pb2[2] = {Sb.Edges[e].iPPanel, Sb.Edges[e].iMPanel};
+      for (int t = 0; t < 2; t++)
+        F.colSlots.push_back(pb2[t] < 0 ? -1 : (int)(std::lower_bound(sl.begin(), sl.end(), pb2[t]) - sl.begin()));
+      maxe = std::max(maxe, e);
+    }
+    const int rowEdges = same ? maxe + 1 : nEa;
+    const int rowLimit = same ? prefRow[std::min(rowEdges, nEa)] : nPa;
+    long fp = 0;
+    for (int pb : sl) {
+      long n = rowLimit;
+      if (same && pb < rowLimit) n--;
+      for (int pa : nearByCol[pb]) if (pa < rowLimit) n--;
+      fp += n;
+    }
+    F.slots.insert(F.slots.end(), sl.begin(), sl.end());
+    F.colStart.push_back((int)F.cols.size());
+    F.slotStart.push_back((int)F.slots.size());
+    F.rowEdges.push_back(rowEdges);
+    F.rowLimit.push_back(rowLimit);
+    F.farPairs.push_back(fp);
+    F.maxSlots = std::max(F.maxSlots, (int)sl.size());
+  }
+  auto up = [](int **d, const std::vector<int> &h) {
+    hipMalloc(d, sizeof(int) * std::max<size_t>(1, h.size()));
+    if (!h.empty()) hipMemcpy(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice);
+  };
+  up(&F.d_cols, F.cols);
+  up(&F.d_colSlots, F.colSlots);
+  up(&F.d_slots, F.slots);
+}
+
+int Assembler::far_and_stamp(NearBlock &B, const DevSurface &Da, const DevSurface &Db, const Surface &Sa,
+                             const Surface &Sb, int nreg, const RegionK *dreg, int needH1, MatOut M, long rowOff,
+                             long colOff, hipStream_t stream, double *t_far, double *t_stamp)
+{
+  const FarBands &F = B.bands;
+  const int same = (B.sa == B.sb);
+  const int stride = needH1 ? 4 : 2;
+  const long recd = (long)std::max(nreg, 1) * 9 * stride;
+  if (ensure(&d_HP_, &cap_HP_, (size_t)std::max(F.maxSlots, 1) * F.rowsCap * recd)) return -2;
+  const bool timed = t_far || t_stamp;
+  const int nb = F.count();
+  if (timed) {
+    while ((int)evpool_.size() < 2 * nb + 1) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return -3;
+      evpool_.push_back(e);
+    }
+    hipEventRecord(evpool_[0], stream);
+  }
+  // algorithmic flops per (panel pair, region): 36 point pairs of 98
+  // (PMCHWT) or 56 (EFIE) flops plus the 9-combination epilogue; the
+  // pairs a band recomputes (its lower slot panels) are not counted
+  const double wpair = needH1 ? 3780.0 : 2160.0;
+  long computed = 0;
+  for (long v : F.farPairs) computed += v;
+  const double useful = computed > 0 ? (double)F.uniqueFar / (double)computed : 0.0;
+  const double nbf_col = Sb.IsPEC ? 1.0 : 2.0, nbf_row = Sa.IsPEC ? 1.0 : 2.0;
+  for (int k = 0; k < nb; k++) {
+    const int c0 = F.colStart[k], nc = F.colStart[k + 1] - c0;
+    const int s0 = F.slotStart[k], ns = F.slotStart[k + 1] - s0;
+    if (nreg > 0) {
+      int tok = prof_begin(stream, PK_FAR);
+      HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, F.rowLimit[k], Db.V, Db.PVI, Db.PC, F.d_slots + s0, ns, same,
+                        B.d_rowPtr, B.d_col, nreg, dreg, needH1, d_HP_, F.rowsCap));
+      prof_end(stream, tok, wpair * (double)F.farPairs[k] * useful * nreg);
+    }
+    if (timed) hipEventRecord(evpool_[2 * k + 1], stream);
+    {
+      // matrix bytes the band writes (upper triangle of a diagonal block)
+      double entries = 0;
+      for (int j = 0; j < nc; j++) {
+        const int eb = F.cols[c0 + j];
+        entries += same ? (double)(eb + 1) : (double)Sa.NumEdges();
+      }
+      entries *= nbf_row * nbf_col;
+      int tok = prof_begin(stream, PK_STAMP);
+      HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, F.d_cols + c0, F.d_colSlots + 2 * c0, nc, F.rowEdges[k],
+                          Db.EI, Db.EL, same, Sa.IsPEC, Sb.IsPEC, nreg, dreg, needH1, d_HP_, F.rowsCap, d_HN_, d_HS_,
+                          B.d_rowPtr, B.d_col, B.d_idx, M, (int)rowOff, (int)colOff));
+      prof_end(stream, tok, 16.0 * entries);
+    }
+    if (timed) hipEventRecord(evpool_[2 * k + 2], stream);
+  }
+  if (timed && nb > 0) {
+    HIPCHK(hipEventSynchronize(evpool_[2 * nb]));
+    for (int k = 0; k < nb; k++) {
+      float ms = 0;
+      if (t_far && hipEventElapsedTime(&ms, evpool_[2 * k], evpool_[2 * k + 1]) == hipSuccess) *t_far += ms;
+      if (t_stamp && hipEventElapsedTime(&ms, evpool_[2 * k + 1], evpool_[2 * k + 2]) == hipSuccess) *t_stamp += ms;
+    }
+  }
+  return 0;
 }
 
 int Assembler::build_fippi(hipStream_t stream)
@@ -459,6 +612,7 @@ int Assembler::regions_for(int sa, int sb, const std::vector<cdouble> &eps, cons
   int nreg = common_regions(Sa, Sb, CR, Sg);
   int used = 0;
   for (int r = 0; r < nreg; r++) {
+    if (region_only_ >= 0 && CR[r] != region_only_) continue;
     cdouble E = eps[CR[r]], Mu = mu[CR[r]];
     if (E == 0.0) continue;
     cdouble k = csqrt2(E * Mu) * omega, I(0, 1);
@@ -567,8 +721,6 @@ int Assembler::assemble(cdouble omega, void *Mp, long ldm, int kind, hipStream_t
       const int stride = needH1 ? 4 : 2;
       HIPCHK(hipMemcpyAsync(d_regs_ + 2 * (bi % 2), rk, sizeof(RegionK) * std::max(nreg, 1), hipMemcpyHostToDevice, stream));
       RegionK *dreg = d_regs_ + 2 * (bi % 2);
-      long npairs = same ? (long)Da.nP * (Da.nP - 1) / 2 : (long)Da.nP * Db.nP;
-      if (ensure(&d_HP_, &cap_HP_, (size_t)npairs * nreg * 9 * stride)) return -2;
       if (ensure(&d_HN_, &cap_HN_, (size_t)B.pairs.size() * nreg * 9 * stride)) return -2;
       if (same) {
         if (ensure(&d_HS_, &cap_HS_, (size_t)Da.nP * nreg * 18)) return -2;
@@ -584,17 +736,8 @@ int Assembler::assemble(cdouble omega, void *Mp, long ldm, int kind, hipStream_t
         hipEventRecord(ev[0], stream);
         hipEventRecord(ev[1], stream);
       }
-      // algorithmic flops per (panel pair, region): 36 point pairs of 98
-      // (PMCHWT) or 56 (EFIE) flops plus the 9-combination epilogue
       const double wpair = needH1 ? 3780.0 : 2160.0;
-      const long nfar_blk = npairs - (same ? (long)B.col.size() / 2 : (long)B.col.size());
-      if (nreg > 0) {
-        int tok = prof_begin(stream, PK_FAR);
-        HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, Da.nP, Db.V, Db.PVI, Db.PC, Db.nP, same,
-                          B.d_rowPtr, B.d_col, nreg, dreg, needH1, d_HP_));
-        prof_end(stream, tok, wpair * (double)nfar_blk * nreg);
-      }
-      hipEventRecord(ev[2], stream);
+      const long nfar_blk = B.bands.uniqueFar;
       if (nreg > 0 && !B.pairs.empty()) {
         int tok = prof_begin(stream, PK_NEAR);
         HIPCHK(launch_near(stream, B.d_pairs, (int)B.pairs.size(), d_records_, nreg, dreg, needH1, d_HN_));
@@ -607,23 +750,18 @@ int Assembler::assemble(cdouble omega, void *Mp, long ldm, int kind, hipStream_t
           return rc2;
         }
       }
-      hipEventRecord(ev[3], stream);
-      {
-        // bytes of matrix written by the block (upper triangle when same)
-        double nbf = (double)Sa.NumBFs * (double)Sb.NumBFs;
-        int tok = prof_begin(stream, PK_STAMP);
-        HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same,
-                            Sa.IsPEC, Sb.IsPEC, nreg, dreg, needH1, d_HP_, d_HN_, d_HS_,
-                            B.d_rowPtr, B.d_col, B.d_idx, M, G_.BFIndexOffset[sa], G_.BFIndexOffset[sb]));
-        prof_end(stream, tok, 16.0 * (same ? 0.5 * nbf : nbf));
-      }
-      hipEventRecord(ev[4], stream);
-      HIPCHK(hipEventSynchronize(ev[4]));
+      hipEventRecord(ev[2], stream);
+      HIPCHK(hipEventSynchronize(ev[2]));
       float ms;
       hipEventElapsedTime(&ms, ev[0], ev[1]); t_self += ms;
-      hipEventElapsedTime(&ms, ev[1], ev[2]); t_far += ms;
-      hipEventElapsedTime(&ms, ev[2], ev[3]); t_near += ms;
-      hipEventElapsedTime(&ms, ev[3], ev[4]); t_stamp += ms;
+      hipEventElapsedTime(&ms, ev[1], ev[2]); t_near += ms;
+      // far records and matrix entries, one column band at a time
+      int rc3 = far_and_stamp(B, Da, Db, Sa, Sb, nreg, dreg, needH1, M, G_.BFIndexOffset[sa], G_.BFIndexOffset[sb],
+                              stream, &t_far, &t_stamp);
+      if (rc3) {
+        for (auto &e : ev) hipEventDestroy(e);
+        return rc3;
+      }
       stats_.nfar += nfar_blk;
       stats_.nnear += (long)B.pairs.size();
     }
@@ -696,8 +834,6 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
   const int needH1 = !(Sa.IsPEC && Sb.IsPEC);
   const int stride = needH1 ? 4 : 2;
   HIPCHK(hipMemcpyAsync(d_regs_, rk, sizeof(RegionK) * std::max(nreg, 1), hipMemcpyHostToDevice, stream));
-  long npairs = same ? (long)Da.nP * (Da.nP - 1) / 2 : (long)Da.nP * Db.nP;
-  if (ensure(&d_HP_, &cap_HP_, (size_t)npairs * nreg * 9 * stride)) return -2;
   if (ensure(&d_HN_, &cap_HN_, (size_t)NB.pairs.size() * nreg * 9 * stride)) return -2;
   if (same && nreg > 0) {
     const int chunk = 16384;
@@ -717,16 +853,16 @@ int Assembler::assemble_block(int sa, int sb, cdouble omega, double2 *B, long ld
     ldt = Sa.NumBFs;
   }
   if (nreg > 0) {
-    HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, Da.nP, Db.V, Db.PVI, Db.PC, Db.nP, same, NB.d_rowPtr, NB.d_col,
-                      nreg, d_regs_, needH1, d_HP_));
     if (!NB.pairs.empty())
       HIPCHK(launch_near(stream, NB.d_pairs, (int)NB.pairs.size(), d_records_, nreg, d_regs_, needH1, d_HN_));
     int rc2 = run_sw(lo, hi, bi, rk, nreg, d_regs_, needH1, stream);
     if (rc2) return rc2;
   }
-  HIPCHK(launch_stamp(stream, Da.nE, Da.EI, Da.EL, Da.nP, Db.nE, Db.EI, Db.EL, Db.nP, same, Sa.IsPEC, Sb.IsPEC,
-                      nreg, d_regs_, needH1, d_HP_, d_HN_, d_HS_, NB.d_rowPtr, NB.d_col, NB.d_idx,
-                      MatOut{target, ldt, MAT_C128}, 0, 0));
+  {
+    int rc3 = far_and_stamp(NB, Da, Db, Sa, Sb, nreg, d_regs_, needH1, MatOut{target, ldt, MAT_C128}, 0, 0, stream,
+                            nullptr, nullptr);
+    if (rc3) return rc3;
+  }
   if (same) HIPCHK(launch_mirror(stream, MatOut{B, ldb, MAT_C128}, Sa.NumBFs));
   if (stage) {
     dim3 grid((Sa.NumBFs + 31) / 32, (Sb.NumBFs + 31) / 32);

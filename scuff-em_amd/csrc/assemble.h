@@ -21,6 +21,25 @@ struct DevSurface {
   int nV = 0, nP = 0, nE = 0;
 };
 
+// Column bands of one surface-pair block (frequency independent).  Band k
+// holds the column edges whose higher panel index lies in a range of `ch`
+// panels, the sorted list of their panels (the band's slots) and, per
+// column edge, the slots of its two panels.  Its far records pair every
+// row panel pa < rowLimit[k] with every slot; its stamp covers row edges
+// < rowEdges[k] (the upper triangle of a diagonal block ends at the
+// band's last column).  One band's records take at most
+// maxSlots x rowsCap records, instead of the whole block's panel pairs.
+struct FarBands {
+  int ch = 0, maxSlots = 0, rowsCap = 0;
+  std::vector<int> colStart, cols, colSlots;   // colSlots: 2 per column edge
+  std::vector<int> slotStart, slots;
+  std::vector<int> rowLimit, rowEdges;
+  std::vector<long> farPairs;                  // far pairs computed per band
+  long uniqueFar = 0;                          // far panel pairs of the block, each once
+  int *d_cols = nullptr, *d_colSlots = nullptr, *d_slots = nullptr;
+  int count() const { return (int)rowLimit.size(); }
+};
+
 // near-pair bookkeeping of one surface-pair block
 struct NearBlock {
   int sa = 0, sb = 0;
@@ -28,6 +47,7 @@ struct NearBlock {
   std::vector<NearPair> pairs;         // first-use order
   int *d_rowPtr = nullptr, *d_col = nullptr, *d_idx = nullptr;
   NearPair *d_pairs = nullptr;
+  FarBands bands;
 };
 
 struct AsmStats {
@@ -48,6 +68,9 @@ class Assembler {
   // its derivatives with respect to a rigid motion of surface sa
   int assemble_block(int sa, int sb, cdouble omega, double2 *B, long ldb, double2 *GB, long ldg, long gstride,
                      int ntorque, const double *gamma, double2 *TB, hipStream_t stream);
+  // r >= 0: later assemblies keep only the contributions of region r
+  // (its terms of every common-region sum); -1: every region
+  void set_region_filter(int r) { region_only_ = r; }
   const AsmStats &stats() const { return stats_; }
   const DevSurface &dev_surface(int s) const { return ds_[s]; }
   std::string error() const { return err_; }
@@ -72,7 +95,16 @@ class Assembler {
   size_t cap_swjobs_ = 0;
   AsmStats stats_;
   std::string err_;
+  std::vector<hipEvent_t> evpool_;   // timing events of the band loop
+  int region_only_ = -1;
   void build_near();
+  void build_bands(NearBlock &B);
+  // far records and matrix entries of block B, band by band (the near,
+  // self and short-wavelength records must be in place); adds the far and
+  // stamp times to *t_far / *t_stamp when they are given
+  int far_and_stamp(NearBlock &B, const DevSurface &Da, const DevSurface &Db, const Surface &Sa,
+                    const Surface &Sb, int nreg, const RegionK *dreg, int needH1, MatOut M, long rowOff,
+                    long colOff, hipStream_t stream, double *t_far, double *t_stamp);
   int regions_for(int sa, int sb, const std::vector<cdouble> &eps, const std::vector<cdouble> &mu, cdouble omega,
                   RegionK rk[2]);
   int ensure(double **p, size_t *cap, size_t need);

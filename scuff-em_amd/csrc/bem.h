@@ -8,9 +8,10 @@
 //   EI     int[4*nE]                   iPPanel, iMPanel, PIndex, MIndex
 //   EL     double[nE]                  edge length
 // Panel-pair integrals, one record per (panel pair, region):
-//   far    unordered pairs pa<pb (same surface, packed upper triangle)
-//          or all pa x pb (distinct surfaces); 9 (Qa,Qb) combinations x
-//          {H0, H1} complex = 36 doubles (H1 omitted for PEC-PEC blocks)
+//   far    per column band of the block: ordered pairs (pa, pb) of every
+//          row panel pa < rowLimit with the band's column panels (slots),
+//          HB[slot][pa]; 9 (Qa,Qb) combinations x {H0, H1} complex per
+//          region = 36 doubles (H1 omitted for PEC-PEC blocks)
 //   near   ordered pairs (pa as the row panel), same record layout
 //   self   panel p with itself, 9 combinations, H0 only
 // The matrix M is column-major complex128, M[r + c*ldm].
@@ -70,10 +71,11 @@ struct RegionK {
 };
 
 // launch wrappers (kern_*.hip); all asynchronous on `stream`
-hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int nPa,
-                      const double *Vb, const int *PVIb, const double *PCb, int nPb, int same,
-                      const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
-                      int needH1, double *HP);
+// far records of one column band: rows pa < rowLimit x the band's slot panels
+hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int rowLimit,
+                      const double *Vb, const int *PVIb, const double *PCb, const int *slotPanels, int nslots,
+                      int same, const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
+                      int needH1, double *HB, long rowsCap);
 hipError_t launch_near(hipStream_t stream, const NearPair *np, int nnear, const double *records,
                        int nreg, const RegionK *regs, int needH1, double *HN);
 hipError_t launch_self(hipStream_t stream, const double *V, const int *PVI, int nP, int nreg,
@@ -81,10 +83,12 @@ hipError_t launch_self(hipStream_t stream, const double *V, const int *PVI, int 
 hipError_t launch_fippi(hipStream_t stream, const FippiJob *jobs, int njobs, double *records,
                         double *heapData, int *heapSlot, double *heapKey, int cap, int nslots,
                         int *counter, int *overflow, const int *jobList);
-hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const double *ELa, int nPa,
-                        int nEb, const int *EIb, const double *ELb, int nPb, int same,
+// matrix entries of one column band: its column edges bandCols[0..ncols),
+// the slots of their two panels colSlots[2 j + t], row edges < rowEdges
+hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const double *ELa, const int *bandCols,
+                        const int *colSlots, int ncols, int rowEdges, const int *EIb, const double *ELb, int same,
                         int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
-                        const double *HP, const double *HN, const double *HS,
+                        const double *HB, long rowsCap, const double *HN, const double *HS,
                         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
                         MatOut M, int rowOff, int colOff);
 hipError_t launch_mirror(hipStream_t stream, MatOut M, int n);

@@ -63,6 +63,8 @@ struct scuffx_geom {
   bool transformed = false;     // a surface moved since the last UnTransform
   std::vector<DevRhs> rhs;
   double *d_out = nullptr;
+  signed char *d_bft = nullptr;   // basis-function types for the EMT power
+  double *d_emt = nullptr;        // its column partial sums (TotalBFs + 1)
   scuffx_geom()
   {
     std::lock_guard<std::mutex> lk(g_geom_mu);
@@ -87,8 +89,13 @@ struct scuffx_geom {
   }
   void drop_rhs()
   {
-    if (!hip_down())
+    if (!hip_down()) {
       for (auto &r : rhs) { hipFree(r.PA); hipFree(r.PZ); hipFree(r.work); hipFree(r.PEI); hipFree(r.EV); }
+      hipFree(d_bft);
+      hipFree(d_emt);
+    }
+    d_bft = nullptr;
+    d_emt = nullptr;
     rhs.clear();
   }
   // the geometry moved: device copies are rebuilt on their next use
@@ -743,6 +750,57 @@ int scuffx_power(scuffx_geom *G, double wr, double wi, const void *KN, const voi
   }
   if (PExt) *PExt = tot[0];
   if (PAbs) *PAbs = tot[1];
+  return 0;
+}
+
+int scuffx_assemble_region(scuffx_geom *G, double wr, double wi, int region, void *M, long ldm)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!G || !M || G->G.Surfaces.empty()) return fail(SCUFFX_ERR_ARG, "bad arguments");
+  if (wr == 0.0 && wi == 0.0) return fail(SCUFFX_ERR_ARG, "zero frequency");
+  if (region < 0 || region >= (int)G->G.Regions.size()) return fail(SCUFFX_ERR_ARG, "region index out of range");
+  Assembler *A = G->assembler();
+  A->set_region_filter(region);
+  int rc = A->assemble(cdouble(wr, wi), M, ldm, MAT_C128, g_stream);
+  A->set_region_filter(-1);
+  if (rc) return fail(rc, A->error());
+  return 0;
+}
+
+int scuffx_power_emt(scuffx_geom *G, double wr, double wi, const void *KN, const void *rhs, const void *Mext,
+                     long ldm, double *PAbs, double *PScat)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!G || !KN || !rhs || !Mext) return fail(SCUFFX_ERR_ARG, "null argument");
+  const int n = G->G.TotalBFs;
+  if (ldm < n) return fail(SCUFFX_ERR_ARG, "leading dimension smaller than the matrix");
+  for (const Surface &S : G->G.Surfaces)
+    if (S.RegionIndices[0] != 0) return fail(SCUFFX_ERR_UNSUPPORTED, "EMT power needs every surface in the exterior region");
+  double pext = 0.0, pabs_opft = 0.0;
+  int rc = scuffx_power(G, wr, wi, KN, rhs, &pabs_opft, &pext);
+  if (rc) return rc;
+  if (!G->d_bft) {
+    std::vector<signed char> t(n);
+    for (size_t s = 0; s < G->G.Surfaces.size(); s++) {
+      const Surface &S = G->G.Surfaces[s];
+      const int off = G->G.BFIndexOffset[s];
+      for (int e = 0; e < S.NumEdges(); e++) {
+        if (S.IsPEC) t[off + e] = 0;
+        else { t[off + 2 * e] = 1; t[off + 2 * e + 1] = 2; }
+      }
+    }
+    if (hipMalloc(&G->d_bft, n) != hipSuccess || hipMalloc(&G->d_emt, sizeof(double) * (n + 1)) != hipSuccess)
+      return fail(SCUFFX_ERR_ALLOC, "allocation failed");
+    hipMemcpy(G->d_bft, t.data(), n, hipMemcpyHostToDevice);
+  }
+  if (launch_emt(g_stream, (const double2 *)Mext, ldm, (const double2 *)KN, G->d_bft, n, G->d_emt, G->d_emt + n) !=
+      hipSuccess)
+    return fail(SCUFFX_ERR_HIP, "EMT power kernel failed");
+  double ps = 0.0;
+  hipMemcpyAsync(&ps, G->d_emt + n, sizeof(double), hipMemcpyDeviceToHost, g_stream);
+  if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "EMT power kernel failed");
+  if (PScat) *PScat = ps;
+  if (PAbs) *PAbs = pext - ps;
   return 0;
 }
 

@@ -1,6 +1,7 @@
 // geometry.cpp -- host-side mesh topology (see geometry.h).
 #include "geometry.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -44,7 +45,7 @@ static void init_panel(Panel *P, const double *V)
   P->ZHat[1] = A[2] * B[0] - A[0] * B[2];
   P->ZHat[2] = A[0] * B[1] - A[1] * B[0];
   double n = std::sqrt(P->ZHat[0] * P->ZHat[0] + P->ZHat[1] * P->ZHat[1] + P->ZHat[2] * P->ZHat[2]);
-  double s = 1.0 / n;
+  double s = (P->ZHatFlipped ? -1.0 : 1.0) / n;
   for (int i = 0; i < 3; i++) P->ZHat[i] *= s;
   P->Area = 0.5 * n;
 }
@@ -160,7 +161,101 @@ int add_surface(Geometry *G, Surface &&S, const Material &interior)
   G->BFIndexOffset.push_back(G->TotalBFs);
   G->TotalBFs += S.NumBFs;
   G->Surfaces.push_back(std::move(S));
+  orient_normals(G);
   return ns;
+}
+
+// Vertical-ray piercing counts of one surface's panels: triangles binned
+// by their xy bounding boxes on a uniform grid, a query walks its cell.
+namespace {
+struct PierceGrid {
+  const Surface *S = nullptr;
+  double lo[2] = {0, 0}, h[2] = {1, 1};
+  int n = 1;
+  std::vector<int> start, tri;
+  explicit PierceGrid(const Surface &Sf) : S(&Sf)
+  {
+    const int nP = Sf.NumPanels();
+    if (nP == 0) return;
+    double hi[2] = {-1e300, -1e300};
+    lo[0] = lo[1] = 1e300;
+    for (int v = 0; v < Sf.NumVertices(); v++)
+      for (int j = 0; j < 2; j++) { lo[j] = std::min(lo[j], Sf.V[3 * v + j]); hi[j] = std::max(hi[j], Sf.V[3 * v + j]); }
+    n = std::max(1, (int)std::sqrt((double)nP));
+    for (int j = 0; j < 2; j++) h[j] = std::max(hi[j] - lo[j], 1e-300) / n;
+    std::vector<std::vector<int>> cells((size_t)n * n);
+    for (int p = 0; p < nP; p++) {
+      int c0[2], c1[2];
+      for (int j = 0; j < 2; j++) {
+        double a = 1e300, b = -1e300;
+        for (int k = 0; k < 3; k++) { double x = Sf.V[3 * Sf.Panels[p].VI[k] + j]; a = std::min(a, x); b = std::max(b, x); }
+        c0[j] = cell(a, j); c1[j] = cell(b, j);
+      }
+      for (int x = c0[0]; x <= c1[0]; x++)
+        for (int y = c0[1]; y <= c1[1]; y++) cells[(size_t)x * n + y].push_back(p);
+    }
+    start.assign((size_t)n * n + 1, 0);
+    for (size_t c = 0; c < cells.size(); c++) start[c + 1] = start[c] + (int)cells[c].size();
+    tri.reserve(start.back());
+    for (auto &c : cells) tri.insert(tri.end(), c.begin(), c.end());
+  }
+  int cell(double x, int j) const { return std::max(0, std::min(n - 1, (int)std::floor((x - lo[j]) / h[j]))); }
+  // triangles crossed by the ray from X toward z = -infinity
+  int piercings(const double X[3]) const
+  {
+    if (start.empty()) return 0;
+    const int c = cell(X[0], 0) * n + cell(X[1], 1);
+    int count = 0;
+    for (int q = start[c]; q < start[c + 1]; q++) {
+      const Panel &P = S->Panels[tri[q]];
+      const double *A = &S->V[3 * P.VI[0]], *B = &S->V[3 * P.VI[1]], *C = &S->V[3 * P.VI[2]];
+      const double d = (B[0] - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (B[1] - A[1]);
+      if (d == 0.0) continue;                         // vertical panel
+      const double u = ((X[0] - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (X[1] - A[1])) / d;
+      const double v = ((B[0] - A[0]) * (X[1] - A[1]) - (X[0] - A[0]) * (B[1] - A[1])) / d;
+      if (u < 0.0 || v < 0.0 || u + v > 1.0) continue;
+      const double z = A[2] + u * (B[2] - A[2]) + v * (C[2] - A[2]);
+      if (z < X[2]) count++;
+    }
+    return count;
+  }
+};
+}  // namespace
+
+// A point just off the panel along its normal should lie in the surface's
+// exterior region; if it lies in the interior instead, the normal is
+// flipped.  Region membership counts the piercings of a vertical ray with
+// every non-PEC surface bounding the region: region 0 for an even count,
+// an interior region for an odd one (PointInRegion, PointInObject.cc:620-657).
+void orient_normals(Geometry *G)
+{
+  const int NS = (int)G->Surfaces.size();
+  std::vector<PierceGrid> grids;
+  grids.reserve(NS);
+  for (int s = 0; s < NS; s++) grids.emplace_back(G->Surfaces[s]);
+  auto in_region = [&](int r, const double X[3]) {
+    int total = 0;
+    for (int s = 0; s < NS; s++) {
+      const Surface &S = G->Surfaces[s];
+      if (S.IsPEC) continue;
+      if (S.RegionIndices[0] == r || S.RegionIndices[1] == r) total += grids[s].piercings(X);
+    }
+    return r == 0 ? (total % 2 == 0) : (total % 2 == 1);
+  };
+  for (int s = 0; s < NS; s++) {
+    Surface &S = G->Surfaces[s];
+    for (Panel &P : S.Panels) {
+      P.ZHatFlipped = false;
+      init_panel(&P, S.V.data());
+      double X[3];
+      for (int i = 0; i < 3; i++) X[i] = P.Centroid[i] + 0.5 * P.Radius * P.ZHat[i];
+      if (in_region(S.RegionIndices[0], X)) continue;
+      if (in_region(S.RegionIndices[1], X)) {
+        P.ZHatFlipped = true;
+        for (double &z : P.ZHat) z = -z;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------

@@ -22,6 +22,7 @@ struct Panel {
   double ZHat[3];
   double Radius;
   double Area;
+  bool ZHatFlipped = false;   // normal reversed to point into the exterior region
 };
 
 struct Edge {
@@ -113,6 +114,9 @@ void untransform_geometry(Geometry *G);
 std::string build_surface(const double *V, int nV, const int *T, int nT,
                           bool pec, Surface *S);
 int add_surface(Geometry *G, Surface &&S, const Material &interior);
+// panel normals made to point into each surface's exterior region
+// (RWGGeometry.cc:653-690); called by add_surface for every surface
+void orient_normals(Geometry *G);
 // Mate[ns] = the first earlier surface identical to ns, or -1: same mesh
 // file, same exterior medium, both PEC or the same interior material
 // (RWGGeometry.cc:733-770; MESHTAG is not supported, so tags always agree)

@@ -17,6 +17,8 @@
 // equal the reference's per-Q sums up to rounding.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bem.h"
 
 namespace scuffx {
@@ -36,12 +38,6 @@ __device__ inline int near_find(const int *rowPtr, const int *col, int pa, int p
     if (c < pb) lo = mid + 1; else hi = mid;
   }
   return -1;
-}
-
-__device__ inline long long pair_index(int pa, int pb, int nPa, int nPb, int same)
-{
-  if (same) return (long long)pa * nPa - (long long)pa * (pa + 1) / 2 + (pb - pa - 1);
-  return (long long)pa * nPb + pb;
 }
 
 // moments of one panel pair for one wavenumber (far: Phi = e^{ikr}/4pi r)
@@ -126,18 +122,25 @@ __device__ inline void combos_from_moments(const Moments &m, const double *Qa, c
 }
 
 // ---------------------------------------------------------------------
-// K1: far panel pairs.  grid.y = row panel pa, threads over pb.
+// K1: far panel pairs of one column band.  The band is a list of column
+// panels (slots); every row panel pa < rowLimit is paired with every slot
+// panel pb in the given order (row panel first), skipping the pair of a
+// panel with itself and the near pairs.  Record of (pa, slot) at
+// HB[(slot * rowsCap + pa) * nreg * 9 * stride]: for each region the 9
+// (Qa, Qb) combinations 3 qa + qb of {H0[, H1]}.
+// grid.x over row panels, grid.y = slot.
 // ---------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_far(const double *Va, const int *PVIa, const double *PCa, int nPa,
-      const double *Vb, const int *PVIb, const double *PCb, int nPb, int same,
+k_far(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
+      const double *Vb, const int *PVIb, const double *PCb, const int *slotPanels, int same,
       const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
-      int needH1, double *HP)
+      int needH1, double *HB, long rowsCap)
 {
-  int pa = blockIdx.y;
-  int pb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pa >= nPa || pb >= nPb) return;
-  if (same && pb <= pa) return;
+  const int slot = blockIdx.y;
+  const int pa = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pa >= rowLimit) return;
+  const int pb = slotPanels[slot];
+  if (same && pb == pa) return;
   if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) return;
   double ca[3] = {PCa[4 * pa], PCa[4 * pa + 1], PCa[4 * pa + 2]};
   double cb[3] = {PCb[4 * pb], PCb[4 * pb + 1], PCb[4 * pb + 2]};
@@ -155,9 +158,8 @@ k_far(const double *Va, const int *PVIa, const double *PCa, int nPa,
     Qb[c] = B0[c] - cb[c]; Qb[3 + c] = B1[c] - cb[c]; Qb[6 + c] = B2[c] - cb[c];
     D[c] = ca[c] - cb[c];
   }
-  long long pidx = pair_index(pa, pb, nPa, nPb, same);
-  int stride = needH1 ? 4 : 2;
-  double *out = HP + pidx * (long long)(nreg * 9 * stride);
+  const int stride = needH1 ? 4 : 2;
+  double *out = HB + ((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride);
   for (int r = 0; r < nreg; r++) {
     Moments m;
     cplx k = regs[r].k;
@@ -172,15 +174,15 @@ k_far(const double *Va, const int *PVIa, const double *PCa, int nPa,
   }
 }
 
-hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int nPa,
-                      const double *Vb, const int *PVIb, const double *PCb, int nPb, int same,
-                      const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
-                      int needH1, double *HP)
+hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int rowLimit,
+                      const double *Vb, const int *PVIb, const double *PCb, const int *slotPanels, int nslots,
+                      int same, const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
+                      int needH1, double *HB, long rowsCap)
 {
-  if (nPa <= 0 || nPb <= 0 || nreg <= 0) return hipSuccess;
-  dim3 grid((nPb + 255) / 256, nPa);
-  hipLaunchKernelGGL(k_far, grid, dim3(256), 0, stream, Va, PVIa, PCa, nPa, Vb, PVIb, PCb, nPb, same,
-                     nearRowPtr, nearCol, nreg, regs, needH1, HP);
+  if (rowLimit <= 0 || nslots <= 0 || nreg <= 0) return hipSuccess;
+  dim3 grid((rowLimit + 255) / 256, nslots);
+  hipLaunchKernelGGL(k_far, grid, dim3(256), 0, stream, Va, PVIa, PCa, rowLimit, Vb, PVIb, PCb, slotPanels, same,
+                     nearRowPtr, nearCol, nreg, regs, needH1, HB, rowsCap);
   return hipGetLastError();
 }
 
@@ -303,16 +305,17 @@ hipError_t launch_near(hipStream_t stream, const NearPair *np, int nnear, const 
 }
 
 // ---------------------------------------------------------------------
-// K2: edge pairs.  GC0 = LaLb(HPP-HPM-HMP+HMM)[0],
+// K2: edge pairs of one column band.  GC0 = LaLb(HPP-HPM-HMP+HMM)[0],
 // GC1 = LaLb/(ik)(...)[1] (EdgeEdgeInteractions.cc:154-188); EFIE entry
 // i w mu s GC0, PMCHWT 2x2 block [[i w mu G, -ik C], [-ik C, -i w eps G]]
 // summed over the common regions (SurfaceSurfaceInteractions.cc:162-327).
-// Threads run down a column of the block (coalesced column-major
-// writes); same-surface blocks fill only the upper triangle.
+// grid.y = column edge of the band, threads run down its column (coalesced
+// column-major writes); same-surface blocks fill only the upper triangle.
+// A panel pair's record comes from the self buffer (a panel with itself),
+// the near buffer, or the band's far records (slot of the column panel).
 // ---------------------------------------------------------------------
-__device__ inline void fetch_h(int pa, int qa, int pb, int qb, int r, int nreg, int stride,
-                               int nPa, int nPb, int same,
-                               const double *HP, const double *HN, const double *HS,
+__device__ inline void fetch_h(int pa, int qa, int pb, int qb, int slot, int r, int nreg, int stride, int same,
+                               const double *HB, long rowsCap, const double *HN, const double *HS,
                                const int *nearRowPtr, const int *nearCol, const int *nearIdx,
                                cplx *h0, cplx *h1)
 {
@@ -331,29 +334,29 @@ __device__ inline void fetch_h(int pa, int qa, int pb, int qb, int r, int nreg, 
       return;
     }
     rec = HN + ((long long)id * nreg + r) * 9 * stride + (3 * qa + qb) * stride;
-  } else if (same && pb < pa) {
-    rec = HP + pair_index(pb, pa, nPa, nPb, same) * (nreg * 9 * stride) + (r * 9 + 3 * qb + qa) * stride;
   } else {
-    rec = HP + pair_index(pa, pb, nPa, nPb, same) * (nreg * 9 * stride) + (r * 9 + 3 * qa + qb) * stride;
+    rec = HB + ((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride) + (r * 9 + 3 * qa + qb) * stride;
   }
   *h0 = mk(rec[0], rec[1]);
   *h1 = stride == 4 ? mk(rec[2], rec[3]) : mk(0, 0);
 }
 
 __global__ void __launch_bounds__(256)
-k_stamp(int nEa, const int *EIa, const double *ELa, int nPa, int nEb, const int *EIb, const double *ELb,
-        int nPb, int same, int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
-        const double *HP, const double *HN, const double *HS,
+k_stamp(int nEa, const int *EIa, const double *ELa, const int *bandCols, const int *colSlots, const int *EIb,
+        const double *ELb, int same, int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
+        const double *HB, long rowsCap, const double *HN, const double *HS,
         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
         MatOut M, int rowOff, int colOff)
 {
-  int ea = blockIdx.x * blockDim.x + threadIdx.x;
-  int eb = blockIdx.y;
-  if (ea >= nEa || eb >= nEb) return;
+  const int ea = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (ea >= nEa) return;
+  const int eb = bandCols[j];
   if (same && ea > eb) return;
   int stride = needH1 ? 4 : 2;
   int pA[2] = {EIa[4 * ea], EIa[4 * ea + 1]}, qA[2] = {EIa[4 * ea + 2], EIa[4 * ea + 3]};
   int pB[2] = {EIb[4 * eb], EIb[4 * eb + 1]}, qB[2] = {EIb[4 * eb + 2], EIb[4 * eb + 3]};
+  int sB[2] = {colSlots[2 * j], colSlots[2 * j + 1]};
   double LL = ELa[ea] * ELb[eb];
   cplx m00 = mk(0, 0), m01 = mk(0, 0), m11 = mk(0, 0);
   for (int r = 0; r < nreg; r++) {
@@ -362,7 +365,7 @@ k_stamp(int nEa, const int *EIa, const double *ELa, int nPa, int nEb, const int 
       for (int t = 0; t < 2; t++) {
         G[2 * s + t] = mk(0, 0); C[2 * s + t] = mk(0, 0);
         if (pA[s] < 0 || pB[t] < 0) continue;
-        fetch_h(pA[s], qA[s], pB[t], qB[t], r, nreg, stride, nPa, nPb, same, HP, HN, HS,
+        fetch_h(pA[s], qA[s], pB[t], qB[t], sB[t], r, nreg, stride, same, HB, rowsCap, HN, HS,
                 nearRowPtr, nearCol, nearIdx, &G[2 * s + t], &C[2 * s + t]);
       }
     cplx k = regs[r].k;
@@ -391,18 +394,18 @@ k_stamp(int nEa, const int *EIa, const double *ELa, int nPa, int nEb, const int 
   }
 }
 
-hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const double *ELa, int nPa,
-                        int nEb, const int *EIb, const double *ELb, int nPb, int same,
+hipError_t launch_stamp(hipStream_t stream, int nEa, const int *EIa, const double *ELa, const int *bandCols,
+                        const int *colSlots, int ncols, int rowEdges, const int *EIb, const double *ELb, int same,
                         int pecA, int pecB, int nreg, const RegionK *regs, int needH1,
-                        const double *HP, const double *HN, const double *HS,
+                        const double *HB, long rowsCap, const double *HN, const double *HS,
                         const int *nearRowPtr, const int *nearCol, const int *nearIdx,
                         MatOut M, int rowOff, int colOff)
 {
-  if (nEa <= 0 || nEb <= 0) return hipSuccess;
-  dim3 grid((nEa + 255) / 256, nEb);
-  hipLaunchKernelGGL(k_stamp, grid, dim3(256), 0, stream, nEa, EIa, ELa, nPa, nEb, EIb, ELb, nPb,
-                     same, pecA, pecB, nreg, regs, needH1, HP, HN, HS, nearRowPtr, nearCol, nearIdx,
-                     M, rowOff, colOff);
+  if (nEa <= 0 || ncols <= 0 || rowEdges <= 0) return hipSuccess;
+  dim3 grid((std::min(nEa, rowEdges) + 255) / 256, ncols);
+  hipLaunchKernelGGL(k_stamp, grid, dim3(256), 0, stream, nEa, EIa, ELa, bandCols, colSlots, EIb, ELb, same, pecA,
+                     pecB, nreg, regs, needH1, HB, rowsCap, HN, HS, nearRowPtr, nearCol, nearIdx, M, rowOff,
+                     colOff);
   return hipGetLastError();
 }
 

@@ -84,14 +84,15 @@ __device__ inline double mag1(double a) { return fabs(a); }
 template <typename T> __device__ inline auto mag1(T a) -> decltype(a.x) { return fabs(a.x) + fabs(a.y); }
 __device__ inline bool nonzero(double a) { return a != 0; }
 template <typename T> __device__ inline bool nonzero(T a) { return a.x != 0 || a.y != 0; }
-template <typename T> __device__ inline T zero_v() { T r = zero_v<T>(); return r; }
+template <typename T> __device__ inline T zero_v() { T r; r.x = 0; r.y = 0; return r; }
 template <> __device__ inline double zero_v<double>() { return 0; }
 template <typename T> __device__ inline T unit_v() { T r; r.x = 1; r.y = 0; return r; }
 template <> __device__ inline double unit_v<double>() { return 1; }
 // acc += a b, real and imaginary parts each as one expression
 template <typename T> __device__ inline void mac(T &acc, T a, T b)
 {
-  mac(acc, a, b);
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
 }
 __device__ inline void mac(double &acc, double a, double b) { acc += a * b; }
 // acc += conj(a) b (conj = identity for real)
@@ -1927,6 +1928,12 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
   if (t == 0) __hip_atomic_store(flags + g, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__global__ void k_zero_u64(unsigned long long *w, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = 0ull;
+}
+
 // lower (forward) then upper (backward) solve of one right-hand side in
 // two cooperative launches; returns -1 if it cannot run (grid not
 // co-resident, workspace), -6 on a time-out
@@ -1947,7 +1954,11 @@ static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *inv
   if (!w) return -1;
   unsigned *err = (unsigned *)w;
   unsigned long long *fl = (unsigned long long *)(w + 64), *fu = fl + nblk;
-  hipMemsetAsync(w, 0, sizeof(unsigned long long) * (2 * nblk + 8), s);
+  // cleared by a kernel (not a memset node, so that a captured sequence
+  // clears it on every replay)
+  const long nwords = 2 * nblk + 8;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, (unsigned long long *)w,
+                     nwords);
   unsigned long long epoch = 1;
   void *al[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invL, (void *)&B, (void *)&W, (void *)&fl,
                 (void *)&err, (void *)&epoch};

@@ -18,4 +18,10 @@ hipError_t launch_power(hipStream_t s, const double *V, const int *PVI, const do
                         const int *PEI, const int *EI, const double *EL, int nE, int pec, const double2 *KN,
                         const double2 *RHS, double *work, double *out2);
 
+// EMT scattered power of the solution x from the exterior-region matrix
+// Mext (n x n); bft: basis-function types (0 K on PEC, 1 K, 2 N); work: n
+// doubles; *out (device) = PScat
+hipError_t launch_emt(hipStream_t s, const double2 *Mext, long ldm, const double2 *x, const signed char *bft, int n,
+                      double *work, double *out);
+
 }  // namespace scuffx

@@ -161,4 +161,61 @@ hipError_t launch_power(hipStream_t s, const double *V, const int *PVI, const do
   return hipGetLastError();
 }
 
+// Scattered power by energy/momentum transfer (EMTPFT.cc:629-803, integral
+// method EHDERIVATIVES2 :334-362) from the exterior-region matrix Mext and
+// the solution x: PScat = ZVAC/2 sum_ij w_ij with, by basis-function type
+// (0 = K on a PEC surface, 1 = K on a dielectric surface, 2 = N),
+//   K,K: -Re(conj(x_i) x_j) Re(M_ij)     N,N: +Re(conj(x_i) x_j) Re(M_ij)
+//   1,2: +Im(conj(x_i) x_j) Im(M_ij)     2,1: -Im(conj(x_i) x_j) Im(M_ij)
+//   0,2 and 2,0: nothing (a PEC surface carries no N).
+// One workgroup per column; partial sums per column, then one workgroup
+// adds them in a fixed order (deterministic).
+__global__ void __launch_bounds__(256)
+k_emt_cols(const double2 *M, long ldm, const double2 *x, const signed char *bft, int n, double *part)
+{
+  const int j = blockIdx.x;
+  const double2 xj = x[j];
+  const int tj = bft[j];
+  const double2 *col = M + (long)j * ldm;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int ti = bft[i];
+    const double2 xi = x[i], m = col[i];
+    const double re = xi.x * xj.x + xi.y * xj.y;   // Re(conj(x_i) x_j)
+    const double im = xi.x * xj.y - xi.y * xj.x;   // Im(conj(x_i) x_j)
+    if (ti < 2 && tj < 2) acc -= re * m.x;
+    else if (ti == 2 && tj == 2) acc += re * m.x;
+    else if (ti == 1 && tj == 2) acc += im * m.y;
+    else if (ti == 2 && tj == 1) acc -= im * m.y;
+  }
+  __shared__ double s[256];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) {
+    if (threadIdx.x < q) s[threadIdx.x] += s[threadIdx.x + q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[j] = s[0];
+}
+
+__global__ void k_sum1(const double *in, int n, double scale, double *out)
+{
+  __shared__ double s[256];
+  double a = 0;
+  for (int i = threadIdx.x; i < n; i += 256) a += in[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) { if (threadIdx.x < q) s[threadIdx.x] += s[threadIdx.x + q]; __syncthreads(); }
+  if (threadIdx.x == 0) *out = scale * s[0];
+}
+
+hipError_t launch_emt(hipStream_t s, const double2 *Mext, long ldm, const double2 *x, const signed char *bft, int n,
+                      double *work, double *out)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emt_cols, dim3((unsigned)n), dim3(256), 0, s, Mext, ldm, x, bft, n, work);
+  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(256), 0, s, work, n, 0.5 * ZVAC_, out);
+  return hipGetLastError();
+}
+
 }  // namespace scuffx

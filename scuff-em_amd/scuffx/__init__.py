@@ -103,6 +103,8 @@ def lib():
             "scuffx_lu_invert": ([vp, C.c_long, C.c_long, vp, C.c_int], C.c_int),
             "scuffx_rhs_planewave": ([vp, C.c_double, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_power": ([vp, C.c_double, C.c_double, vp, vp, dp, dp], C.c_int),
+            "scuffx_power_emt": ([vp, C.c_double, C.c_double, vp, vp, vp, C.c_long, dp, dp], C.c_int),
+            "scuffx_assemble_region": ([vp, C.c_double, C.c_double, C.c_int, vp, C.c_long], C.c_int),
             "scuffx_lu_logdet": ([vp, C.c_long, C.c_long, vp, C.c_int, dp, dp], C.c_int),
             "scuffx_trace_solve": ([vp, C.c_long, C.c_long, vp, vp, C.c_long, C.c_long, C.c_int, dp, dp], C.c_int),
             "scuffx_preload_cache": ([C.c_char_p], C.c_long),
@@ -830,13 +832,41 @@ class RWGGeometry:
                                           nh.ctypes.data_as(C.c_void_p), _ptr(RHS.t)))
         return RHS
 
-    def GetPower(self, Omega, KN, RHS):
-        """{'PAbs', 'PExt', 'PScat'} in the reference's power units."""
+    def GetPower(self, Omega, KN, RHS, Method="OPFT"):
+        """{'PAbs', 'PExt', 'PScat'} in the reference's power units.
+        Method "OPFT": absorbed power from the crossed overlap integrals
+        (OPFT.cc:310-364), PScat = PExt - PAbs.  Method "EMT": scattered
+        power by energy/momentum transfer (EMTPFT.cc:468-803, the
+        reference's default), PAbs = PExt - PScat; it assembles the
+        exterior region's matrix (one more N x N complex128 buffer)."""
         pa, pe = C.c_double(), C.c_double()
         Omega = complex(Omega)
         _sync_stream()
+        if Method.upper() == "EMT":
+            torch = _torch()
+            N = self.TotalBFs
+            Mext = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+            _check(lib().scuffx_assemble_region(self.h, Omega.real, Omega.imag, 0, _ptr(Mext), N))
+            ps = C.c_double()
+            _check(lib().scuffx_power_emt(self.h, Omega.real, Omega.imag, _ptr(KN.t), _ptr(RHS.t), _ptr(Mext), N,
+                                          C.byref(pa), C.byref(ps)))
+            del Mext
+            return {"PAbs": pa.value, "PScat": ps.value, "PExt": pa.value + ps.value}
+        if Method.upper() != "OPFT":
+            raise ValueError("Method must be 'OPFT' or 'EMT'")
         _check(lib().scuffx_power(self.h, Omega.real, Omega.imag, _ptr(KN.t), _ptr(RHS.t), C.byref(pa), C.byref(pe)))
         return {"PAbs": pa.value, "PExt": pe.value, "PScat": pe.value - pa.value}
+
+    def AssembleRegionMatrix(self, Omega, Region=0):
+        """the BEM matrix with one region's contributions only (0: the
+        exterior), complex128"""
+        torch = _torch()
+        N = self.TotalBFs
+        M = HMatrix(N, N, dtype="complex128")
+        Omega = complex(Omega)
+        _sync_stream()
+        _check(lib().scuffx_assemble_region(self.h, Omega.real, Omega.imag, Region, _ptr(M.t), N))
+        return M
 
 
 def BatchSweep(G, Omegas, IF, prec=64):

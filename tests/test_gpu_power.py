@@ -70,7 +70,8 @@ def test_chain_pec_l2_config1(sx, O):
     print(f"  P={P} oracle={Po}", flush=True)
     assert abs(P["PScat"] - Po["PScat"]) <= 1e-10 * abs(Po["PScat"])
     assert abs(P["PExt"] - Po["PExt"]) <= 1e-10 * abs(Po["PExt"])
-    assert abs(P["PScat"] - 8.488640e-03) <= 0.06 * 8.488640e-03
+    # mesh error of the level-2 icosphere (320 flat panels): -3.8%
+    assert abs(P["PScat"] - 8.488640e-03) <= 0.05 * 8.488640e-03
 
 
 def test_chain_dielectric_l2_vs_mie(sx, O):
@@ -78,8 +79,10 @@ def test_chain_dielectric_l2_vs_mie(sx, O):
     P, _ = solve_chain(sx, G, 1.0)
     qs, qe = O.mie(1.0, np.sqrt(4 + 0.1j))
     f = np.pi / (2 * O.ZVAC)
-    assert abs(P["PExt"] - qe * f) <= 0.1 * qe * f
-    assert abs(P["PAbs"] - (qe - qs) * f) <= 0.1 * (qe - qs) * f
+    # mesh error of the level-2 icosphere: PExt -6.3%, PAbs -5.0% (the
+    # chain itself is pinned to 1e-5 by the LossySphere_327 regression)
+    assert abs(P["PExt"] - qe * f) <= 0.08 * qe * f
+    assert abs(P["PAbs"] - (qe - qs) * f) <= 0.07 * (qe - qs) * f
 
 
 @pytest.mark.parametrize("level,omega", [(2, 1.0), (3, 1.0)])

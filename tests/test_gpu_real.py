@@ -44,7 +44,8 @@ def test_real_lu_matches_dgetrf(sx, n):
     la = M.LogDet()
     sign, ld = np.linalg.slogdet(A)
     assert abs(la.real - ld) <= 1e-10 * abs(ld)
-    assert abs(np.exp(1j * la.imag) - sign) <= 1e-12
+    # the phase sums arg(U_ii) and pi per interchange (up to ~n terms)
+    assert abs(np.exp(1j * la.imag) - sign) <= 1e-14 * n
 
 
 def test_real_lu_singular_info(sx):

@@ -9,7 +9,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, ROOT
+from conftest import GOLDEN, ROOT, read_msh
 
 HEADER = os.path.join(ROOT, "include", "scuffx.h")
 
@@ -179,3 +179,38 @@ def test_mates(sx, tmp_path):
     G2.AddMeshFile(mesh, "PEC")
     G2.AddMeshFile(mesh, "PEC")
     assert G2.Mate == [-1, 0]
+
+
+@pytest.mark.parametrize("mesh", ["Sphere_327.msh", "UnitTestSphere_R0P75_414.msh", "reversed-icosphere"])
+def test_panel_normals_point_into_exterior(sx, O, mesh):
+    """Panel normals of a dielectric surface point into its exterior
+    region whatever the vertex order of the mesh's triangles
+    (RWGGeometry.cc:653-690; the crossed-overlap absorbed power depends on
+    it).  A PEC surface keeps the right-hand-rule normals: no non-PEC
+    surface bounds its exterior.  Product and oracle agree bit for bit."""
+    if mesh == "reversed-icosphere":
+        V, T = O.icosphere(2)
+        T = T[:, ::-1].copy()
+        path = None
+    else:
+        path = os.path.join(GOLDEN, mesh)
+    for mat, kind in (("CONST_EPS_4", 1), ("PEC", 0)):
+        G = sx.RWGGeometry()
+        Go = O.Geometry()
+        if path:
+            G.AddMeshFile(path, mat)
+            Go.add_msh(path, kind=kind, eps=4.0)
+        else:
+            G.AddMesh(V, T, mat)
+            Go.add_surface(V, T, kind=kind, eps=4.0)
+        P = G.Panels(0)
+        assert np.array_equal(P, Go.panels(0))
+        c = P[:, 0:3] - P[:, 0:3].mean(axis=0)
+        outward = np.einsum("ij,ij->i", c, P[:, 3:6]) > 0
+        if kind == 1:
+            assert np.all(outward)
+        else:
+            # right-hand rule from the vertex order
+            Vg, Tg = (V, T) if path is None else read_msh(path)
+            n = np.cross(Vg[Tg[:, 1]] - Vg[Tg[:, 0]], Vg[Tg[:, 2]] - Vg[Tg[:, 0]])
+            assert np.all(np.einsum("ij,ij->i", n, P[:, 3:6]) > 0)

@@ -95,3 +95,20 @@ def test_oracle_mate_copies_diagonal_block(O):
     assert np.array_equal(Mb[:n, :], Ma[:n, :]) and np.array_equal(Mb[:, :n], Ma[:, :n])
     # the computed block agrees with the copy to the FIPPI cubature tolerance
     assert np.linalg.norm(Ma[n:, n:] - Mb[n:, n:]) <= 1e-6 * np.linalg.norm(Ma[n:, n:])
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_oracle_compact_rows_equal_full_matrix_rows(O, kind):
+    """The compact row assembly (used where the full matrix does not fit on
+    the host) gives the full assembly's upper-triangle rows bit for bit."""
+    V, T = O.icosphere(1)
+    G = O.Geometry()
+    G.add_surface(V, T, kind=kind, eps=4 + 0.1j)
+    M = G.assemble(0.7)
+    nb = 1 if kind == 0 else 2
+    for e0 in (0, 17, 27):
+        R = G.rows(0.7, e0, e0 + 3)
+        r0 = nb * e0
+        for i in range(R.shape[0]):
+            r = r0 + i
+            assert np.array_equal(R[i, r:], M[r, r:])
