@@ -239,6 +239,17 @@ int scuffx_lu_factor(void *A_dev, long n, long lda, int *ipiv_dev, int prec);
 int scuffx_lu_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev, char trans,
                     long nrhs, void *B_dev, long ldb, int prec);
 
+/* ---- one-right-hand-side solve that allocates nothing.  No counterpart in
+        the reference: it is for callers that cannot allocate device memory
+        (code under HIP graph capture, tight serving loops, harnesses that
+        forbid hipMalloc).  Same result as scuffx_lu_solve(trans 'N', nrhs 1)
+        on the same factors.  prec 64 or 32; n <= 16384 (larger n:
+        SCUFFX_ERR_ARG).  Its buffers live in one fixed device array per GPU
+        (scuffx_lu_solve_noalloc_bytes() bytes, held until the process
+        ends), so calls that overlap on one GPU race. ---- */
+int scuffx_lu_solve_noalloc(const void *LU_dev, long n, long lda, const int *ipiv_dev, void *B_dev, int prec);
+long scuffx_lu_solve_noalloc_bytes(void);
+
 /* ---- HMatrix::LUInvert (LBWrappers.cc:251-335): A <- A^{-1} from the
         factors of scuffx_lu_factor (zgetri / dgetri); prec 64, 32 or 8.
    Returns the first zero diagonal entry of U (1-based; A unchanged) or 0. */

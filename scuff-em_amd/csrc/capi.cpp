@@ -562,6 +562,23 @@ int scuffx_lu_solve(const void *LU, long n, long lda, const int *ipiv, char tran
   return 0;
 }
 
+int scuffx_lu_solve_noalloc(const void *LU, long n, long lda, const int *ipiv, void *B, int prec)
+{
+  if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");
+  if (!LU || !ipiv || !B) return fail(SCUFFX_ERR_ARG, "null argument");
+  int rc;
+  if (prec == 64) rc = lu_solve_noalloc<double2>((const double2 *)LU, n, lda, ipiv, (double2 *)B, g_stream);
+  else if (prec == 32) rc = lu_solve_noalloc<float2>((const float2 *)LU, n, lda, ipiv, (float2 *)B, g_stream);
+  else return fail(SCUFFX_ERR_ARG, "prec must be 64 or 32");
+  if (rc == -7) return fail(SCUFFX_ERR_ARG, "n exceeds the no-allocation solve's maximum of 16384");
+  if (rc == -1) return fail(SCUFFX_ERR_ARG, "bad LU solve arguments");
+  if (rc) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
+  if (hipStreamSynchronize(g_stream) != hipSuccess) return fail(SCUFFX_ERR_HIP, "LU solve failed on the device");
+  return 0;
+}
+
+long scuffx_lu_solve_noalloc_bytes(void) { return noalloc_scratch_bytes(); }
+
 int scuffx_lu_logdet(const void *LU, long n, long lda, const int *ipiv, int prec, double *logabs, double *phase)
 {
   if (hip_down()) return fail(SCUFFX_ERR_ARG, "library shut down");

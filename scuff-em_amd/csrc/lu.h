@@ -46,6 +46,14 @@ template <typename T>
 int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long nrhs, T *B,
              long ldb, hipStream_t stream, SolveWs<T> *ws = nullptr);
 
+// lu_solve(trans 'N', one right-hand side) with its buffers in a fixed
+// device array (no allocation; n <= NOALLOC_MAX_N, else -7).  Calls that
+// overlap on one GPU race: they share that GPU's array.
+constexpr long NOALLOC_MAX_N = 16384;
+template <typename T>
+int lu_solve_noalloc(const T *LU, long n, long lda, const int *d_ipiv, T *B, hipStream_t stream);
+long noalloc_scratch_bytes();
+
 // log |det| and arg det of an LU-factored matrix (out[0], out[1]; host)
 template <typename T>
 int lu_logdet(const T *LU, long n, long lda, const int *ipiv, double out[2], hipStream_t stream);

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -2159,6 +2160,56 @@ template size_t solve_ws_bytes<double2>(long);
 template size_t solve_ws_bytes<float2>(long);
 template SolveWs<double2> solve_ws_carve<double2>(char *, long);
 template SolveWs<float2> solve_ws_carve<float2>(char *, long);
+
+// ---------------------------------------------------------------------
+// one-right-hand-side solve that allocates nothing: the buffers of a
+// SolveWs (W, inverses of the 64x64 diagonal blocks of L and U, composed
+// permutation, one-launch words) are carved from a fixed device array,
+// one copy per GPU, held until the process ends.  Every call on a GPU
+// shares that GPU's array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------
+constexpr long r256_const(long b) { return 256 * ((b + 255) / 256); }
+constexpr long NOALLOC_NBLK = (NOALLOC_MAX_N + TB - 1) / TB;
+constexpr long NOALLOC_BYTES = r256_const((long)sizeof(double2) * NOALLOC_MAX_N) +
+                               2 * r256_const((long)sizeof(double2) * NOALLOC_NBLK * TB * TB) +
+                               r256_const((long)sizeof(int) * NOALLOC_MAX_N) +
+                               r256_const((long)sizeof(unsigned long long) * (2 * NOALLOC_NBLK + 8));
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+long noalloc_scratch_bytes() { return NOALLOC_BYTES; }
+
+// device address of this GPU's copy of `scratch`, looked up once per device
+// with that device current (the array's name in host code is only a
+// host-side stand-in, never a device address)
+static void *noalloc_scratch_ptr()
+{
+  static std::mutex mu;
+  static void *ptr[64] = {nullptr};
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[d]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    ptr[d] = p;
+  }
+  return ptr[d];
+}
+
+template <typename T>
+int lu_solve_noalloc(const T *LU, long n, long lda, const int *d_ipiv, T *B, hipStream_t stream)
+{
+  if (n < 0 || lda < std::max(1L, n)) return -1;
+  if (n > NOALLOC_MAX_N || (long)solve_ws_bytes<T>(n) > NOALLOC_BYTES) return -7;
+  if (n == 0) return 0;
+  char *base = (char *)noalloc_scratch_ptr();
+  if (!base) return -3;
+  SolveWs<T> w = solve_ws_carve<T>(base, n);
+  w.inv_ready = false;   // the array holds whatever the last call left
+  return lu_solve<T>(LU, n, lda, d_ipiv, 'N', 1, B, std::max(1L, n), stream, &w);
+}
+template int lu_solve_noalloc<double2>(const double2 *, long, long, const int *, double2 *, hipStream_t);
+template int lu_solve_noalloc<float2>(const float2 *, long, long, const int *, float2 *, hipStream_t);
 
 // ---------------------------------------------------------------------
 // refinement helpers

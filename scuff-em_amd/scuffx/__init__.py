@@ -101,6 +101,8 @@ def lib():
                                             vp, C.c_long, C.c_long, C.c_int, vp, vp], C.c_int),
             "scuffx_unpack_symmetric": ([vp, C.c_long, vp, C.c_long, C.c_int], C.c_int),
             "scuffx_lu_invert": ([vp, C.c_long, C.c_long, vp, C.c_int], C.c_int),
+            "scuffx_lu_solve_noalloc": ([vp, C.c_long, C.c_long, vp, vp, C.c_int], C.c_int),
+            "scuffx_lu_solve_noalloc_bytes": ([], C.c_long),
             "scuffx_rhs_planewave": ([vp, C.c_double, C.c_double, vp, vp, vp], C.c_int),
             "scuffx_power": ([vp, C.c_double, C.c_double, vp, vp, dp, dp], C.c_int),
             "scuffx_power_emt": ([vp, C.c_double, C.c_double, vp, vp, vp, C.c_long, dp, dp], C.c_int),
