@@ -80,6 +80,10 @@ int scuffx_geom_bf_range(const scuffx_geom *G, int ns, int *offset, int *nbf);
    identical to ns (same mesh file, same exterior medium, both PEC or the same
    interior material), or -1; surfaces added from raw arrays have no mates */
 int scuffx_geom_mate(const scuffx_geom *G, int ns);
+/* eps and mu of region r (0: exterior) at omega (units of c / 1 um), as
+   RWGGeometry::RegionMPs[r]->GetEpsMu (libMatProp.cc:501); eps2, mu2:
+   re, im */
+int scuffx_geom_epsmu(const scuffx_geom *G, int region, double omega_re, double omega_im, double *eps2, double *mu2);
 /* per edge: iV1 iV2 iQP iQM iPPanel iMPanel PIndex MIndex Index (libscuff.h:120-146) */
 int scuffx_geom_edges(const scuffx_geom *G, int ns, int *out9, double *lengths);
 /* per panel: centroid[3] zhat[3] radius area (libscuff.h:100-112) */
@@ -118,6 +122,13 @@ int scuffx_geom_transform_surface(scuffx_geom *G, int ns, const double *R9, cons
 int scuffx_geom_untransform(scuffx_geom *G);
 /* SurfaceMoved[ns] of the last scuffx_geom_transform (RWGGeometry.cc:774-781) */
 int scuffx_geom_surface_moved(const scuffx_geom *G, int ns);
+/* new coordinates for the nV vertices of surface ns, same mesh topology
+   (the reference re-reads a mesh for that: RWGSurface.cc:119-217, then
+   InitRWGPanel / InitEdgeList on the new vertices).  Panels and edges are
+   re-derived, normals keep their orientation, the surface stops being a
+   mate of another.  Identical coordinates are a no-op; an error if the
+   geometry is transformed or a panel would degenerate. */
+int scuffx_geom_set_vertices(scuffx_geom *G, int ns, const double *V, int nV);
 
 /* ---- adjoint contraction ----
    For a rigid motion of surface ns (displacement along x, y, z, then
@@ -133,6 +144,54 @@ int scuffx_geom_surface_moved(const scuffx_geom *G, int ns);
 int scuffx_adjoint(scuffx_geom *G, int ns, double wr, double wi, const void *x, const void *y, int ntorque,
                    const double *gamma, double *out);
 
+/* ---- vertex-position derivatives (no counterpart in the reference, whose
+        derivatives are the rigid motions of GradM / dMdT; the panel-pair
+        gradient structure follows PanelPanelInteractions.cc:132-199) ----
+   Gradient arrays have one slot per vertex coordinate, surfaces in order:
+   slot 3 (sum of NumVertices of earlier surfaces + v) + c;
+   scuffx_vertex_slots gives their number. */
+int scuffx_vertex_slots(scuffx_geom *G);
+/* grad[theta] = sum_ij C_ij dM_ij / dtheta (complex128), with C = u v^T
+   (C NULL; u, v device vectors of TotalBFs) or a dense column-major C
+   (device, leading dimension ldc).  M: the matrix assembled at this
+   frequency (device, column-major, ldm), used for the edge-length terms.
+   Far panel pairs are differentiated analytically with the assembly's own
+   rule, separated near pairs with the 78-point rule, touching and self
+   pairs by central differences (step 1e-4 of the panel radius) of their
+   Taylor-Duffy integrals; nothing of dM is stored.  Touching surfaces
+   (panels of two surfaces sharing a vertex) are SCUFFX_ERR_UNSUPPORTED. */
+int scuffx_vertex_gradient(scuffx_geom *G, double wr, double wi, const void *M, long ldm, const void *u,
+                           const void *v, const void *C, long ldc, void *grad);
+/* grad[theta] = sum_i mu_i d b_i / dtheta for the plane-wave RHS of
+   scuffx_rhs_planewave (same E0, nhat); mu, grad: device complex128 */
+int scuffx_rhs_vertex_gradient(scuffx_geom *G, double wr, double wi, const double *E0, const double *nhat,
+                               const void *mu, void *grad);
+/* absorbed power of scuffx_power (crossed overlaps) for the currents KN:
+   grad[theta] = its derivative at fixed KN (real, device doubles) and
+   gamma (device complex128, TotalBFs) with dPAbs = Re(gamma^T dKN) */
+int scuffx_pabs_vertex_gradient(scuffx_geom *G, const void *KN, double *grad, void *gamma);
+
+/* ---- batched frequency sweep over several geometries of uneven order
+        (no counterpart in the reference: the scattering caller's loop,
+        scuff-scatter.cc:373-567, over many instances at once) ----
+   For each of the ngeom geometries, the nfreq frequencies (complex, 2
+   doubles each) are assembled, factored in complex64 by one batched LU
+   (tournament pivoting, block width autotuned per order and batch size),
+   solved for the plane wave (E0: 3 complex, nhat) and refined in complex128
+   with refine_steps fixed steps, and their power computed -- all recorded
+   into one hipGraph and launched once.  Instances left above a 1e-11
+   relative residual, or with a factorization problem, are solved again by
+   the step-by-step mixed path.  out[3 (g nfreq + f) + {0,1,2}] = PAbs, PExt,
+   PScat; relres (optional) the final residuals; info (optional, 2 ints):
+   instances solved again, LU block width of the first geometry; ms
+   (optional, 5 doubles): host milliseconds of set-up, capture,
+   instantiation, graph run, results.  The device buffers are kept for the
+   next sweep of the same shapes (until scuffx_shutdown).  Returns
+   SCUFFX_ERR_UNSUPPORTED if a frequency needs the short-wavelength kernels
+   (their job lists are built on the host). */
+int scuffx_sweep_batched(scuffx_geom **Gs, int ngeom, const double *omegas, int nfreq, const double *E0,
+                         const double *nhat, int refine_steps, double *out, double *relres, int *info,
+                         double *ms);
 /* ---- frequency sweep ----
    The scattering caller's loop over frequencies on one geometry
    (scuff-scatter.cc:373-567, no transformations): per frequency the BEM

@@ -16,6 +16,7 @@
 #include "ws.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -35,7 +36,7 @@ static double dist3h(const double *a, const double *b)
 // relative centroid distance and common-vertex count with the common
 // vertices moved to the front (AssessPanelPair.cc:52-172).  ia/ib
 // receive the mesh-local vertex slots in call order.
-static int assess_pair(const Surface &Sa, int pa, const Surface &Sb, int pb, double *rRel, int ia[3], int ib[3])
+int assess_pair(const Surface &Sa, int pa, const Surface &Sb, int pb, double *rRel, int ia[3], int ib[3])
 {
   const Panel &Pa = Sa.Panels[pa], &Pb = Sb.Panels[pb];
   for (int i = 0; i < 3; i++) { ia[i] = i; ib[i] = i; }
@@ -130,6 +131,8 @@ static int canonical(const double *Va[3], const double *Vb[3], int ncv, const do
 
 Assembler::Assembler(const Geometry &G) : G_(G)
 {
+  static std::atomic<unsigned long long> counter{0};
+  serial_ = ++counter;
   ds_.resize(G.Surfaces.size());
   for (size_t s = 0; s < G.Surfaces.size(); s++) {
     const Surface &S = G.Surfaces[s];
@@ -176,15 +179,18 @@ Assembler::~Assembler()
     hipFree(B.bands.d_cols); hipFree(B.bands.d_colSlots); hipFree(B.bands.d_slots);
   }
   for (auto &e : evpool_) hipEventDestroy(e);
+  for (void *p : vg_alloc_) hipFree(p);
   hipFree(d_jobs_); hipFree(d_records_);
   hipFree(d_HP_); hipFree(d_HN_); hipFree(d_HS_); hipFree(d_scratch_); hipFree(d_stage_);
   hipFree(d_regs_);
   hipFree(d_swjobs_);
+  hipFree(d_regtab_);
 }
 
 int Assembler::ensure(double **p, size_t *cap, size_t need)
 {
   if (need <= *cap && *p) return 0;
+  alloc_gen_++;
   if (*p) hipFree(*p);
   *p = nullptr;
   *cap = 0;
@@ -775,6 +781,95 @@ int Assembler::assemble(cdouble omega, void *Mp, long ldm, int kind, hipStream_t
   stats_.self_ms = t_self; stats_.far_ms = t_far; stats_.near_ms = t_near; stats_.stamp_ms = t_stamp;
   for (auto &e : ev) hipEventDestroy(e);
   if (prof_enabled()) prof_collect();
+  return 0;
+}
+
+int Assembler::prepare_graph(const std::vector<cdouble> &omegas, hipStream_t stream)
+{
+  if (!err_.empty()) return -3;
+  int rc = build_fippi(stream);
+  if (rc) return rc;
+  const int NS = (int)G_.Surfaces.size(), nf = (int)omegas.size();
+  const int nblk = NS * (NS + 1) / 2;
+  std::vector<RegionK> tab((size_t)std::max(nf, 1) * nblk * 2);
+  nreg_tab_.assign((size_t)std::max(nf, 1) * nblk, 0);
+  size_t need_hn = 1, need_hs = 1, need_hb = 1;
+  for (int f = 0; f < nf; f++) {
+    std::vector<cdouble> eps(G_.Regions.size()), mu(G_.Regions.size());
+    for (size_t r = 0; r < G_.Regions.size(); r++) G_.Regions[r].eps_mu(omegas[f], &eps[r], &mu[r]);
+    int bi = 0;
+    for (int sa = 0; sa < NS; sa++)
+      for (int sb = sa; sb < NS; sb++, bi++) {
+        RegionK rk[2];
+        const int nreg = regions_for(sa, sb, eps, mu, omegas[f], rk);
+        if (nreg < 0) return nreg;
+        nreg_tab_[(size_t)f * nblk + bi] = nreg;
+        for (int r = 0; r < nreg; r++) tab[((size_t)f * nblk + bi) * 2 + r] = rk[r];
+        // the short-wavelength kernels build their job lists on the host
+        const NearBlock &B = blocks_[bi];
+        for (int r = 0; r < nreg; r++) {
+          const double ak = std::hypot(rk[r].k.r, rk[r].k.i);
+          const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
+          for (int pa = 0; pa + 1 < (int)B.rowPtr.size(); pa++)
+            for (int e = B.rowPtr[pa]; e < B.rowPtr[pa + 1]; e++)
+              if (B.idx[e] >= 0 && ak * std::max(Sa.Panels[pa].Radius, Sb.Panels[B.col[e]].Radius) > M_PI) {
+                err_ = "a sweep frequency needs the short-wavelength kernels (|k| R_panel > pi): not capturable";
+                return -5;
+              }
+        }
+        const int needH1 = !(G_.Surfaces[sa].IsPEC && G_.Surfaces[sb].IsPEC);
+        const size_t stride = needH1 ? 4 : 2;
+        need_hn = std::max(need_hn, B.pairs.size() * std::max(nreg, 1) * 9 * stride);
+        if (sa == sb) need_hs = std::max(need_hs, (size_t)ds_[sa].nP * std::max(nreg, 1) * 18);
+        need_hb = std::max(need_hb, (size_t)std::max(B.bands.maxSlots, 1) * B.bands.rowsCap * std::max(nreg, 1) * 9 * stride);
+      }
+  }
+  if (ensure(&d_HN_, &cap_HN_, need_hn) || ensure(&d_HS_, &cap_HS_, need_hs) || ensure(&d_HP_, &cap_HP_, need_hb) ||
+      ensure(&d_scratch_, &cap_scratch_, (size_t)16384 * 1025 * 4))
+    return -2;
+  const size_t bytes = sizeof(RegionK) * tab.size();
+  if (bytes > cap_regtab_) {
+    alloc_gen_++;
+    if (d_regtab_) hipFree(d_regtab_);
+    d_regtab_ = nullptr;
+    cap_regtab_ = 0;
+    if (hipMalloc(&d_regtab_, bytes) != hipSuccess) { err_ = "device allocation failed"; return -2; }
+    cap_regtab_ = bytes;
+  }
+  HIPCHK(hipMemcpy(d_regtab_, tab.data(), bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int Assembler::assemble_graph(int f, void *Mp, long ldm, hipStream_t stream)
+{
+  if (!err_.empty()) return -3;
+  if (!d_regtab_ || ldm < G_.TotalBFs) { err_ = "sweep not prepared"; return -1; }
+  const MatOut M{Mp, ldm, MAT_C128};
+  const int NS = (int)G_.Surfaces.size(), nblk = NS * (NS + 1) / 2;
+  int bi = 0;
+  for (int sa = 0; sa < NS; sa++)
+    for (int sb = sa; sb < NS; sb++, bi++) {
+      const Surface &Sa = G_.Surfaces[sa], &Sb = G_.Surfaces[sb];
+      const DevSurface &Da = ds_[sa], &Db = ds_[sb];
+      NearBlock &B = blocks_[bi];
+      const int same = (sa == sb);
+      if (same && mates_[sa] >= 0) {
+        const long om = G_.BFIndexOffset[mates_[sa]], os = G_.BFIndexOffset[sa], n = Sa.NumBFs;
+        HIPCHK(launch_copy_block(stream, M, om, os, n));
+        continue;
+      }
+      const int nreg = nreg_tab_[(size_t)f * nblk + bi];
+      const RegionK *dreg = d_regtab_ + ((size_t)f * nblk + bi) * 2;
+      const int needH1 = !(Sa.IsPEC && Sb.IsPEC);
+      if (same && nreg > 0)
+        HIPCHK(launch_self(stream, Da.V, Da.PVI, Da.nP, nreg, dreg, d_HS_, d_scratch_, 16384));
+      if (nreg > 0 && !B.pairs.empty())
+        HIPCHK(launch_near(stream, B.d_pairs, (int)B.pairs.size(), d_records_, nreg, dreg, needH1, d_HN_));
+      int rc = far_and_stamp(B, Da, Db, Sa, Sb, nreg, dreg, needH1, M, G_.BFIndexOffset[sa], G_.BFIndexOffset[sb],
+                             stream, nullptr, nullptr);
+      if (rc) return rc;
+    }
+  HIPCHK(launch_mirror(stream, M, G_.TotalBFs));
   return 0;
 }
 

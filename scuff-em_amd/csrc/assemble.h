@@ -9,6 +9,7 @@
 #include "bem.h"
 #include "fippi_cache.h"
 #include "geometry.h"
+#include "vgrad.h"
 
 namespace scuffx {
 
@@ -64,6 +65,14 @@ class Assembler {
   // MAT_F64_PACKED (real parts, packed upper triangle; ldm unused)
   int assemble(cdouble omega, void *M, long ldm, int kind, hipStream_t stream);
   int build_fippi(hipStream_t stream);
+  // Capturable assembly of a frequency sweep.  prepare_graph (outside any
+  // stream capture) builds the FIPPI table, checks that no frequency needs
+  // the short-wavelength kernels, sizes every buffer for the largest block
+  // and uploads each frequency's region constants; assemble_graph(f, ...)
+  // then assembles frequency f with no host synchronization, allocation or
+  // host-to-device copy (complex128, full matrix).
+  int prepare_graph(const std::vector<cdouble> &omegas, hipStream_t stream);
+  int assemble_graph(int f, void *M, long ldm, hipStream_t stream);
   // block of surfaces sa (rows) and sb (columns); for sa != sb optionally
   // its derivatives with respect to a rigid motion of surface sa
   int assemble_block(int sa, int sb, cdouble omega, double2 *B, long ldb, double2 *GB, long ldg, long gstride,
@@ -72,7 +81,24 @@ class Assembler {
   // (its terms of every common-region sum); -1: every region
   void set_region_filter(int r) { region_only_ = r; }
   const AsmStats &stats() const { return stats_; }
+  // unique per Assembler object of the process (device pointers recorded
+  // in a graph belong to one assembler)
+  unsigned long long serial() const { return serial_; }
+  // changes whenever a device buffer of the assembler is reallocated
+  unsigned long long alloc_generation() const { return alloc_gen_; }
+  const std::vector<int> &nreg_table() const { return nreg_tab_; }
   const DevSurface &dev_surface(int s) const { return ds_[s]; }
+  // sum_ij C_ij dM_ij / dtheta for every vertex coordinate theta of every
+  // surface (grad: complex, slot gofs(s) + 3 v + c, vg_slots() of them);
+  // M: the matrix assembled at omega, for the edge-length term; *msg says
+  // why on failure (vgrad.cpp)
+  int vertex_gradient(cdouble omega, const double2 *M, long ldm, const VGW &w, double2 *grad, hipStream_t stream,
+                      std::string *msg);
+  // per-surface views for the vertex-gradient kernels, built on first use
+  int vg_tables();
+  const VGSurf &vg_surface(int s) const { return vgs_[s]; }
+  const int *vg_edges(int s) const { return vg_ev_[s]; }   // iV1, iV2, iQP, iQM per edge
+  long vg_slots() const { return vg_slots_; }
   std::string error() const { return err_; }
   int num_fippi() const { return (int)jobs_.size(); }
   // host copy of the FIPPI records (36 doubles each) for checks
@@ -97,6 +123,14 @@ class Assembler {
   std::string err_;
   std::vector<hipEvent_t> evpool_;   // timing events of the band loop
   int region_only_ = -1;
+  unsigned long long serial_ = 0, alloc_gen_ = 0;
+  RegionK *d_regtab_ = nullptr;      // [frequency][block][2] of a prepared sweep
+  size_t cap_regtab_ = 0;
+  std::vector<int> nreg_tab_;        // [frequency][block]
+  std::vector<VGSurf> vgs_;          // vertex-gradient views (vg_tables)
+  std::vector<void *> vg_alloc_;
+  std::vector<int *> vg_ev_;
+  long vg_slots_ = 0;
   void build_near();
   void build_bands(NearBlock &B);
   // far records and matrix entries of block B, band by band (the near,
@@ -113,5 +147,9 @@ class Assembler {
   int run_sw(int sa, int sb, int bi, const RegionK *rk, int nreg, const RegionK *dreg, int needH1,
              hipStream_t stream);
 };
+
+// relative centroid distance and common-vertex count of two panels, the
+// common vertices first in ia / ib (AssessPanelPair.cc:52-172)
+int assess_pair(const Surface &Sa, int pa, const Surface &Sb, int pb, double *rRel, int ia[3], int ib[3]);
 
 }  // namespace scuffx

@@ -2,6 +2,7 @@
 #include "geometry.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,6 +27,14 @@ void Material::eps_mu(cdouble omega, cdouble *e, cdouble *m) const
     case MAT_VACUUM: *e = 1.0; *m = 1.0; break;
     case MAT_CONST: *e = eps; *m = mu; break;
     case MAT_SIGMA: *e = eps.real() + cdouble(0.0, 1.0) * sigma / omega; *m = mu; break;
+    case MAT_EXPR: {
+      // user-defined expressions take w in rad/s (omega in units of c / 1 um,
+      // MatProp::FreqUnit, libMatProp.cc:44)
+      const cdouble w = omega * 2.99792458e14;
+      *e = epsx ? eval_cexpr(*epsx, w) : cdouble(1.0);
+      *m = mux ? eval_cexpr(*mux, w) : cdouble(1.0);
+      break;
+    }
     default: *e = 1.0 - wp * wp / (omega * (omega + cdouble(0.0, gamma))); *m = 1.0; break;
   }
 }
@@ -592,6 +601,38 @@ void transform_geometry(Geometry *G, const GTComplex &C)
   }
 }
 
+std::string set_surface_vertices(Geometry *G, int ns, const double *V, int nV)
+{
+  if (ns < 0 || ns >= (int)G->Surfaces.size()) return "bad surface index";
+  Surface &S = G->Surfaces[ns];
+  if (nV != S.NumVertices()) return "vertex count differs from the mesh";
+  for (long i = 0; i < 3L * nV; i++)
+    if (!std::isfinite(V[i])) return "non-finite vertex coordinate";
+  std::vector<double> old = S.V;
+  S.V.assign(V, V + 3 * (size_t)nV);
+  reinit_panels(&S);
+  for (const Panel &P : S.Panels)
+    if (!(P.Area > 0.0)) {
+      S.V = old;
+      reinit_panels(&S);
+      return "a panel became degenerate";
+    }
+  S.MeshFile.clear();   // no longer the file's mesh: never a mate
+  // edge data as build_surface derives it from the vertices
+  for (Edge &E : S.Edges) {
+    const double *VL = S.V.data() + 3 * E.iV1, *VG = S.V.data() + 3 * E.iV2;
+    for (int i = 0; i < 3; i++) E.Centroid[i] = (VL[i] + VG[i]) / 2.0;
+    E.Length = dist3(VL, VG);
+    E.Radius = dist3(E.Centroid, S.V.data() + 3 * E.iQP);
+    E.Radius = std::fmax(E.Radius, dist3(E.Centroid, VL));
+    E.Radius = std::fmax(E.Radius, dist3(E.Centroid, VG));
+    if (E.iQM >= 0) E.Radius = std::fmax(E.Radius, dist3(E.Centroid, S.V.data() + 3 * E.iQM));
+  }
+  if ((int)G->SurfaceMoved.size() != (int)G->Surfaces.size()) G->SurfaceMoved.assign(G->Surfaces.size(), 0);
+  G->SurfaceMoved[ns] = 1;
+  return "";
+}
+
 void untransform_geometry(Geometry *G)
 {
   for (Surface &S : G->Surfaces) {
@@ -613,8 +654,49 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
   bool inobj = false, moved = false;
   GTransform T;
   int lineno = 0;
+  // user-defined materials: MATERIAL name / constants / Eps(w) = ... /
+  // Mu(w) = ... / ENDMATERIAL (UserDefinedMaterials.cc:145-320)
+  std::map<std::string, Material> user;
+  std::string inmat;
+  std::map<std::string, double> consts;
+  Material cur;
   while (std::getline(in, line)) {
     lineno++;
+    std::string where = fn + ":" + std::to_string(lineno) + ": ";
+    if (!inmat.empty()) {
+      size_t hash = line.find('#');
+      if (hash != std::string::npos) line.resize(hash);
+      size_t b = line.find_first_not_of(" \t\r\n");
+      if (b == std::string::npos) continue;
+      std::string body = line.substr(b);
+      if (!strncasecmp(body.c_str(), "ENDMATERIAL", 11)) {
+        user[inmat] = cur;
+        inmat.clear();
+        continue;
+      }
+      size_t eq = body.find('=');
+      if (eq == std::string::npos) return where + "unrecognized input in MATERIAL " + inmat;
+      std::string lhs = body.substr(0, eq), rhs = body.substr(eq + 1);
+      while (!lhs.empty() && std::isspace((unsigned char)lhs.back())) lhs.pop_back();
+      size_t semi = rhs.rfind(';');
+      if (semi != std::string::npos) rhs.resize(semi);
+      while (!rhs.empty() && std::isspace((unsigned char)rhs.back())) rhs.pop_back();
+      if (!strncasecmp(lhs.c_str(), "Eps(w)", 6) || !strncasecmp(lhs.c_str(), "Mu(w)", 5)) {
+        CExprPtr e;
+        std::string err = parse_cexpr(rhs, consts, &e);
+        if (!err.empty()) return where + "invalid expression: " + err;
+        if (!strncasecmp(lhs.c_str(), "Eps(w)", 6)) cur.epsx = e; else cur.mux = e;
+      } else {
+        char *end = nullptr;
+        const char *r0 = rhs.c_str();
+        double v = std::strtod(r0, &end);
+        if (end == r0) return where + "invalid constant value";
+        while (*end && std::isspace((unsigned char)*end)) end++;
+        if (*end) return where + "junk at end of line";
+        consts[lhs] = v;
+      }
+      continue;
+    }
     size_t hash = line.find('#');
     if (hash != std::string::npos) line.resize(hash);
     std::stringstream ss(line);
@@ -623,7 +705,14 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
     while (ss >> w) tok.push_back(w);
     if (tok.empty()) continue;
     const std::string &kw = tok[0];
-    std::string where = fn + ":" + std::to_string(lineno) + ": ";
+    if (!inobj && !strcasecmp(kw.c_str(), "MATERIAL")) {
+      if (tok.size() != 2) return where + "no name given for MATERIAL";
+      inmat = tok[1];
+      consts.clear();
+      cur = Material();
+      cur.kind = MAT_EXPR;
+      continue;
+    }
     if (!strcasecmp(kw.c_str(), "OBJECT")) {
       if (inobj) return where + "OBJECT inside OBJECT";
       inobj = true; mesh.clear(); mat = "PEC"; moved = false; T = GTransform();
@@ -651,9 +740,13 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
       std::string err = read_gmsh(mesh[0] == '/' ? mesh : dir + mesh, &V, &Tri);
       if (!err.empty()) return err;
       if (moved) T.apply(V.data(), V.size() / 3);
-      Material m; bool pec;
-      err = parse_material(mat, &m, &pec);
-      if (!err.empty()) return where + err;
+      Material m; bool pec = false;
+      auto um = user.find(mat);
+      if (um != user.end()) m = um->second;
+      else {
+        err = parse_material(mat, &m, &pec);
+        if (!err.empty()) return where + err;
+      }
       Surface S;
       err = build_surface(V.data(), (int)V.size() / 3, Tri.data(), (int)Tri.size() / 3, pec, &S);
       if (!err.empty()) return where + err;
@@ -666,6 +759,7 @@ std::string read_scuffgeo(const std::string &fn, Geometry *G)
       return where + "unsupported keyword " + kw + " in OBJECT";
     }
   }
+  if (!inmat.empty()) return fn + ": MATERIAL " + inmat + " without ENDMATERIAL";
   if (inobj) return fn + ": OBJECT without ENDOBJECT";
   if (G->Surfaces.empty()) return fn + ": no objects";
   return "";

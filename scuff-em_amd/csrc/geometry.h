@@ -8,8 +8,11 @@
 #pragma once
 
 #include <complex>
+#include <map>
 #include <string>
 #include <vector>
+
+#include "cexpr.h"
 
 namespace scuffx {
 
@@ -33,12 +36,13 @@ struct Edge {
   int Index;
 };
 
-enum MaterialKind { MAT_VACUUM = 0, MAT_CONST = 1, MAT_DRUDE = 2, MAT_SIGMA = 3 };
+enum MaterialKind { MAT_VACUUM = 0, MAT_CONST = 1, MAT_DRUDE = 2, MAT_SIGMA = 3, MAT_EXPR = 4 };
 struct Material {
   int kind = MAT_VACUUM;
   cdouble eps = 1.0, mu = 1.0;
   double wp = 0.0, gamma = 0.0;   // Drude: eps = 1 - wp^2/(w(w+i gamma))
   double sigma = 0.0;             // SIGMA model: eps = Re(eps) + i sigma/w
+  CExprPtr epsx, mux;             // user-defined Eps(w), Mu(w) of w in rad/s (null: 1)
   void eps_mu(cdouble omega, cdouble *e, cdouble *m) const;
 };
 
@@ -108,6 +112,10 @@ void transform_geometry(Geometry *G, const GTComplex &C);
 // one surface, by index (the Transform of a single labelled entry)
 void transform_surface(Geometry *G, int ns, const GTransform &D);
 void untransform_geometry(Geometry *G);
+// new coordinates for the vertices of surface ns (same mesh topology):
+// panels and edges are re-derived from them, the normals keep their
+// orientation; an empty string on success
+std::string set_surface_vertices(Geometry *G, int ns, const double *V, int nV);
 
 // Builds a surface from vertex coordinates and triangle vertex indices.
 // Returns an empty string on success, an error message otherwise.

@@ -7,8 +7,8 @@
 // remaining 1-D integral is done by p-adaptive Clenshaw-Curtis
 // quadrature, relative tolerance 1e-10, at most 1000 new points
 // (TaylorDuffy.cc:152-367, libs/libSGJC/pcubature.c:323-385).
-// One thread per (panel, Qa, Qb, region) job; the function values of a
-// job live in a scratch slab so every refinement level reuses them.
+// One wave per (panel, Qa, Qb, region) job; the function values of a job
+// live in a scratch slab so every refinement level reuses them.
 #include <hip/hip_runtime.h>
 
 #include "bem.h"
@@ -151,44 +151,35 @@ __device__ void ct_integrand(const CTJob &J, double y, double *f)
   }
 }
 
-// Clenshaw-Curtis estimate of the given rule order over the cached
-// values in entry order (center, +-x0 | +-x1 | +-x2,+-x3 | ...);
-// order < 0 requests the one-point rule.  Summation order matches the
-// reference's cache walk.
-__device__ void cc_rule(const double *vals, int mcur, int order, double V, double *out)
+// weight of cache entry e in the Clenshaw-Curtis rule of the given order
+// (entries: center, the pair at x0, then level l >= 1 pairs from 2^l + 1),
+// 0 for entries of levels above the order; order < 0: one-point rule
+__device__ inline double cc_weight(int e, int order)
 {
-  for (int c = 0; c < 4; c++) out[c] = 0.0;
-  if (order < 0) {
-    for (int c = 0; c < 4; c++) out[c] += vals[c] * (V * 2.0);
-    return;
-  }
+  if (order < 0) return e == 0 ? 2.0 : 0.0;
   const double *w = c_ccw + order + (1 << order) - 1;
-  // entry 0: center, then the pair at x0
-  for (int c = 0; c < 4; c++) out[c] += vals[c] * (V * w[0]);
-  for (int c = 0; c < 4; c++) out[c] += vals[4 + c] * (V * w[1]);
-  for (int c = 0; c < 4; c++) out[c] += vals[8 + c] * (V * w[1]);
-  int base = 3;
-  for (int l = 1; l <= mcur && l <= order; l++) {
-    int nl = 1 << (l - 1);
-    for (int i = 0; i < nl; i++) {
-      double wt = V * w[1 + nl + i];
-      const double *vp = vals + 4 * (base + 2 * i), *vm = vp + 4;
-      for (int c = 0; c < 4; c++) out[c] += vp[c] * wt;
-      for (int c = 0; c < 4; c++) out[c] += vm[c] * wt;
-    }
-    base += 2 * nl;
-  }
+  if (e == 0) return w[0];
+  if (e < 3) return w[1];
+  const int l = 31 - __clz(e - 1);          // level of the entry
+  if (l > order) return 0.0;
+  const int nl = 1 << (l - 1);
+  return w[1 + nl + ((e - (1 << l) - 1) >> 1)];
 }
 
+// One wave per (panel, Qa, Qb, region) job: each refinement level's new
+// points are evaluated across the 64 lanes, and the two rule estimates
+// are lane-strided partial sums reduced over the wave (the same stopping
+// rule as the reference's p-adaptive cubature: relative 1e-10 per
+// component, at most 1000 new points).
 __global__ void __launch_bounds__(64)
 k_self(const double *V, const int *PVI, int nP, int nreg, const RegionK *regs,
        double *HS, double *scratch, int job0, int njobs)
 {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x, lane = threadIdx.x;
   if (t >= njobs) return;
-  int job = job0 + t;
-  int r = job % nreg, q = (job / nreg) % 9, p = job / (9 * nreg);
-  int qa = q / 3, qb = q % 3;
+  const int job = job0 + t;
+  const int r = job % nreg, q = (job / nreg) % 9, p = job / (9 * nreg);
+  const int qa = q / 3, qb = q % 3;
   const double *V1 = V + 3 * PVI[3 * p], *V2 = V + 3 * PVI[3 * p + 1], *V3 = V + 3 * PVI[3 * p + 2];
   const double *Q = V + 3 * PVI[3 * p + qa], *QP = V + 3 * PVI[3 * p + qb];
   double A[3], B[3], D[3], DP[3];
@@ -201,35 +192,46 @@ k_self(const double *V, const int *PVI, int nP, int nreg, const RegionK *regs,
              dot3(D, A), dot3(D, B), dot3(D, DP), J.P[1]);
   double *vals = scratch + (long long)t * TD_NODES * 4;
   const double c = 0.5, rr = 0.5, Vol = 0.5;
-  ct_integrand(J, c, vals);
-  ct_integrand(J, c + rr * c_ccx[0], vals + 4);
-  ct_integrand(J, c - rr * c_ccx[0], vals + 8);
+  if (lane < 3) {
+    const double y = lane == 0 ? c : lane == 1 ? c + rr * c_ccx[0] : c - rr * c_ccx[0];
+    ct_integrand(J, y, vals + 4 * lane);
+  }
+  __syncthreads();
   int m = 0, nstored = 3;
   long numEval = 0;
   double val[4], val1[4];
   for (;;) {
-    cc_rule(vals, m, m, Vol, val);
-    cc_rule(vals, m, m - 1, Vol, val1);
+    double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    for (int e = lane; e < nstored; e += 64) {
+      const double wa = cc_weight(e, m) * Vol, wb = cc_weight(e, m - 1) * Vol;
+      for (int cc = 0; cc < 4; cc++) { a[cc] += vals[4 * e + cc] * wa; b[cc] += vals[4 * e + cc] * wb; }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+      for (int cc = 0; cc < 4; cc++) { a[cc] += __shfl_xor(a[cc], off); b[cc] += __shfl_xor(b[cc], off); }
     bool conv = true;
     for (int cc = 0; cc < 4; cc++) {
-      double e = fabs(val[cc] - val1[cc]);
+      val[cc] = a[cc]; val1[cc] = b[cc];
+      const double e = fabs(val[cc] - val1[cc]);
       if (e > 0.0 && e > fabs(val[cc]) * 1.0e-10) conv = false;
     }
     if (conv || numEval > 1000 || m >= TD_MAXM) break;
     m += 1;
-    int nl = 1 << (m - 1);
-    for (int i = 0; i < nl; i++) {
-      double x = c_ccx[nl + i];
-      ct_integrand(J, c + rr * x, vals + 4 * (nstored++));
-      ct_integrand(J, c - rr * x, vals + 4 * (nstored++));
+    const int nl = 1 << (m - 1);
+    for (int k = lane; k < 2 * nl; k += 64) {
+      const double x = c_ccx[nl + (k >> 1)];
+      ct_integrand(J, (k & 1) ? c - rr * x : c + rr * x, vals + 4 * (nstored + k));
     }
+    nstored += 2 * nl;
     numEval += 2 * nl;
+    __syncthreads();
   }
-  cplx I0 = mk(val[0], val[1]), I1 = mk(val[2], val[3]);
-  cplx k = J.k;
-  cplx H0 = I1 - 4.0 * I0 / (k * k);
-  double *o = HS + ((long long)p * nreg + r) * 18 + 2 * q;
-  o[0] = H0.r; o[1] = H0.i;
+  if (lane == 0) {
+    cplx I0 = mk(val[0], val[1]), I1 = mk(val[2], val[3]);
+    cplx k = J.k;
+    cplx H0 = I1 - 4.0 * I0 / (k * k);
+    double *o = HS + ((long long)p * nreg + r) * 18 + 2 * q;
+    o[0] = H0.r; o[1] = H0.i;
+  }
 }
 
 static bool g_cc_ready[64] = {false};
@@ -277,8 +279,7 @@ hipError_t launch_self(hipStream_t stream, const double *V, const int *PVI, int 
   int njobs = nP * 9 * nreg;
   for (int j0 = 0; j0 < njobs; j0 += chunk) {
     int nj = njobs - j0 < chunk ? njobs - j0 : chunk;
-    hipLaunchKernelGGL(k_self, dim3((nj + 63) / 64), dim3(64), 0, stream, V, PVI, nP, nreg, regs,
-                       HS, scratch, j0, nj);
+    hipLaunchKernelGGL(k_self, dim3(nj), dim3(64), 0, stream, V, PVI, nP, nreg, regs, HS, scratch, j0, nj);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -54,6 +54,33 @@ template <typename T>
 int lu_solve_noalloc(const T *LU, long n, long lda, const int *d_ipiv, T *B, hipStream_t stream);
 long noalloc_scratch_bytes();
 
+// Batched complex64 LU with tournament pivoting of nbat matrices of order
+// n at A + b sa (ipiv + b sp), all in lockstep; no host synchronization,
+// allocation or memset (a stream capture can record it).  scratch:
+// nbat * lu_batch_scratch_bytes(n) bytes; status (device): per instance
+// {LAPACK info, bounded-wait time-out flag}.  Returns -5 if the batch's
+// panel workgroups cannot all be resident (nbat > lu_batch_max(n)).
+size_t lu_batch_scratch_bytes(long n);
+int lu_batch_max(long n);
+// nb: columns per block (a multiple of 32, at most 256)
+int lu_factor_batch(float2 *A, long n, long lda, int nbat, long sa, int *ipiv, long sp, void *scratch, int *status,
+                    hipStream_t s, int nb = 256);
+// one-right-hand-side solves with those factors: prepare once per
+// factorization (permutations, inverses of the diagonal blocks into ws,
+// nbat * lu_solve_batch_bytes(n) bytes), then solve B (n per instance, sb
+// apart) in place as often as needed
+size_t lu_solve_batch_bytes(long n);
+int lu_solve_batch_prepare(const float2 *LU, long n, long lda, int nbat, long sa, const int *ipiv, long sp, void *ws,
+                           hipStream_t s);
+int lu_solve_batch(const float2 *LU, long n, long lda, int nbat, long sa, float2 *B, long sb, void *ws, hipStream_t s);
+// r = b - A x for a batch (A + z sa; x, b, r n apart; work n * chunks apart)
+// per instance: max|r| / max|b| over n entries (|re| + |im|), out on the device
+int batch_relres(const double2 *r, const double2 *b, long n, int nbat, double *out, hipStream_t s);
+// x[0..n) = 0 by a kernel (no memset node in a captured sequence)
+int zero_c128(double2 *x, long n, hipStream_t s);
+int residual_c128_batch(const double2 *A, long n, long lda, long sa, int nbat, const double2 *x, const double2 *b,
+                        double2 *r, double2 *work, hipStream_t stream);
+
 // log |det| and arg det of an LU-factored matrix (out[0], out[1]; host)
 template <typename T>
 int lu_logdet(const T *LU, long n, long lda, const int *ipiv, double out[2], hipStream_t stream);

@@ -121,6 +121,18 @@ template <typename T> __device__ inline T cdiv(T a, T b)
   return r;
 }
 
+// Strides between the instances of a batch of equal-size factorizations:
+// matrix elements (a), pivot entries (p) and bytes of each instance's
+// panel scratch (s).  A single factorization is a batch of one, strides 0;
+// batched kernels take the instance from blockIdx.y or blockIdx.z.
+struct BStride {
+  long a = 0, p = 0, s = 0;
+};
+template <typename P> __device__ inline P *bshift(P *p, long bytes)
+{
+  return p ? (P *)((char *)p + bytes) : p;
+}
+
 // ---------------------------------------------------------------------
 // C[m x n] -= A[m x k] B[k x n]  (ACC)  or  C = A B  (!ACC);
 // 64x64 block tile, 4 waves of 32x32, K staged 16 at a time through LDS
@@ -132,8 +144,12 @@ template <typename T> __device__ inline T cdiv(T a, T b)
 
 template <typename T, bool ACC>
 __global__ void __launch_bounds__(256)
-k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc)
+k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc, long bsa)
 {
+  {
+    const long z = blockIdx.z;   // instance of a batch (bsa elements apart)
+    A += z * bsa; B += z * bsa; C += z * bsa;
+  }
   typedef typename CT<T>::R R;
   typedef typename CT<T>::V4 V4;
   __shared__ R sAr[GBK][GBM + 4], sAi[GBK][GBM + 4], sBr[GBK][GBN + 4], sBi[GBK][GBN + 4];
@@ -219,8 +235,12 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
 // strip (2 x 4 tiles of 16x16), so one LDS read of A feeds four products.
 template <bool ACC>
 __global__ void __launch_bounds__(256)
-k_gemm_r(long m, long n, long k, const double *A, long lda, const double *B, long ldb, double *C, long ldc)
+k_gemm_r(long m, long n, long k, const double *A, long lda, const double *B, long ldb, double *C, long ldc, long bsa)
 {
+  {
+    const long z = blockIdx.z;
+    A += z * bsa; B += z * bsa; C += z * bsa;
+  }
   __shared__ double sA[GBK][2 * GBM + 4], sB[GBK][GBN + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long row0 = (long)blockIdx.x * 2 * GBM, col0 = (long)blockIdx.y * GBN;
@@ -287,26 +307,26 @@ static int gemm_kind(const double *) { return PK_GEMM_F64; }
 // workgroups share a CU
 template <typename T, bool ACC>
 static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
-                 hipStream_t s, unsigned lds_pad = 0)
+                 hipStream_t s, unsigned lds_pad = 0, int nbat = 1, long bsa = 0)
 {
-  if (m <= 0 || n <= 0 || k <= 0) return;
+  if (m <= 0 || n <= 0 || k <= 0 || nbat <= 0) return;
   int tok = prof_begin(s, gemm_kind(A));
   if constexpr (IsReal<T>::v) {
-    dim3 grid((unsigned)((m + 2 * GBM - 1) / (2 * GBM)), (unsigned)((n + GBN - 1) / GBN));
-    hipLaunchKernelGGL((k_gemm_r<ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc);
-    prof_end(s, tok, 2.0 * (double)m * (double)n * (double)k);
+    dim3 grid((unsigned)((m + 2 * GBM - 1) / (2 * GBM)), (unsigned)((n + GBN - 1) / GBN), (unsigned)nbat);
+    hipLaunchKernelGGL((k_gemm_r<ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc, bsa);
+    prof_end(s, tok, 2.0 * (double)m * (double)n * (double)k * nbat);
   } else {
-    dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN));
-    hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc);
-    prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k);
+    dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN), (unsigned)nbat);
+    hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc, bsa);
+    prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k * nbat);
   }
 }
 
 template <typename T>
 static void gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
-                     hipStream_t s, unsigned lds_pad = 0)
+                     hipStream_t s, unsigned lds_pad = 0, int nbat = 1, long bsa = 0)
 {
-  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s, lds_pad);
+  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s, lds_pad, nbat, bsa);
 }
 
 // ---------------------------------------------------------------------
@@ -336,6 +356,16 @@ struct PanelScratch {
   int *plan;       // [2*PW+1]: dst rows, src rows, count of the panel's net row permutation
   void *linv;      // [npanels][PW][PW] T or null: inverse of each panel's unit-lower block (column-major)
   unsigned long long *ts;   // tournament-pivoting sets and root area (k_panel_ts), or null
+  // the same scratch of the instance `bytes` further on (a batch's scratch
+  // areas are equal and contiguous)
+  __device__ inline PanelScratch shift(long bytes) const
+  {
+    PanelScratch o = *this;
+    o.bar = bshift(bar, bytes); o.candv = bshift(candv, bytes); o.ckey = bshift(ckey, bytes);
+    o.candrow = bshift((char *)candrow, bytes); o.rowc = bshift((char *)rowc, bytes);
+    o.plan = bshift(plan, bytes); o.linv = bshift((char *)linv, bytes); o.ts = bshift(ts, bytes);
+    return o;
+  }
 };
 
 
@@ -772,7 +802,8 @@ __device__ void ts_wait(const unsigned long long *f, long stride, int n, unsigne
 // (at most 168 VGPRs, so that a workgroup fits on SIMDs that run three
 // waves of the trailing GEMM instead of waiting for them to retire)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax, unsigned seq0)
+k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, PanelScratch S, int Gmax, unsigned seq0,
+           BStride bs)
 {
   constexpr int NR = 256;
   __shared__ float2 sF[PW][PW + 1];     // final block, row-major
@@ -784,6 +815,14 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   __shared__ long sidx[NR];
   // the panel is the critical path and the trailing GEMM shares its CUs:
   // its waves win instruction issue on their SIMDs
+  {
+    // instance blockIdx.y of a batch
+    const long bi = blockIdx.y;
+    A += bi * bs.a;
+    ipiv += bi * bs.p;
+    info = bshift(info, bi * bs.s);
+    S = S.shift(bi * bs.s);
+  }
   __builtin_amdgcn_s_setprio(3);
   // workgroups 0 .. G - 1 hold the panel's rows; workgroup G only inverts
   // the final block's unit-lower triangle, beside the others' last phase
@@ -1030,8 +1069,13 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
 #define PLAN_LEN (4 * PW + 1)
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_laswp_plans(T *A, long lda, long c0, long c1, long s0, long s1, const int *plans, int nplans)
+k_laswp_plans(T *A, long lda, long c0, long c1, long s0, long s1, const int *plans, int nplans, BStride bs)
 {
+  {
+    const long bi = blockIdx.y;   // instance of a batch
+    A += bi * bs.a;
+    plans = bshift(plans, bi * bs.s);
+  }
   const int lane = threadIdx.x & 63;
   const long skip = s1 - s0;
   const long ncol = (c1 - c0) - skip;
@@ -1062,13 +1106,14 @@ k_laswp_plans(T *A, long lda, long c0, long c1, long s0, long s1, const int *pla
 
 template <typename T>
 static void laswp_plans(T *A, long lda, long c0, long c1, long s0, long s1, const int *plans, int nplans,
-                        hipStream_t st)
+                        hipStream_t st, int nbat = 1, BStride bs = BStride())
 {
   long ncol = (c1 - c0) - (s1 - s0);
   if (ncol <= 0 || nplans <= 0) return;
   long nwaves = (ncol + 3) / 4;
   unsigned blocks = (unsigned)std::min<long>(4096, (nwaves + 3) / 4);
-  hipLaunchKernelGGL(k_laswp_plans<T>, dim3(blocks), dim3(256), 0, st, A, lda, c0, c1, s0, s1, plans, nplans);
+  hipLaunchKernelGGL(k_laswp_plans<T>, dim3(blocks, (unsigned)nbat), dim3(256), 0, st, A, lda, c0, c1, s0, s1, plans,
+                     nplans, bs);
 }
 
 // row interchanges rows [k1,k2) (absolute, ipiv 1-based) on columns [c0,c1)
@@ -1267,8 +1312,15 @@ __device__ inline double readlane_f(double v, int k)
 }
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan, const T *Linv, int w, long r0)
+k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan, const T *Linv, int w, long r0,
+            BStride bs)
 {
+  {
+    const long bi = blockIdx.y;   // instance of a batch
+    A += bi * bs.a;
+    plan = bshift(plan, bi * bs.s);
+    Linv = bshift(Linv, bi * bs.s);
+  }
   const int lane = threadIdx.x & 63;
   const long ci = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const long nleft = s0 - c0;
@@ -1304,13 +1356,13 @@ k_swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan,
 
 template <typename T>
 static void swap_trmm(T *A, long lda, long c0, long c1, long s0, long s1, const int *plan, const T *Linv, int w,
-                      long r0, hipStream_t s)
+                      long r0, hipStream_t s, int nbat = 1, BStride bs = BStride())
 {
   const long ncol = (c1 - c0) - (s1 - s0);
   if (ncol <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
-  hipLaunchKernelGGL(k_swap_trmm<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s, A, lda, c0, c1, s0, s1, plan,
-                     Linv, w, r0);
+  hipLaunchKernelGGL(k_swap_trmm<T>, dim3((unsigned)((ncol + 3) / 4), (unsigned)nbat), dim3(256), 0, s, A, lda, c0, c1,
+                     s0, s1, plan, Linv, w, r0, bs);
   prof_end(s, tok, 4.0 * (double)w * (double)w * (double)(c1 - s1));
 }
 
@@ -1337,8 +1389,14 @@ static void trsm_block_inv(const T *A, long lda, long r0, int nb, const T *linv,
 // l + 128, l + 192; the panel rows are read across lanes by lane reads.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol)
+k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol, BStride bs)
 {
+  {
+    const long bi = blockIdx.y;   // instance of a batch
+    A += bi * bs.a;
+    B += bi * bs.a;
+    linv = bshift(linv, bi * bs.s);
+  }
   const int lane = threadIdx.x & 63;
   const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncol) return;
@@ -1404,12 +1462,13 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
 }
 
 template <typename T>
-static void trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol, hipStream_t s)
+static void trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol, hipStream_t s,
+                     int nbat = 1, BStride bs = BStride())
 {
   if (ncol <= 0 || jb <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
-  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s, A, lda, r0, jb, linv, B, ldb,
-                     ncol);
+  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)((ncol + 3) / 4), (unsigned)nbat), dim3(256), 0, s, A, lda, r0, jb,
+                     linv, B, ldb, ncol, bs);
   prof_end(s, tok, 4.0 * (double)jb * (double)jb * (double)ncol);
 }
 
@@ -1486,7 +1545,14 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   S.ckey = (unsigned long long *)(wsb + off_ci);
   S.candrow = (void *)(wsb + off_cr);
   S.rowc = (void *)(wsb + off_rc);
-  hipStream_t side = side_stream();
+  // The trailing update runs beside the next panels on a second stream,
+  // unless the panel needs more than half of the resident workgroup slots
+  // of its kernel: then the GEMM's workgroups, dispatched without pause
+  // on the other stream, could keep part of the panel's grid waiting until
+  // its bounded waits expire (seen at n = 61440 complex128: 480 panel
+  // workgroups of 2 per CU).  Everything then runs on the caller's stream.
+  const bool overlap = 2 * (Gmax + (tourn ? 1 : 0)) <= (long)std::max(1, ncu) * std::max(1, perCU);
+  hipStream_t side = overlap ? side_stream() : stream;
   hipEvent_t evP = nullptr, evR = nullptr;
   hipEventCreateWithFlags(&evP, hipEventDisableTiming);
   hipEventCreateWithFlags(&evR, hipEventDisableTiming);
@@ -1512,8 +1578,9 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       S.plan = plans + (j0 / PW) * PLAN_LEN;
       // column sequence numbers: fresh for every panel of this factorization
       unsigned seq0 = (unsigned)(j0 + 1);
+      BStride one;
       void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&d_ipiv,
-                      (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0};
+                      (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0, (void *)&one};
       int tok = prof_begin(stream, IsReal<T>::v ? PK_PANEL_F64
                                    : std::is_same<T, double2>::value ? PK_PANEL_C128 : PK_PANEL_C64);
       // the tournament panel has no grid barrier, only bounded waits along
@@ -1681,8 +1748,10 @@ __global__ void k_laswp_rev(T *B, long ldb, long n, long nrhs, const int *ipiv)
 // sides are then gathered (forward) or scattered (reverse) in parallel,
 // instead of one thread per column walking all n interchanges in memory.
 #define PERM_LDS_MAX 20480
-__global__ void __launch_bounds__(64) k_perm_build(const int *ipiv, long n, int *perm)
+__global__ void __launch_bounds__(64) k_perm_build(const int *ipiv, long n, int *perm, long sp, long sperm)
 {
+  ipiv += (long)blockIdx.x * sp;   // instance blockIdx.x of a batch
+  perm += (long)blockIdx.x * sperm;
   __shared__ int P[PERM_LDS_MAX], V[PERM_LDS_MAX];
   const int lane = threadIdx.x;
   for (long i = lane; i < n; i += 64) { P[i] = (int)i; V[i] = ipiv[i] - 1; }
@@ -1705,8 +1774,13 @@ __global__ void __launch_bounds__(64) k_perm_build(const int *ipiv, long n, int 
 
 // forward: out[s] = in[perm[s]]; reverse: out[perm[s]] = in[s]
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(256) k_perm_apply(const T *in, long ldi, T *out, long ldo, const int *perm, long n)
+__global__ void __launch_bounds__(256) k_perm_apply(const T *in, long ldi, T *out, long ldo, const int *perm, long n,
+                                                    long sin, long sout, long sperm)
 {
+  {
+    const long z = blockIdx.z;   // instance of a batch
+    in += z * sin; out += z * sout; perm += z * sperm;
+  }
   const long s = (long)blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   const long c = blockIdx.y;
@@ -1729,8 +1803,12 @@ static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, b
 #define TB 64
 template <typename T, bool LOWER>
 __global__ void __launch_bounds__(TB)
-k_diag_inv(const T *LU, long lda, long n, T *inv)
+k_diag_inv(const T *LU, long lda, long n, T *inv, long sa, long sinv)
 {
+  {
+    const long z = blockIdx.z;   // instance of a batch
+    LU += z * sa; inv += z * sinv;
+  }
   __shared__ T D[TB][TB + 1];
   __shared__ T X[TB][TB + 1];       // X[r][j]: column j of the inverse
   const int t = threadIdx.x;
@@ -1771,8 +1849,13 @@ k_diag_inv(const T *LU, long lda, long n, T *inv)
 // updated with x_b (LOWER: rows below, else rows above); blockIdx.y = rhs
 template <typename T, bool LOWER>
 __global__ void __launch_bounds__(256)
-k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin, long ldb, T *Xout, long ldx)
+k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin, long ldb, T *Xout, long ldx,
+           long sa, long sinv, long sb, long sx)
 {
+  {
+    const long z = blockIdx.z;   // instance of a batch
+    LU += z * sa; invb += z * sinv; Bin += z * sb; Xout += z * sx;
+  }
   __shared__ T rs[TB], xs[TB];
   const int t = threadIdx.x;
   T *b = Bin + (long)blockIdx.y * ldb;
@@ -1988,10 +2071,10 @@ static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, b
     else hipLaunchKernelGGL(k_laswp_rev<T>, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, B, ldb, n, nrhs, ipiv);
     return;
   }
-  hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(64), 0, s, ipiv, n, perm);
+  hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(64), 0, s, ipiv, n, perm, 0L, 0L);
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)nrhs);
-  if (fwd) hipLaunchKernelGGL((k_perm_apply<T, true>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n);
-  else hipLaunchKernelGGL((k_perm_apply<T, false>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n);
+  if (fwd) hipLaunchKernelGGL((k_perm_apply<T, true>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n, 0L, 0L, 0L);
+  else hipLaunchKernelGGL((k_perm_apply<T, false>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n, 0L, 0L, 0L);
   hipMemcpy2DAsync(B, sizeof(T) * ldb, tmp, sizeof(T) * n, sizeof(T) * n, nrhs, hipMemcpyDeviceToDevice, s);
 }
 
@@ -2032,8 +2115,8 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
     T *invU = ws ? ws->invU : (T *)ws_get(WS_SOLVE_INVU, sizeof(T) * nblk * TB * TB);
     if (!W || !invL || !invU) return -2;
     if (!ws || !ws->inv_ready) {
-      hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL);
-      hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU);
+      hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL, 0L, 0L);
+      hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU, 0L, 0L);
       if (ws) ws->inv_ready = true;
     }
     apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream, ws ? ws->perm : nullptr);
@@ -2047,7 +2130,7 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
       long rows = n - kb - bs;
       dim3 grid((unsigned)std::max<long>(1, (rows + 255) / 256), (unsigned)nrhs);
       hipLaunchKernelGGL((k_trsv_inv<T, true>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs,
-                         invL + (kb / TB) * TB * TB, B, ldb, W, n);
+                         invL + (kb / TB) * TB * TB, B, ldb, W, n, 0L, 0L, 0L, 0L);
     }
     // U x = y: W is updated in place, the solution lands in B
     long last = ((n - 1) / TB) * TB;
@@ -2055,7 +2138,7 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
       int bs = (int)std::min<long>(TB, n - kb);
       dim3 grid((unsigned)std::max<long>(1, (kb + 255) / 256), (unsigned)nrhs);
       hipLaunchKernelGGL((k_trsv_inv<T, false>), grid, dim3(256), 0, stream, LU, lda, n, kb, bs,
-                         invU + (kb / TB) * TB * TB, W, n, B, ldb);
+                         invU + (kb / TB) * TB * TB, W, n, B, ldb, 0L, 0L, 0L, 0L);
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
   }
@@ -2216,8 +2299,12 @@ template int lu_solve_noalloc<float2>(const float2 *, long, long, const int *, f
 // ---------------------------------------------------------------------
 #define RES_COLCHUNK 512
 __global__ void __launch_bounds__(256)
-k_resid_partial(const double2 *A, long n, long lda, const double2 *x, double2 *work)
+k_resid_partial(const double2 *A, long n, long lda, const double2 *x, double2 *work, long sa, long sx, long sw)
 {
+  {
+    const long z = blockIdx.z;   // instance of a batch
+    A += z * sa; x += z * sx; work += z * sw;
+  }
   long i = (long)blockIdx.x * 256 + threadIdx.x;
   long c0 = (long)blockIdx.y * RES_COLCHUNK, c1 = std::min(n, c0 + RES_COLCHUNK);
   if (i >= n) return;
@@ -2230,8 +2317,14 @@ k_resid_partial(const double2 *A, long n, long lda, const double2 *x, double2 *w
   work[blockIdx.y * n + i] = make_double2(sr, si);
 }
 
-__global__ void k_resid_final(long n, int nch, const double2 *b, const double2 *work, double2 *r)
+__global__ void k_resid_final(long n, int nch, const double2 *b, const double2 *work, double2 *r, long sb, long sw,
+                              long srr)
 {
+  {
+    const long z = blockIdx.y;   // instance of a batch
+    if (b) b += z * sb;
+    work += z * sw; r += z * srr;
+  }
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double sr = 0, si = 0;
@@ -2245,8 +2338,10 @@ int residual_c128(const double2 *A, long n, long lda, const double2 *x, const do
                   double2 *work, hipStream_t stream)
 {
   int nch = (int)((n + RES_COLCHUNK - 1) / RES_COLCHUNK);
-  hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)((n + 255) / 256), nch), dim3(256), 0, stream, A, n, lda, x, work);
-  hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, nch, b, work, r);
+  hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)((n + 255) / 256), nch), dim3(256), 0, stream, A, n, lda, x, work,
+                     0L, 0L, 0L);
+  hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, nch, b, work, r, 0L, 0L,
+                     0L);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -2400,6 +2495,253 @@ double absmax_c128(const double2 *x, long n, double *work, hipStream_t stream)
   hipMemcpyAsync(&h, work, sizeof(double), hipMemcpyDeviceToHost, stream);
   hipStreamSynchronize(stream);
   return h;
+}
+
+// ---------------------------------------------------------------------
+// Batched complex64 LU with tournament pivoting: nbat matrices of equal
+// order n at A + b sa, factored in lockstep, every step one launch over
+// the whole batch (panels: (G + 1) x nbat workgroups; updates: grid.y or
+// grid.z = instance).  No lookahead (the batch fills the GPU), no host
+// synchronization, no allocation and no memset, so that a stream capture
+// records it.  The per-instance panel scratch (layout of lu_factor's
+// complex64 tournament path) is cleared by a kernel at the start.
+// ---------------------------------------------------------------------
+struct BatchLayout {
+  size_t off_info, off_plan, off_li, off_ts, total;
+  long Gmax, npanels;
+  size_t off_cv, off_ci, off_cr, off_rc;
+};
+
+static BatchLayout batch_layout(long n)
+{
+  auto r256 = [](size_t b) { return 256 * ((b + 255) / 256); };
+  BatchLayout L;
+  L.Gmax = (n + 255) / 256;
+  L.npanels = (n + PW - 1) / PW;
+  L.off_info = 96 * sizeof(unsigned);
+  L.off_plan = L.off_info + 64;
+  L.off_cv = L.off_plan + r256(sizeof(int) * PLAN_LEN * L.npanels);
+  L.off_ci = L.off_cv + r256(sizeof(double) * 2 * L.Gmax);
+  L.off_cr = L.off_ci + r256(sizeof(unsigned long long) * 2 * L.Gmax);
+  L.off_rc = L.off_cr + r256(sizeof(float2) * 2 * L.Gmax * PW);
+  L.off_li = L.off_rc + r256(sizeof(float2) * 2 * PW);
+  L.off_ts = L.off_li + r256(sizeof(float2) * PW * PW * L.npanels);
+  const size_t ts_words = (size_t)ts_root_off(ts_levels(L.Gmax), L.Gmax) + TS_ROOT;
+  L.total = r256(L.off_ts + sizeof(unsigned long long) * ts_words);
+  return L;
+}
+
+size_t lu_batch_scratch_bytes(long n) { return batch_layout(n).total; }
+
+__global__ void k_batch_status(const char *scratch, long sS, long off_info, int nbat, int *status)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbat) return;
+  const char *base = scratch + (long)b * sS;
+  status[2 * b] = *(const int *)(base + off_info);
+  status[2 * b + 1] = (int)((const unsigned *)base)[64];
+}
+
+int lu_batch_max(long n)
+{
+  int dev = 0, ncu = 0, perCU = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void *)k_panel_ts, 256, 0);
+  const long per = (n + 255) / 256 + 1;
+  return (int)((long)std::max(1, ncu) * std::max(1, perCU) / per);
+}
+
+int lu_factor_batch(float2 *A, long n, long lda, int nbat, long sa, int *ipiv, long sp, void *scratch, int *status,
+                    hipStream_t s, int nb)
+{
+  if (n <= 0 || nbat <= 0 || lda < n || !scratch || !status) return -1;
+  if (nb <= 0 || nb > 256 || nb % PW) return -1;
+  const BatchLayout L = batch_layout(n);
+  if (L.Gmax > 64 * KEYS_PER_LANE) return -5;
+  if (nbat > lu_batch_max(n)) return -5;      // every panel workgroup of the batch must be co-resident
+  const long sS = (long)L.total;
+  char *base = (char *)scratch;
+  const long nwords = (long)nbat * sS / 8;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s,
+                     (unsigned long long *)base, nwords);
+  PanelScratch S;
+  S.bar = (unsigned *)base;
+  int *dinfo = (int *)(base + L.off_info);
+  int *plans = (int *)(base + L.off_plan);
+  S.candv = (double *)(base + L.off_cv);
+  S.ckey = (unsigned long long *)(base + L.off_ci);
+  S.candrow = (void *)(base + L.off_cr);
+  S.rowc = (void *)(base + L.off_rc);
+  S.linv = (void *)(base + L.off_li);
+  S.ts = (unsigned long long *)(base + L.off_ts);
+  const float2 *linv = (const float2 *)S.linv;
+  BStride bs;
+  bs.a = sa; bs.p = sp; bs.s = sS;
+  int gmaxi = (int)L.Gmax;
+  for (long K = 0; K < n; K += nb) {
+    const long jb = std::min<long>(nb, n - K);
+    const long p0 = K / PW;
+    for (long j0 = K; j0 < K + jb; j0 += PW) {
+      int w = (int)std::min<long>(PW, K + jb - j0);
+      long m = n - j0, G = (m + 255) / 256, r0 = j0;
+      S.plan = plans + (j0 / PW) * PLAN_LEN;
+      unsigned seq0 = (unsigned)(j0 + 1);
+      void *args[] = {(void *)&A, (void *)&lda, (void *)&r0, (void *)&m, (void *)&w, (void *)&ipiv,
+                      (void *)&dinfo, (void *)&S, (void *)&gmaxi, (void *)&seq0, (void *)&bs};
+      int tok = prof_begin(s, PK_PANEL_C64);
+      hipError_t e = hipLaunchKernel((const void *)k_panel_ts, dim3((unsigned)G + 1, (unsigned)nbat), dim3(256), args,
+                                     0, s);
+      prof_end(s, tok, 0.0);
+      if (e != hipSuccess) return -3;
+      swap_trmm(A, lda, K, K + jb, j0, j0 + w, S.plan, linv + (j0 / PW) * PW * PW, w, j0, s, nbat, bs);
+      const long nr = K + jb - (j0 + w);
+      if (nr > 0)
+        gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
+                 A + (j0 + w) + (j0 + w) * lda, lda, s, 0, nbat, sa);
+    }
+    const long c1 = K + jb;
+    if (c1 < n) {
+      const int bplans = (int)((jb + PW - 1) / PW);
+      laswp_plans(A, lda, c1, n, 0, 0, plans + p0 * PLAN_LEN, bplans, s, nbat, bs);
+      trsm_blk(A, lda, K, (int)jb, linv + p0 * PW * PW, A + K + c1 * lda, lda, n - c1, s, nbat, bs);
+      gemm_sub(n - c1, n - c1, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, s, 0, nbat,
+               sa);
+    }
+  }
+  // interchanges of each block on the columns left of it
+  for (long K = nb; K < n; K += nb) {
+    const long jb = std::min<long>(nb, n - K);
+    laswp_plans(A, lda, 0, K, 0, 0, plans + (K / PW) * PLAN_LEN, (int)((jb + PW - 1) / PW), s, nbat, bs);
+  }
+  hipLaunchKernelGGL(k_batch_status, dim3((unsigned)((nbat + 63) / 64)), dim3(64), 0, s, (const char *)base, sS,
+                     (long)L.off_info, nbat, status);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// Batched one-right-hand-side solves with batched complex64 factors:
+// prepare (composed row permutations and inverses of the 64 x 64 diagonal
+// blocks of L and U, once per factorization), then any number of solves
+// of B (n per instance, sb apart) in place.  No host synchronization.
+struct BatchSolveLayout {
+  size_t off_invL, off_invU, off_W, total;
+};
+static BatchSolveLayout batch_solve_layout(long n)
+{
+  auto r256 = [](size_t b) { return 256 * ((b + 255) / 256); };
+  const long nblk = (n + TB - 1) / TB;
+  BatchSolveLayout L;
+  L.off_invL = r256(sizeof(int) * n);
+  L.off_invU = L.off_invL + r256(sizeof(float2) * nblk * TB * TB);
+  L.off_W = L.off_invU + r256(sizeof(float2) * nblk * TB * TB);
+  L.total = L.off_W + r256(sizeof(float2) * n);
+  return L;
+}
+size_t lu_solve_batch_bytes(long n) { return batch_solve_layout(n).total; }
+
+int lu_solve_batch_prepare(const float2 *LU, long n, long lda, int nbat, long sa, const int *ipiv, long sp, void *ws,
+                           hipStream_t s)
+{
+  if (n <= 0 || nbat <= 0 || n > PERM_LDS_MAX || !ws) return -1;
+  const BatchSolveLayout L = batch_solve_layout(n);
+  const long sw = (long)L.total, nblk = (n + TB - 1) / TB;
+  char *base = (char *)ws;
+  hipLaunchKernelGGL(k_perm_build, dim3((unsigned)nbat), dim3(64), 0, s, ipiv, n, (int *)base, sp, sw / 4);
+  hipLaunchKernelGGL((k_diag_inv<float2, true>), dim3((unsigned)nblk, 1, (unsigned)nbat), dim3(TB), 0, s, LU, lda, n,
+                     (float2 *)(base + L.off_invL), sa, sw / 8);
+  hipLaunchKernelGGL((k_diag_inv<float2, false>), dim3((unsigned)nblk, 1, (unsigned)nbat), dim3(TB), 0, s, LU, lda, n,
+                     (float2 *)(base + L.off_invU), sa, sw / 8);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+__global__ void k_copy_batch(const float2 *in, long sin, float2 *out, long sout, long n)
+{
+  const long i = (long)blockIdx.x * 256 + threadIdx.x, z = blockIdx.z;
+  if (i < n) out[z * sout + i] = in[z * sin + i];
+}
+
+int lu_solve_batch(const float2 *LU, long n, long lda, int nbat, long sa, float2 *B, long sb, void *ws, hipStream_t s)
+{
+  if (n <= 0 || nbat <= 0 || !ws) return -1;
+  const BatchSolveLayout L = batch_solve_layout(n);
+  const long sw = (long)L.total;
+  char *base = (char *)ws;
+  const int *perm = (const int *)base;
+  const float2 *invL = (const float2 *)(base + L.off_invL), *invU = (const float2 *)(base + L.off_invU);
+  float2 *W = (float2 *)(base + L.off_W);
+  const long swp = sw / 4, swe = sw / 8;   // the stride in ints and in complex64 elements
+  // W = P B
+  hipLaunchKernelGGL((k_perm_apply<float2, true>), dim3((unsigned)((n + 255) / 256), 1, (unsigned)nbat), dim3(256), 0,
+                     s, (const float2 *)B, n, W, n, perm, n, sb, swe, swp);
+  // L y = W: W is the running right-hand side, y lands in B
+  for (long kb = 0; kb < n; kb += TB) {
+    const int bs = (int)std::min<long>(TB, n - kb);
+    const long rows = n - kb - bs;
+    dim3 grid((unsigned)std::max<long>(1, (rows + 255) / 256), 1, (unsigned)nbat);
+    hipLaunchKernelGGL((k_trsv_inv<float2, true>), grid, dim3(256), 0, s, LU, lda, n, kb, bs,
+                       invL + (kb / TB) * TB * TB, W, n, B, n, sa, swe, swe, sb);
+  }
+  // U x = y: B is the running right-hand side, x lands in W
+  const long last = ((n - 1) / TB) * TB;
+  for (long kb = last; kb >= 0; kb -= TB) {
+    const int bs = (int)std::min<long>(TB, n - kb);
+    dim3 grid((unsigned)std::max<long>(1, (kb + 255) / 256), 1, (unsigned)nbat);
+    hipLaunchKernelGGL((k_trsv_inv<float2, false>), grid, dim3(256), 0, s, LU, lda, n, kb, bs,
+                       invU + (kb / TB) * TB * TB, B, n, W, n, sa, swe, sb, swe);
+  }
+  // B = x
+  hipLaunchKernelGGL(k_copy_batch, dim3((unsigned)((n + 255) / 256), 1, (unsigned)nbat), dim3(256), 0, s,
+                     (const float2 *)W, swe, B, sb, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// per instance of a batch: max|r_i| / max|b_i| (|re| + |im|), vectors n apart
+__global__ void k_batch_relres(const double2 *r, const double2 *b, long n, double *out)
+{
+  __shared__ double sr[256], sb[256];
+  const long z = blockIdx.x;
+  r += z * n; b += z * n;
+  double mr = 0, mb = 0;
+  for (long i = threadIdx.x; i < n; i += 256) {
+    mr = fmax(mr, fabs(r[i].x) + fabs(r[i].y));
+    mb = fmax(mb, fabs(b[i].x) + fabs(b[i].y));
+  }
+  sr[threadIdx.x] = mr; sb[threadIdx.x] = mb;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) {
+    if (threadIdx.x < q) { sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + q]); sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + q]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[z] = sb[0] > 0 ? sr[0] / sb[0] : sr[0];
+}
+int batch_relres(const double2 *r, const double2 *b, long n, int nbat, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_batch_relres, dim3((unsigned)nbat), dim3(256), 0, s, r, b, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+__global__ void k_zero_c128(double2 *x, long n)
+{
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = make_double2(0.0, 0.0);
+}
+int zero_c128(double2 *x, long n, hipStream_t s)
+{
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_zero_c128, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// r = b - A x for a batch (A + z sa, x / b / r n apart, work n * chunks apart)
+int residual_c128_batch(const double2 *A, long n, long lda, long sa, int nbat, const double2 *x, const double2 *b,
+                        double2 *r, double2 *work, hipStream_t stream)
+{
+  const int nch = (int)((n + RES_COLCHUNK - 1) / RES_COLCHUNK);
+  hipLaunchKernelGGL(k_resid_partial, dim3((unsigned)((n + 255) / 256), nch, (unsigned)nbat), dim3(256), 0, stream, A,
+                     n, lda, x, work, sa, n, n * (long)nch);
+  hipLaunchKernelGGL(k_resid_final, dim3((unsigned)((n + 255) / 256), (unsigned)nbat), dim3(256), 0, stream, n, nch, b,
+                     work, r, n, n * (long)nch, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 }  // namespace scuffx

@@ -9,10 +9,11 @@ namespace scuffx {
 
 using cdouble = std::complex<double>;
 
-// EV[4e] = iV1, iV2, iQP, iQM ; EI[4e] = iPPanel, iMPanel, PIndex, MIndex
+// EV[4e] = iV1, iV2, iQP, iQM ; EI[4e] = iPPanel, iMPanel, PIndex, MIndex;
+// kz (optional, device): K, Z read from memory instead of the values given
 hipError_t launch_rhs(hipStream_t s, const double *V, const int *PVI, const double *PA, const int *EV,
                       const int *EI, const double *EL, int nE, int pec, double sgn, const cdouble *E0,
-                      const double *nhat, cdouble K, cdouble Z, double2 *rhs);
+                      const double *nhat, cdouble K, cdouble Z, double2 *rhs, const double *kz = nullptr);
 // out2 = {extinction, absorbed}; work holds 2*nE doubles
 hipError_t launch_power(hipStream_t s, const double *V, const int *PVI, const double *PA, const double *PZ,
                         const int *PEI, const int *EI, const double *EL, int nE, int pec, const double2 *KN,

@@ -53,12 +53,15 @@ __device__ void panel_ip(const double *V1, const double *V2, const double *V3, c
   }
 }
 
+// kz (optional, device): K and Z as re, im, re, im in place of the values
+// passed (a recorded sequence then reads the frequency from memory)
 __global__ void k_rhs(const double *V, const int *PVI, const double *PA, const int *EV, const int *EI,
                       const double *EL, int nE, int pec, double sgn, cplx e0x, cplx e0y, cplx e0z,
-                      double nx, double ny, double nz, cplx K, cplx Z, double2 *rhs)
+                      double nx, double ny, double nz, cplx K, cplx Z, const double *kz, double2 *rhs)
 {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nE) return;
+  if (kz) { K = mk(kz[0], kz[1]); Z = mk(kz[2], kz[3]); }
   const cplx E0[3] = {e0x, e0y, e0z};
   const double nh[3] = {nx, ny, nz};
   const double *QP = V + 3 * EV[4 * e + 2], *V1 = V + 3 * EV[4 * e], *V2 = V + 3 * EV[4 * e + 1];
@@ -80,7 +83,7 @@ static bool g_tcr20_ready[64] = {false};
 
 hipError_t launch_rhs(hipStream_t s, const double *V, const int *PVI, const double *PA, const int *EV,
                       const int *EI, const double *EL, int nE, int pec, double sgn, const cdouble *E0,
-                      const double *nhat, cdouble K, cdouble Z, double2 *rhs)
+                      const double *nhat, cdouble K, cdouble Z, double2 *rhs, const double *kz)
 {
   int dev = 0;
   hipGetDevice(&dev);
@@ -92,7 +95,7 @@ hipError_t launch_rhs(hipStream_t s, const double *V, const int *PVI, const doub
   if (nE <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_rhs, dim3((nE + 63) / 64), dim3(64), 0, s, V, PVI, PA, EV, EI, EL, nE, pec, sgn,
                      mk(E0[0].real(), E0[0].imag()), mk(E0[1].real(), E0[1].imag()), mk(E0[2].real(), E0[2].imag()),
-                     nhat[0], nhat[1], nhat[2], mk(K.real(), K.imag()), mk(Z.real(), Z.imag()), rhs);
+                     nhat[0], nhat[1], nhat[2], mk(K.real(), K.imag()), mk(Z.real(), Z.imag()), kz, rhs);
   return hipGetLastError();
 }
 

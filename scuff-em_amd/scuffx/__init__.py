@@ -61,6 +61,7 @@ def lib():
             "scuffx_geom_panels": ([vp, C.c_int, vp], C.c_int),
             "scuffx_geom_vertices": ([vp, C.c_int, vp], C.c_int),
             "scuffx_geom_mate": ([vp, C.c_int], C.c_int),
+            "scuffx_geom_epsmu": ([vp, C.c_int, C.c_double, C.c_double, vp, vp], C.c_int),
             "scuffx_trans_open": ([C.c_char_p, vp], C.c_int),
             "scuffx_trans_count": ([vp], C.c_int),
             "scuffx_trans_tag": ([vp, C.c_int, C.c_char_p, C.c_int], C.c_int),
@@ -72,8 +73,15 @@ def lib():
             "scuffx_geom_untransform": ([vp], C.c_int),
             "scuffx_geom_surface_moved": ([vp, C.c_int], C.c_int),
             "scuffx_batch_sweep": ([vp, vp, C.c_int, vp, vp, C.c_int, vp], C.c_int),
+            "scuffx_sweep_batched": ([vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp], C.c_int),
             "scuffx_adjoint": ([vp, C.c_int, C.c_double, C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
             "scuffx_geom_transform_surface": ([vp, C.c_int, vp, vp], C.c_int),
+            "scuffx_geom_set_vertices": ([vp, C.c_int, vp, C.c_int], C.c_int),
+            "scuffx_vertex_slots": ([vp], C.c_int),
+            "scuffx_vertex_gradient": ([vp, C.c_double, C.c_double, vp, C.c_long, vp, vp, vp, C.c_long, vp],
+                                       C.c_int),
+            "scuffx_rhs_vertex_gradient": ([vp, C.c_double, C.c_double, vp, vp, vp, vp], C.c_int),
+            "scuffx_pabs_vertex_gradient": ([vp, vp, vp, vp], C.c_int),
             "scuffx_hdf5_open": ([C.c_char_p, vp], C.c_int),
             "scuffx_hdf5_close": ([vp], C.c_int),
             "scuffx_hdf5_write_matrix": ([vp, C.c_char_p, vp, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -619,6 +627,15 @@ class RWGGeometry:
         (RWGGeometry.cc:733-770)"""
         return [lib().scuffx_geom_mate(self.h, ns) for ns in range(self.NumSurfaces)]
 
+    def GetEpsMu(self, Omega, Region=0):
+        """(eps, mu) of a region at Omega (RegionMPs[r]->GetEpsMu,
+        libMatProp.cc:501); region 0 is the exterior"""
+        e, m = np.zeros(2), np.zeros(2)
+        Omega = complex(Omega)
+        _check(lib().scuffx_geom_epsmu(self.h, Region, Omega.real, Omega.imag, e.ctypes.data_as(C.c_void_p),
+                                       m.ctypes.data_as(C.c_void_p)))
+        return complex(e[0], e[1]), complex(m[0], m[1])
+
     def Label(self, ns=0):
         """surface label: OBJECT label, mesh file name, or ByHand"""
         buf = C.create_string_buffer(4096)
@@ -691,6 +708,68 @@ class RWGGeometry:
         out = np.zeros((nv, 3))
         _check(lib().scuffx_geom_vertices(self.h, ns, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def SetVertices(self, ns, V):
+        """new vertex coordinates (NumVertices x 3) for surface ns, same
+        topology; the next assembly rebuilds the device geometry"""
+        Vc = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        _sync_stream_if_gpu()
+        _check(lib().scuffx_geom_set_vertices(self.h, ns, Vc.ctypes.data_as(C.c_void_p), Vc.shape[0]))
+
+    def _vertex_split(self, g):
+        """(slots,) array -> list of per-surface (NumVertices, 3) arrays"""
+        out, o = [], 0
+        for ns in range(self.NumSurfaces):
+            nv = self.Counts(ns)[0]
+            out.append(g[o:o + 3 * nv].reshape(nv, 3))
+            o += 3 * nv
+        return out
+
+    def VertexGradient(self, Omega, M, u=None, v=None, C=None):
+        """sum_ij C_ij dM_ij / d(vertex coordinates), C = u v^T or a dense
+        complex128 matrix: a list over surfaces of complex (NumVertices, 3)
+        arrays.  M: the matrix assembled at Omega (HMatrix, complex128)."""
+        torch = _torch()
+        n = _check(lib().scuffx_vertex_slots(self.h))
+        g = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        Omega = complex(Omega)
+        N = self.TotalBFs
+        ut = None if u is None else (u.t if isinstance(u, HVector) else u).contiguous()
+        vt = None if v is None else (v.t if isinstance(v, HVector) else v).contiguous()
+        Ct = None if C is None else (C.t if isinstance(C, HMatrix) else C)
+        _sync_stream()
+        _check(lib().scuffx_vertex_gradient(self.h, Omega.real, Omega.imag, _ptr(M.t), N,
+                                            None if ut is None else _ptr(ut), None if vt is None else _ptr(vt),
+                                            None if Ct is None else _ptr(Ct), N, _ptr(g)))
+        return self._vertex_split(g.cpu().numpy())
+
+    def RHSVertexGradient(self, Omega, IF, mu):
+        """sum_i mu_i d RHS_i / d(vertex coordinates) for the plane wave IF"""
+        torch = _torch()
+        n = _check(lib().scuffx_vertex_slots(self.h))
+        g = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        E = np.zeros(6)
+        E[0::2] = IF.E0.real
+        E[1::2] = IF.E0.imag
+        nh = np.ascontiguousarray(IF.nHat)
+        Omega = complex(Omega)
+        mt = (mu.t if isinstance(mu, HVector) else mu).contiguous()
+        _sync_stream()
+        _check(lib().scuffx_rhs_vertex_gradient(self.h, Omega.real, Omega.imag, E.ctypes.data_as(C.c_void_p),
+                                                nh.ctypes.data_as(C.c_void_p), _ptr(mt), _ptr(g)))
+        return self._vertex_split(g.cpu().numpy())
+
+    def PAbsVertexGradient(self, KN):
+        """(explicit derivative of GetPower's OPFT PAbs at fixed KN, per
+        surface (NumVertices, 3) real arrays; gamma, a complex128 device
+        tensor with dPAbs = Re(gamma^T dKN))"""
+        torch = _torch()
+        n = _check(lib().scuffx_vertex_slots(self.h))
+        g = torch.zeros(n, dtype=torch.float64, device="cuda")
+        gam = torch.zeros(self.TotalBFs, dtype=torch.complex128, device="cuda")
+        _sync_stream()
+        _check(lib().scuffx_pabs_vertex_gradient(self.h, _ptr(KN.t), _ptr(g), _ptr(gam)))
+        return self._vertex_split(g.cpu().numpy()), gam
 
     def AllocateBEMMatrix(self, PureImagFreq=False, Packed=False):
         """RWGGeometry::AllocateBEMMatrix (AssembleBEMMatrix.cc:778-783): a
@@ -885,6 +964,33 @@ def BatchSweep(G, Omegas, IF, prec=64):
                                     E0.ctypes.data_as(C.c_void_p), nh.ctypes.data_as(C.c_void_p), prec,
                                     out.ctypes.data_as(C.c_void_p)))
     return out
+
+
+def SweepBatched(Gs, Omegas, IF, RefineSteps=5, Timing=None):
+    """the frequency sweep of several geometries at once: every (geometry,
+    frequency) instance assembled, factored in complex64 by one batched LU
+    per geometry, solved and refined in complex128, powered -- one captured
+    hipGraph (scuffx_sweep_batched).  Returns (P, relres, info): P of shape
+    (len(Gs), len(Omegas), 3) = PAbs, PExt, PScat; relres the final
+    residuals; info = (instances solved again step by step, LU block width
+    chosen for the first geometry)."""
+    Gs = list(Gs)
+    om = np.asarray([complex(w) for w in Omegas], dtype=np.complex128)
+    E0 = np.ascontiguousarray(np.asarray(IF.E0, dtype=np.complex128)).view(np.float64)
+    nh = np.ascontiguousarray(IF.nHat, dtype=np.float64)
+    hs = (C.c_void_p * len(Gs))(*[G.h.value for G in Gs])
+    out = np.zeros((len(Gs), len(om), 3))
+    rel = np.zeros((len(Gs), len(om)))
+    info = np.zeros(2, dtype=np.int32)
+    ms = np.zeros(5)
+    _sync_stream()
+    _check(lib().scuffx_sweep_batched(hs, len(Gs), om.view(np.float64).ctypes.data_as(C.c_void_p), len(om),
+                                      E0.ctypes.data_as(C.c_void_p), nh.ctypes.data_as(C.c_void_p), RefineSteps,
+                                      out.ctypes.data_as(C.c_void_p), rel.ctypes.data_as(C.c_void_p),
+                                      info.ctypes.data_as(C.c_void_p), ms.ctypes.data_as(C.c_void_p)))
+    if Timing is not None:
+        Timing.update(zip(("setup_ms", "capture_ms", "instantiate_ms", "run_ms", "results_ms"), ms.tolist()))
+    return out, rel, (int(info[0]), int(info[1]))
 
 
 PROFILED_KERNELS = ("gemm_c128", "gemm_c64", "panel_c128", "panel_c64", "trsm_c128", "trsm_c64",

@@ -63,3 +63,44 @@ def test_sweep_arguments(sx, O):
     with pytest.raises(sx.SCUFFXError):
         sx.BatchSweep(G, [1.0], sx.PlaneWave(*PW), prec=16)
     assert sx.BatchSweep(G, [], sx.PlaneWave(*PW)).shape == (0, 3)
+
+
+def test_batched_sweep_config3(sx, O):
+    """The batch of 64 instances: 32 log-spaced frequencies in [0.05, 5]
+    on two Drude spheres of uneven order (level 2, N=960; level 3,
+    N=3840), swept by one captured graph per call (batched complex64 LU,
+    complex128 refinement).  Every instance's power equals the
+    step-by-step complex128 chain (AssembleBEMMatrix, LUFactorize, LUSolve,
+    GetPower) within 1e-10 of its largest component."""
+    om = 0.05 * 100.0 ** (np.arange(32) / 31.0)
+    Gs = []
+    for L in (2, 3):
+        V, T = O.icosphere(L)
+        G = sx.RWGGeometry()
+        G.AddMesh(V, T, DRUDE)
+        Gs.append(G)
+    assert [G.TotalBFs for G in Gs] == [960, 3840]
+    print("batched sweep: 2 Drude spheres x 32 frequencies", flush=True)
+    t0 = time.perf_counter()
+    P, rel, info = sx.SweepBatched(Gs, om, sx.PlaneWave(*PW))
+    print(f"  {1e3 * (time.perf_counter() - t0):.1f} ms, solved again {info[0]}, LU block width {info[1]}, "
+          f"max residual {rel.max():.2e}", flush=True)
+    worst = 0.0
+    for g, G in enumerate(Gs):
+        for k, w in enumerate(om):
+            ref = chain(sx, G, w)
+            d = np.abs(P[g, k] - ref).max() / np.abs(ref).max()
+            worst = max(worst, d)
+            assert d <= 1e-10, (g, w, P[g, k], ref)
+    print(f"  worst power difference vs the step-by-step chain {worst:.2e}", flush=True)
+    assert rel.max() <= 1e-11
+
+
+def test_batched_sweep_rejects_short_wavelength(sx, O):
+    """a frequency whose near pairs need the short-wavelength kernels is
+    reported, not captured"""
+    V, T = O.icosphere(1)
+    G = sx.RWGGeometry()
+    G.AddMesh(V, T, "CONST_EPS_100")
+    with pytest.raises(sx.SCUFFXError):
+        sx.SweepBatched([G], [1.0], sx.PlaneWave(*PW))

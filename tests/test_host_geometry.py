@@ -214,3 +214,83 @@ def test_panel_normals_point_into_exterior(sx, O, mesh):
             Vg, Tg = (V, T) if path is None else read_msh(path)
             n = np.cross(Vg[Tg[:, 1]] - Vg[Tg[:, 0]], Vg[Tg[:, 2]] - Vg[Tg[:, 0]])
             assert np.all(np.einsum("ij,ij->i", n, P[:, 3:6]) > 0)
+
+
+def test_scuffgeo_user_defined_material(sx, tmp_path):
+    """MATERIAL ... ENDMATERIAL sections (UserDefinedMaterials.cc:145-320):
+    constants, Eps(w) and Mu(w) expressions of w in rad/s; an OBJECT refers
+    to the material by name.  A Drude block equals the closed form, and
+    equals the built-in DRUDE material at the same plasma frequency."""
+    mesh = os.path.join(GOLDEN, "Sphere_327.msh")
+    geo = tmp_path / "gold.scuffgeo"
+    geo.write_text("MATERIAL Gold\n"
+                   "   wp = 1.37e16;    # plasma frequency, rad/s\n"
+                   "   gamma = 5.32e13;\n"
+                   "   Eps(w) = 1 - wp^2 / (w*(w + i*gamma));\n"
+                   "ENDMATERIAL\n\n"
+                   "MATERIAL Lossy\n  s = 2.5\n  Eps(w) = 3 + i*s*2.99792458e14/w\n  Mu(w) = 1.5\nENDMATERIAL\n\n"
+                   f"OBJECT A\n\tMESHFILE {mesh}\n\tMATERIAL Gold\nENDOBJECT\n"
+                   f"OBJECT B\n\tMESHFILE {mesh}\n\tDISPLACED 0 0 3\n\tMATERIAL Lossy\nENDOBJECT\n")
+    G = sx.RWGGeometry(str(geo))
+    c = 2.99792458e14
+    for om in (0.1, 1.0, 2.5, 1.0j):
+        w = om * c
+        e, m = G.GetEpsMu(om, Region=1)
+        ref = 1 - 1.37e16 ** 2 / (w * (w + 1j * 5.32e13))
+        assert abs(e - ref) <= 1e-13 * abs(ref) and m == 1
+        e2, m2 = G.GetEpsMu(om, Region=2)
+        assert abs(e2 - (3 + 2.5j / om)) <= 1e-14 * abs(e2) and m2 == 1.5
+    assert G.GetEpsMu(1.0, Region=0) == (1, 1)
+    # the built-in Drude material in units of c / 1 um
+    geo2 = tmp_path / "drude.scuffgeo"
+    geo2.write_text(f"OBJECT A\n\tMESHFILE {mesh}\n\tMATERIAL DRUDE_{1.37e16 / c!r}_{5.32e13 / c!r}\nENDOBJECT\n")
+    G2 = sx.RWGGeometry(str(geo2))
+    for om in (0.1, 1.0):
+        e, _ = G.GetEpsMu(om, Region=1)
+        e2, _ = G2.GetEpsMu(om, Region=1)
+        assert abs(e - e2) <= 1e-12 * abs(e)
+    for bad in ("MATERIAL X\n  Eps(w) = 1 + foo\nENDMATERIAL\n", "MATERIAL X\n  a = 1 2\nENDMATERIAL\n",
+                "MATERIAL X\n  Eps(w) = (1 + w\nENDMATERIAL\n", "MATERIAL X\n  Eps(w) = 1\n"):
+        geo3 = tmp_path / "bad.scuffgeo"
+        geo3.write_text(bad + f"OBJECT A\n\tMESHFILE {mesh}\n\tMATERIAL X\nENDOBJECT\n")
+        with pytest.raises(sx.SCUFFXError):
+            sx.RWGGeometry(str(geo3))
+
+
+def test_set_vertices_rederives_panels_and_edges(sx, O):
+    """SetVertices keeps the topology and re-derives panel and edge data:
+    a uniform scaling by 2 doubles lengths and radii and quadruples areas;
+    a vertex moved and moved back restores everything bit for bit; the
+    result equals a fresh surface built from the new vertices (the oracle's
+    geometry on them, bit for bit); errors leave the mesh untouched."""
+    V, T = O.icosphere(2)
+    G = sx.RWGGeometry()
+    G.AddMesh(V, T, "CONST_EPS_4")
+    P0, E0 = G.Panels(0), G.Edges(0)
+    G.SetVertices(0, 2.0 * V)
+    P1, E1 = G.Panels(0), G.Edges(0)
+    assert np.array_equal(E1[0], E0[0])                    # topology
+    assert np.allclose(E1[1], 2.0 * E0[1], rtol=1e-15)     # lengths
+    assert np.allclose(P1[:, 7], 4.0 * P0[:, 7], rtol=1e-14)
+    assert np.allclose(P1[:, 3:6], P0[:, 3:6], atol=1e-15)  # normals keep their orientation
+    W = V.copy()
+    W[5] += np.array([0.01, -0.02, 0.03])
+    G.SetVertices(0, W)
+    Go = O.Geometry()
+    Go.add_surface(W, T, kind=1, eps=4.0)
+    assert np.array_equal(G.Panels(0), Go.panels(0))
+    assert np.array_equal(G.Vertices(0), W)
+    G.SetVertices(0, V)
+    assert np.array_equal(G.Panels(0), P0) and np.array_equal(G.Edges(0)[1], E0[1])
+    with pytest.raises(sx.SCUFFXError):
+        G.SetVertices(0, V[:-1])
+    D = V.copy()
+    D[T[0, 1]] = D[T[0, 0]]            # a degenerate panel
+    with pytest.raises(sx.SCUFFXError):
+        G.SetVertices(0, D)
+    assert np.array_equal(G.Panels(0), P0)
+    G.TransformSurface(0, D=[0.0, 0.0, 1.0])
+    with pytest.raises(sx.SCUFFXError):
+        G.SetVertices(0, V)
+    G.UnTransform()
+    G.SetVertices(0, V)
