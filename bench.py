@@ -32,6 +32,7 @@ def parse():
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--no-far32", action="store_true", help="skip the single-precision far-pair leg")
     ap.add_argument("--cpu-lu-n", type=int, default=0,
                     help="order of the CPU LAPACK sample (0: the full N)")
     ap.add_argument("--cpu-rows", type=int, default=2, help="oracle assembly rows per host core")
@@ -326,6 +327,29 @@ def main():
                     "rate": (v[1] / (v[0] * 1e-3)) if v[0] > 0 and v[1] > 0 else None}
                 for k, v in p.items() if v[2] > 0}
 
+    # the mixed form with single-precision far pairs, beside the headline
+    # (which keeps them in double: its power gate is 1e-8)
+    far32 = None
+    if not args.no_far32:
+        saved = {p: dict(d) for p, d in phases.items()}
+        G.SetFarPrecision(32)
+        step_mixed()
+        print("# far32 warmup done", file=sys.stderr, flush=True)
+        phases["mixed"].clear()
+        t32, (P32, it32, rel32) = timed(step_mixed, args.steps, "far32")
+        asm32 = phases["mixed"]["assemble"] / args.steps
+        far_ms32 = G.AssemblyStats()["far_ms"]
+        M32 = M.t.clone()
+        G.SetFarPrecision(64)
+        G.AssembleBEMMatrix(omega, M)
+        mdiff = float(torch.linalg.norm(M32 - M.t) / torch.linalg.norm(M.t))
+        del M32
+        phases = saved
+        far32 = {"s_per_freq": t32, "speedup_vs_fp64": t_fp64 / t32, "assemble_s": asm32, "far_ms": far_ms32,
+                 "far_ms_fp64": asm_stats["far_ms"], "matrix_rel_fro_vs_fp64": mdiff,
+                 "power_rel_err_vs_fp64": max(abs(P32[k] - Pd[k]) / abs(Pd[k]) for k in ("PAbs", "PExt", "PScat")),
+                 "refine_iters": it32, "refine_relres": rel32}
+
     sweep = None
     if not args.no_sweep:
         print("# batched sweep leg", file=sys.stderr, flush=True)
@@ -355,7 +379,7 @@ def main():
         "lu_tflops": {"c64": W_lu / (phases["mixed"]["lu_c64"] / args.steps) / 1e12,
                       "c128": W_lu / (phases["fp64"]["lu_c128"] / args.steps) / 1e12},
         "kernels_mixed": kern_summary(prof_mixed), "kernels_fp64": kern_summary(prof_fp64),
-        "roofline": roofline, "cpu_baseline": cpu, "sweep": sweep,
+        "roofline": roofline, "cpu_baseline": cpu, "sweep": sweep, "far32": far32,
     }
     print(json.dumps(line))
 

@@ -129,6 +129,12 @@ int scuffx_geom_surface_moved(const scuffx_geom *G, int ns);
    mate of another.  Identical coordinates are a no-op; an error if the
    geometry is transformed or a panel would degenerate. */
 int scuffx_geom_set_vertices(scuffx_geom *G, int ns, const double *V, int nV);
+/* precision of the far panel pairs in later assemblies of G: 64 (default,
+   the reference's double-precision GetPPIs_Cubature,
+   PanelPanelInteractions.cc:211-350) or 32 (the mixed form's option: the
+   same 36-point moments in single precision, ~1e-7 relative error per far
+   entry; near, self and singular terms stay double) */
+int scuffx_geom_set_far_precision(scuffx_geom *G, int bits);
 
 /* ---- adjoint contraction ----
    For a rigid motion of surface ns (displacement along x, y, z, then
@@ -156,7 +162,7 @@ int scuffx_vertex_slots(scuffx_geom *G);
    (device, leading dimension ldc).  M: the matrix assembled at this
    frequency (device, column-major, ldm), used for the edge-length terms.
    Far panel pairs are differentiated analytically with the assembly's own
-   rule, separated near pairs with the 78-point rule, touching and self
+   rule, separated near pairs with the 78-point rule on each panel quarter, touching and self
    pairs by central differences (step 1e-4 of the panel radius) of their
    Taylor-Duffy integrals; nothing of dM is stored.  Touching surfaces
    (panels of two surfaces sharing a vertex) are SCUFFX_ERR_UNSUPPORTED. */

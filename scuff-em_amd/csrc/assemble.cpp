@@ -473,7 +473,7 @@ int Assembler::far_and_stamp(NearBlock &B, const DevSurface &Da, const DevSurfac
     if (nreg > 0) {
       int tok = prof_begin(stream, PK_FAR);
       HIPCHK(launch_far(stream, Da.V, Da.PVI, Da.PC, F.rowLimit[k], Db.V, Db.PVI, Db.PC, F.d_slots + s0, ns, same,
-                        B.d_rowPtr, B.d_col, nreg, dreg, needH1, d_HP_, F.rowsCap));
+                        B.d_rowPtr, B.d_col, nreg, dreg, needH1, d_HP_, F.rowsCap, far_bits_ == 32));
       prof_end(stream, tok, wpair * (double)F.farPairs[k] * useful * nreg);
     }
     if (timed) hipEventRecord(evpool_[2 * k + 1], stream);

@@ -80,6 +80,10 @@ class Assembler {
   // r >= 0: later assemblies keep only the contributions of region r
   // (its terms of every common-region sum); -1: every region
   void set_region_filter(int r) { region_only_ = r; }
+  // 32: far panel pairs integrated in single precision by later
+  // assemblies (the mixed form's option); 64: double (default)
+  void set_far_precision(int bits) { far_bits_ = bits == 32 ? 32 : 64; }
+  int far_precision() const { return far_bits_; }
   const AsmStats &stats() const { return stats_; }
   // unique per Assembler object of the process (device pointers recorded
   // in a graph belong to one assembler)
@@ -123,6 +127,7 @@ class Assembler {
   std::string err_;
   std::vector<hipEvent_t> evpool_;   // timing events of the band loop
   int region_only_ = -1;
+  int far_bits_ = 64;
   unsigned long long serial_ = 0, alloc_gen_ = 0;
   RegionK *d_regtab_ = nullptr;      // [frequency][block][2] of a prepared sweep
   size_t cap_regtab_ = 0;

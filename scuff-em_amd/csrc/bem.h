@@ -72,10 +72,11 @@ struct RegionK {
 
 // launch wrappers (kern_*.hip); all asynchronous on `stream`
 // far records of one column band: rows pa < rowLimit x the band's slot panels
+// (fp32: the point sums in single precision, records still double)
 hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int rowLimit,
                       const double *Vb, const int *PVIb, const double *PCb, const int *slotPanels, int nslots,
                       int same, const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
-                      int needH1, double *HB, long rowsCap);
+                      int needH1, double *HB, long rowsCap, int fp32 = 0);
 hipError_t launch_near(hipStream_t stream, const NearPair *np, int nnear, const double *records,
                        int nreg, const RegionK *regs, int needH1, double *HN);
 hipError_t launch_self(hipStream_t stream, const double *V, const int *PVI, int nP, int nreg,

@@ -67,6 +67,7 @@ struct scuffx_geom {
   std::unique_ptr<Assembler> A;
   bool frozen = false;          // surfaces fixed once an assembler existed
   bool transformed = false;     // a surface moved since the last UnTransform
+  int far_bits = 64;            // precision of the far panel pairs (32 or 64)
   std::vector<DevRhs> rhs;
   double *d_out = nullptr;
   signed char *d_bft = nullptr;   // basis-function types for the EMT power
@@ -113,7 +114,10 @@ struct scuffx_geom {
   }
   Assembler *assembler()
   {
-    if (!A) A.reset(new Assembler(G));
+    if (!A) {
+      A.reset(new Assembler(G));
+      A->set_far_precision(far_bits);
+    }
     frozen = true;
     return A.get();
   }
@@ -336,6 +340,14 @@ int scuffx_geom_transform_surface(scuffx_geom *G, int ns, const double *R9, cons
   transform_surface(&G->G, ns, T);
   G->moved();
   G->transformed = true;
+  return 0;
+}
+
+int scuffx_geom_set_far_precision(scuffx_geom *G, int bits)
+{
+  if (!G || (bits != 32 && bits != 64)) return fail(SCUFFX_ERR_ARG, "precision must be 32 or 64");
+  G->far_bits = bits;
+  if (G->A) G->A->set_far_precision(bits);
   return 0;
 }
 
@@ -1256,6 +1268,7 @@ int scuffx_sweep_batched(scuffx_geom **Gs, int ngeom, const double *omegas, int 
     key.push_back(A->alloc_generation());
     key.push_back((unsigned long long)(uintptr_t)bufs[g].M);
     key.push_back((unsigned long long)nbsel[g]);
+    key.push_back((unsigned long long)A->far_precision());
     for (int v : A->nreg_table()) key.push_back((unsigned long long)v);
   }
   std::vector<double> fkey(E0, E0 + 6);

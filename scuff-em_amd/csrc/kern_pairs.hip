@@ -174,15 +174,126 @@ k_far(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   }
 }
 
+// ---------------------------------------------------------------------
+// K1 in single precision (the mixed form's optional far pairs): the same
+// records from the same 36-point moments, with the point arithmetic, the
+// kernel values and the sums in FP32 (panel-local coordinates, so the
+// lost digits are relative to the panel size and the centroid distance).
+// Each far entry then carries ~1e-7 relative error instead of ~1e-16.
+// ---------------------------------------------------------------------
+struct fcx {
+  float r, i;
+};
+__device__ inline fcx fmk(float r, float i) { fcx z; z.r = r; z.i = i; return z; }
+__device__ inline fcx operator+(fcx a, fcx b) { return fmk(a.r + b.r, a.i + b.i); }
+__device__ inline fcx operator-(fcx a, fcx b) { return fmk(a.r - b.r, a.i - b.i); }
+__device__ inline fcx operator*(fcx a, fcx b) { return fmk(a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r); }
+__device__ inline fcx operator*(float s, fcx a) { return fmk(s * a.r, s * a.i); }
+__device__ inline fcx &operator+=(fcx &a, fcx b) { a.r += b.r; a.i += b.i; return a; }
+
+struct Moments32 {
+  fcx S1, Sx[3], Sy[3], Sxy;
+  fcx s, u[3], v[3], w[3];
+};
+
+__device__ inline void pair_moments32(const float *xi, const float *eta, const float *D, fcx k, int needH1,
+                                      Moments32 &m)
+{
+#pragma clang fp contract(fast)
+  m.S1 = fmk(0, 0); m.Sxy = fmk(0, 0); m.s = fmk(0, 0);
+  for (int c = 0; c < 3; c++) { m.Sx[c] = fmk(0, 0); m.Sy[c] = fmk(0, 0); m.u[c] = fmk(0, 0); m.v[c] = fmk(0, 0); m.w[c] = fmk(0, 0); }
+  const fcx ik = fmk(-k.i, k.r);
+  for (int i = 0; i < 6; i++) {
+    const float wi = (float)c_tcr4[3 * i + 2];
+    const float *x = xi + 3 * i;
+    for (int j = 0; j < 6; j++) {
+      const float wj = (float)c_tcr4[3 * j + 2];
+      const float *y = eta + 3 * j;
+      const float R[3] = {D[0] + x[0] - y[0], D[1] + x[1] - y[1], D[2] + x[2] - y[2]};
+      const float r = sqrtf(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]), ir = 1.0f / r;
+      float sn, cs;
+      sincosf(k.r * r, &sn, &cs);
+      const float e = expf(-k.i * r) * (float)SCUFFX_INV4PI * ir * (wi * wj);
+      const fcx Phi = fmk(e * cs, e * sn);
+      m.S1 += Phi;
+      for (int c = 0; c < 3; c++) { m.Sx[c] += x[c] * Phi; m.Sy[c] += y[c] * Phi; }
+      m.Sxy += (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) * Phi;
+      if (needH1) {
+        const fcx Psi = (Phi * (ik - fmk(ir, 0))) * fmk(ir, 0);
+        const float xy[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
+        const float Rx[3] = {R[1] * x[2] - R[2] * x[1], R[2] * x[0] - R[0] * x[2], R[0] * x[1] - R[1] * x[0]};
+        const float yR[3] = {y[1] * R[2] - y[2] * R[1], y[2] * R[0] - y[0] * R[2], y[0] * R[1] - y[1] * R[0]};
+        m.s += (xy[0] * R[0] + xy[1] * R[1] + xy[2] * R[2]) * Psi;
+        for (int c = 0; c < 3; c++) { m.u[c] += Rx[c] * Psi; m.v[c] += yR[c] * Psi; m.w[c] += R[c] * Psi; }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_far32(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
+        const double *Vb, const int *PVIb, const double *PCb, const int *slotPanels, int same,
+        const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
+        int needH1, double *HB, long rowsCap)
+{
+  const int slot = blockIdx.y;
+  const int pa = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pa >= rowLimit) return;
+  const int pb = slotPanels[slot];
+  if (same && pb == pa) return;
+  if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) return;
+  const double *A0 = Va + 3 * PVIa[3 * pa], *A1 = Va + 3 * PVIa[3 * pa + 1], *A2 = Va + 3 * PVIa[3 * pa + 2];
+  const double *B0 = Vb + 3 * PVIb[3 * pb], *B1 = Vb + 3 * PVIb[3 * pb + 1], *B2 = Vb + 3 * PVIb[3 * pb + 2];
+  float xi[18], eta[18], D[3];
+  double Qa[9], Qb[9];
+  for (int c = 0; c < 3; c++) {
+    const double ca = PCa[4 * pa + c], cb = PCb[4 * pb + c];
+    const double a0 = A0[c] - ca, aA = A1[c] - A0[c], aB = A2[c] - A0[c];
+    const double b0 = B0[c] - cb, bA = B1[c] - B0[c], bB = B2[c] - B0[c];
+    for (int i = 0; i < 6; i++) {
+      xi[3 * i + c] = (float)(a0 + c_tcr4[3 * i] * aA + c_tcr4[3 * i + 1] * aB);
+      eta[3 * i + c] = (float)(b0 + c_tcr4[3 * i] * bA + c_tcr4[3 * i + 1] * bB);
+    }
+    Qa[c] = A0[c] - ca; Qa[3 + c] = A1[c] - ca; Qa[6 + c] = A2[c] - ca;
+    Qb[c] = B0[c] - cb; Qb[3 + c] = B1[c] - cb; Qb[6 + c] = B2[c] - cb;
+    D[c] = (float)(ca - cb);
+  }
+  const int stride = needH1 ? 4 : 2;
+  double *out = HB + ((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride);
+  for (int r = 0; r < nreg; r++) {
+    const cplx k = regs[r].k;
+    Moments32 m32;
+    pair_moments32(xi, eta, D, fmk((float)k.r, (float)k.i), needH1, m32);
+    // the 9 combinations from the single-precision moments
+    Moments m;
+    m.S1 = mk(m32.S1.r, m32.S1.i); m.Sxy = mk(m32.Sxy.r, m32.Sxy.i); m.s = mk(m32.s.r, m32.s.i);
+    for (int c = 0; c < 3; c++) {
+      m.Sx[c] = mk(m32.Sx[c].r, m32.Sx[c].i); m.Sy[c] = mk(m32.Sy[c].r, m32.Sy[c].i);
+      m.u[c] = mk(m32.u[c].r, m32.u[c].i); m.v[c] = mk(m32.v[c].r, m32.v[c].i); m.w[c] = mk(m32.w[c].r, m32.w[c].i);
+    }
+    cplx H0[9], H1[9];
+    combos_from_moments(m, Qa, Qb, k, needH1, H0, H1);
+    for (int q = 0; q < 9; q++) {
+      double *o = out + (r * 9 + q) * stride;
+      o[0] = H0[q].r; o[1] = H0[q].i;
+      if (needH1) { o[2] = H1[q].r; o[3] = H1[q].i; }
+    }
+  }
+}
+
 hipError_t launch_far(hipStream_t stream, const double *Va, const int *PVIa, const double *PCa, int rowLimit,
                       const double *Vb, const int *PVIb, const double *PCb, const int *slotPanels, int nslots,
                       int same, const int *nearRowPtr, const int *nearCol, int nreg, const RegionK *regs,
-                      int needH1, double *HB, long rowsCap)
+                      int needH1, double *HB, long rowsCap, int fp32)
 {
   if (rowLimit <= 0 || nslots <= 0 || nreg <= 0) return hipSuccess;
   dim3 grid((rowLimit + 255) / 256, nslots);
-  hipLaunchKernelGGL(k_far, grid, dim3(256), 0, stream, Va, PVIa, PCa, rowLimit, Vb, PVIb, PCb, slotPanels, same,
-                     nearRowPtr, nearCol, nreg, regs, needH1, HB, rowsCap);
+  if (fp32)
+    hipLaunchKernelGGL(k_far32, grid, dim3(256), 0, stream, Va, PVIa, PCa, rowLimit, Vb, PVIb, PCb, slotPanels, same,
+                       nearRowPtr, nearCol, nreg, regs, needH1, HB, rowsCap);
+  else
+    hipLaunchKernelGGL(k_far, grid, dim3(256), 0, stream, Va, PVIa, PCa, rowLimit, Vb, PVIb, PCb, slotPanels, same,
+                       nearRowPtr, nearCol, nreg, regs, needH1, HB, rowsCap);
   return hipGetLastError();
 }
 

@@ -18,7 +18,8 @@
 // evaluated point by point; no derivative block is stored.  A far pair
 // uses the degree-4 rule of the assembly (its derivative is the exact
 // derivative of the assembled value); a near pair without a common vertex
-// uses the 78-point rule; touching and self pairs come from central
+// uses the 78-point rule on each quarter of both panels; touching and self
+// pairs come from central
 // differences of their Taylor-Duffy integrals (kernels of kern_sw.hip and
 // kern_self.hip evaluated at displaced vertices).  The edge lengths enter
 // through the assembled matrix: sum_ij C_ij M_ij (d ln L_i + d ln L_j).
@@ -43,6 +44,9 @@ namespace scuffx {
 
 __constant__ double c_vg_tcr4[18] = SCUFFX_TCR4_INIT;
 __constant__ double c_vg_tcr20[234];
+// the 78-point rule on each quarter of the triangle (midpoint subdivision):
+// 312 points, barycentric (u, v) of the whole triangle, weights / 4
+__constant__ double c_vg_tcr20q[936];
 
 __device__ inline cplx ld2(const double2 *p, long i)
 {
@@ -276,7 +280,10 @@ k_vg_far(VGSurf A, VGSurf B, int same, const int *nearRowPtr, const int *nearCol
   }
 }
 
-// one wave per listed near pair, lanes over the 78 x 78 point pairs
+// one wave per listed near pair, lanes over the 312 x 312 point pairs of
+// the subdivided 78-point rule (the derivative's integrand falls off as
+// 1/r^3 at these distances: one 78-point rule per panel leaves ~1e-5 of
+// it for the closest pairs)
 __global__ void __launch_bounds__(64)
 k_vg_nearsep(VGSurf A, VGSurf B, const int *pairs, int npairs, int nreg, const RegionK *regs, int needH1, VGW w,
              double2 *grad)
@@ -293,7 +300,7 @@ k_vg_nearsep(VGSurf A, VGSurf B, const int *pairs, int npairs, int nreg, const R
   for (int r = 0; r < nreg; r++) {
     cplx W0[9], W1[9];
     pair_weights(A, p, B, q, w, regs[r], 1.0, W0, W1);
-    vg_pair(Pp, Pq, D, W0, W1, regs[r].k, needH1, c_vg_tcr20, 78, threadIdx.x, 64, G);
+    vg_pair(Pp, Pq, D, W0, W1, regs[r].k, needH1, c_vg_tcr20q, 312, threadIdx.x, 64, G);
   }
   double v[18];
   for (int kk = 0; kk < 3; kk++)
@@ -567,6 +574,21 @@ static hipError_t vg_tables()
   if (e != hipSuccess) return e;
   if (g_vg_ready[dev & 63]) return hipSuccess;
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_vg_tcr20), scuffx_tcr20, sizeof(scuffx_tcr20));
+  if (e != hipSuccess) return e;
+  // quarters: corners (a, b, c) in (u, v) of the whole triangle
+  static const double Q[4][3][2] = {{{0, 0}, {0.5, 0}, {0, 0.5}},
+                                    {{0.5, 0}, {1, 0}, {0.5, 0.5}},
+                                    {{0, 0.5}, {0.5, 0.5}, {0, 1}},
+                                    {{0.5, 0.5}, {0, 0.5}, {0.5, 0}}};
+  static double hq[936];
+  for (int t = 0; t < 4; t++)
+    for (int i = 0; i < 78; i++) {
+      const double u = scuffx_tcr20[3 * i], v = scuffx_tcr20[3 * i + 1], w = scuffx_tcr20[3 * i + 2];
+      double *o = hq + 3 * (78 * t + i);
+      for (int c = 0; c < 2; c++) o[c] = Q[t][0][c] + u * (Q[t][1][c] - Q[t][0][c]) + v * (Q[t][2][c] - Q[t][0][c]);
+      o[2] = 0.25 * w;
+    }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_vg_tcr20q), hq, sizeof(hq));
   if (e == hipSuccess) g_vg_ready[dev & 63] = true;
   return e;
 }

@@ -5,7 +5,7 @@
 // split as the assembly splits them (AssessPanelPair.cc:52-172; far when
 // the relative centroid distance exceeds 4):
 //   far pairs           k_vg_far, analytic, degree-4 rule (the assembly's)
-//   near, no common     k_vg_nearsep, analytic, 78-point rule
+//   near, no common     k_vg_nearsep, analytic, 78-point rule per quarter
 //   common edge/vertex  central differences of Taylor-Duffy integrals at
 //                       displaced vertices (k_sw), contracted by k_vg_fd
 //   a panel with itself central differences of k_self, likewise

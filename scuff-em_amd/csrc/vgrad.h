@@ -55,7 +55,7 @@ hipError_t launch_vg_far(hipStream_t s, const VGSurf &A, const VGSurf &B, int sa
                          const int *nearCol, int nreg, const RegionK *regs, int needH1, const VGW &w,
                          double2 *grad);
 // listed ordered near pairs without a common vertex (pairs[2 i], pairs[2 i + 1]),
-// 78-point x 78-point rule
+// 78-point rule on each quarter of both panels (312 x 312 point pairs)
 hipError_t launch_vg_nearsep(hipStream_t s, const VGSurf &A, const VGSurf &B, const int *pairs, int npairs,
                              int nreg, const RegionK *regs, int needH1, const VGW &w, double2 *grad);
 // central differences of touching / self pairs from the results of the

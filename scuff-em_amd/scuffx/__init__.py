@@ -77,6 +77,7 @@ def lib():
             "scuffx_adjoint": ([vp, C.c_int, C.c_double, C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
             "scuffx_geom_transform_surface": ([vp, C.c_int, vp, vp], C.c_int),
             "scuffx_geom_set_vertices": ([vp, C.c_int, vp, C.c_int], C.c_int),
+            "scuffx_geom_set_far_precision": ([vp, C.c_int], C.c_int),
             "scuffx_vertex_slots": ([vp], C.c_int),
             "scuffx_vertex_gradient": ([vp, C.c_double, C.c_double, vp, C.c_long, vp, vp, vp, C.c_long, vp],
                                        C.c_int),
@@ -708,6 +709,11 @@ class RWGGeometry:
         out = np.zeros((nv, 3))
         _check(lib().scuffx_geom_vertices(self.h, ns, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def SetFarPrecision(self, bits):
+        """64 (default) or 32: precision of the far panel pairs in later
+        assemblies (32: the mixed form's single-precision far pairs)"""
+        _check(lib().scuffx_geom_set_far_precision(self.h, int(bits)))
 
     def SetVertices(self, ns, V):
         """new vertex coordinates (NumVertices x 3) for surface ns, same
