@@ -342,9 +342,11 @@ int scuffx_trace_solve(const void *LU_dev, long n, long lda, const int *ipiv_dev
    256-row groups), which differs from row-by-row partial pivoting; the
    environment variable SCUFFX_MIXED_PIVOTING=partial selects row-by-row
    partial pivoting instead.  The block-row triangular solves use the
-   explicit inverses of the 32x32 unit-lower panel blocks, so the factors
-   carry slightly larger rounding than scuffx_lu_factor(prec=32).  Returns
-   the first zero pivot or 0. */
+   explicit inverses of the 32x32 unit-lower panel blocks, and the trailing
+   updates three real products per complex product (Gauss's 3M; the
+   environment variable SCUFFX_GEMM_3M=0 selects four), so the factors
+   carry larger rounding than scuffx_lu_factor(prec=32): they are the
+   refinement's preconditioner.  Returns the first zero pivot or 0. */
 int scuffx_lu32_from_c128(const void *M_dev, long n, long ldm, void *LU32_dev, long ld32,
                           int *ipiv_dev);
 /* x = M^{-1} b (complex128) from the complex64 factors: x0 = LU32^{-1} b,

@@ -14,8 +14,9 @@ namespace scuffx {
 // T = double2 (complex128), float2 (complex64) or double (real, the LHM_REAL
 // matrices of imaginary frequencies: dgetrf/dgetrs semantics).  fast = 1 solves the
 // block rows with the explicit inverses of the 32x32 unit-lower panel
-// blocks (products instead of substitution): same pivots, slightly larger
-// rounding; meant for preconditioner factorizations.
+// blocks (products instead of substitution), slightly larger rounding; for
+// complex64 it also pivots by tournament and updates with the 3M product
+// (see lu.hip); meant for preconditioner factorizations.
 template <typename T>
 int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream, int nb = 256,
               int fast = 0);

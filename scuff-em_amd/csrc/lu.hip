@@ -142,7 +142,11 @@ template <typename P> __device__ inline P *bshift(P *p, long bytes)
 #define GBN 64
 #define GBK 16
 
-template <typename T, bool ACC>
+// G3M (complex64 only): three real products per complex product instead of
+// four (Gauss / "3M": T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi),
+// Re = T1 - T2, Im = T3 - T1 - T2; normwise stable, Higham, SIAM J. Matrix
+// Anal. Appl. 13 (1992) 681-687), a quarter fewer MFMAs
+template <typename T, bool ACC, bool G3M = false>
 __global__ void __launch_bounds__(256)
 k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc, long bsa)
 {
@@ -156,8 +160,9 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const long row0 = (long)blockIdx.x * GBM, col0 = (long)blockIdx.y * GBN;
-  V4 accr[2][2], acci[2][2];
-  for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); }
+  V4 accr[2][2], acci[2][2], acc3[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) { accr[a][b] = CT<T>::zero(); acci[a][b] = CT<T>::zero(); acc3[a][b] = CT<T>::zero(); }
   // tiles of chunk k0 into registers: all eight global loads in flight
   // before any LDS store
   auto load = [&](long k0, T (&v)[4], T (&u)[4]) {
@@ -204,15 +209,27 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
         br[t] = sBr[kk][wc * 32 + t * 16 + (lane & 15)];
         bi[t] = sBi[kk][wc * 32 + t * 16 + (lane & 15)];
       }
-      for (int ti = 0; ti < 2; ti++)
-        for (int tj = 0; tj < 2; tj++) {
-          // acc = Re/Im of -(A B) when accumulating into C, +(A B) otherwise
-          const R sg = ACC ? (R)-1 : (R)1;
-          accr[ti][tj] = CT<T>::mfma(sg * ar[ti], br[tj], accr[ti][tj]);
-          accr[ti][tj] = CT<T>::mfma(-sg * ai[ti], bi[tj], accr[ti][tj]);
-          acci[ti][tj] = CT<T>::mfma(sg * ar[ti], bi[tj], acci[ti][tj]);
-          acci[ti][tj] = CT<T>::mfma(sg * ai[ti], br[tj], acci[ti][tj]);
-        }
+      if constexpr (G3M) {
+        // accr: T1, acci: T2, acc3: T3
+        R as[2], bs[2];
+        for (int t = 0; t < 2; t++) { as[t] = ar[t] + ai[t]; bs[t] = br[t] + bi[t]; }
+        for (int ti = 0; ti < 2; ti++)
+          for (int tj = 0; tj < 2; tj++) {
+            accr[ti][tj] = CT<T>::mfma(ar[ti], br[tj], accr[ti][tj]);
+            acci[ti][tj] = CT<T>::mfma(ai[ti], bi[tj], acci[ti][tj]);
+            acc3[ti][tj] = CT<T>::mfma(as[ti], bs[tj], acc3[ti][tj]);
+          }
+      } else {
+        for (int ti = 0; ti < 2; ti++)
+          for (int tj = 0; tj < 2; tj++) {
+            // acc = Re/Im of -(A B) when accumulating into C, +(A B) otherwise
+            const R sg = ACC ? (R)-1 : (R)1;
+            accr[ti][tj] = CT<T>::mfma(sg * ar[ti], br[tj], accr[ti][tj]);
+            accr[ti][tj] = CT<T>::mfma(-sg * ai[ti], bi[tj], accr[ti][tj]);
+            acci[ti][tj] = CT<T>::mfma(sg * ar[ti], bi[tj], acci[ti][tj]);
+            acci[ti][tj] = CT<T>::mfma(sg * ai[ti], br[tj], acci[ti][tj]);
+          }
+      }
     }
     __syncthreads();
   }
@@ -222,9 +239,15 @@ k_gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C,
         long gi = row0 + wr * 32 + ti * 16 + CT<T>::row(lane, r);
         long gj = col0 + wc * 32 + tj * 16 + (lane & 15);
         if (gi < m && gj < n) {
+          R re = accr[ti][tj][r], im = acci[ti][tj][r];
+          if constexpr (G3M) {
+            const R t1 = accr[ti][tj][r], t2 = acci[ti][tj][r];
+            re = ACC ? t2 - t1 : t1 - t2;
+            im = ACC ? (t1 + t2) - acc3[ti][tj][r] : acc3[ti][tj][r] - (t1 + t2);
+          }
           T c;
-          if (ACC) { c = C[gi + gj * ldc]; c.x += accr[ti][tj][r]; c.y += acci[ti][tj][r]; }
-          else { c.x = accr[ti][tj][r]; c.y = acci[ti][tj][r]; }
+          if (ACC) { c = C[gi + gj * ldc]; c.x += re; c.y += im; }
+          else { c.x = re; c.y = im; }
           C[gi + gj * ldc] = c;
         }
       }
@@ -307,7 +330,7 @@ static int gemm_kind(const double *) { return PK_GEMM_F64; }
 // workgroups share a CU
 template <typename T, bool ACC>
 static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
-                 hipStream_t s, unsigned lds_pad = 0, int nbat = 1, long bsa = 0)
+                 hipStream_t s, unsigned lds_pad = 0, int nbat = 1, long bsa = 0, bool g3m = false)
 {
   if (m <= 0 || n <= 0 || k <= 0 || nbat <= 0) return;
   int tok = prof_begin(s, gemm_kind(A));
@@ -317,16 +340,30 @@ static void gemm(long m, long n, long k, const T *A, long lda, const T *B, long 
     prof_end(s, tok, 2.0 * (double)m * (double)n * (double)k * nbat);
   } else {
     dim3 grid((unsigned)((m + GBM - 1) / GBM), (unsigned)((n + GBN - 1) / GBN), (unsigned)nbat);
-    hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc, bsa);
+    if (std::is_same<T, float2>::value && ACC && g3m)
+      hipLaunchKernelGGL((k_gemm<T, ACC, true>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc, bsa);
+    else
+      hipLaunchKernelGGL((k_gemm<T, ACC>), grid, dim3(256), lds_pad, s, m, n, k, A, lda, B, ldb, C, ldc, bsa);
     prof_end(s, tok, 8.0 * (double)m * (double)n * (double)k * nbat);
   }
 }
 
 template <typename T>
 static void gemm_sub(long m, long n, long k, const T *A, long lda, const T *B, long ldb, T *C, long ldc,
-                     hipStream_t s, unsigned lds_pad = 0, int nbat = 1, long bsa = 0)
+                     hipStream_t s, unsigned lds_pad = 0, int nbat = 1, long bsa = 0, bool g3m = false)
 {
-  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s, lds_pad, nbat, bsa);
+  gemm<T, true>(m, n, k, A, lda, B, ldb, C, ldc, s, lds_pad, nbat, bsa, g3m);
+}
+
+// The mixed solver's complex64 factorization (a preconditioner for the
+// complex128 refinement) updates with three real products per complex
+// product; SCUFFX_GEMM_3M=0, read at every factorization, keeps four (its
+// normwise error is 2-4x smaller, which matters only past condition
+// numbers of ~1e7, where complex64 factors barely precondition anyway)
+static bool mixed_gemm_3m()
+{
+  const char *e = getenv("SCUFFX_GEMM_3M");
+  return !(e && !strcmp(e, "0"));
 }
 
 // ---------------------------------------------------------------------
@@ -1512,6 +1549,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
   const bool partial_env = piv_env && !strcmp(piv_env, "partial");
   if (IsReal<T>::v) fast = 0;     // the explicit-inverse block rows are a complex64 option
   const bool tourn = fast && std::is_same<T, float2>::value && !partial_env;
+  const bool g3m = tourn && mixed_gemm_3m();
   const void *panel_fn = tourn ? (const void *)k_panel_ts : (const void *)k_panel<T>;
   int dev = 0, ncu = 0, perCU = 0;
   hipGetDevice(&dev);
@@ -1606,7 +1644,7 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       if (nr > 0) {
         if (!fast) trsm_small(A, lda, j0, w, A + j0 + (j0 + w) * lda, lda, nr, stream);
         gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
-                 A + (j0 + w) + (j0 + w) * lda, lda, stream);
+                 A + (j0 + w) + (j0 + w) * lda, lda, stream, 0, 1, 0, g3m);
       }
     }
     if (rc) break;
@@ -1624,7 +1662,8 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
           trsm_blk(A, lda, K, (int)jb, (const T *)S.linv + p0 * PW * PW, A + K + c1 * lda, lda, nnext, stream);
       } else
         trsm_block(A, lda, K, (int)jb, A + K + c1 * lda, lda, nnext, stream);
-      gemm_sub(n - c1, nnext, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, stream);
+      gemm_sub(n - c1, nnext, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, stream, 0, 1, 0,
+               g3m);
     }
     // the rest of the trailing matrix (side stream)
     if (c2 < n) {
@@ -1640,7 +1679,8 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
       // per wave, 12 KB) finds room on every CU instead of waiting for
       // GEMM workgroups to retire
       const unsigned pad = tourn ? 44u * 1024u - (unsigned)(4 * GBK * (GBM + 4) * sizeof(float)) : 0u;
-      gemm_sub(n - c1, n - c2, jb, A + c1 + K * lda, lda, A + K + c2 * lda, lda, A + c1 + c2 * lda, lda, side, pad);
+      gemm_sub(n - c1, n - c2, jb, A + c1 + K * lda, lda, A + K + c2 * lda, lda, A + c1 + c2 * lda, lda, side, pad, 1,
+               0, g3m);
       hipEventRecord(evR, side);
     }
   }
@@ -2576,6 +2616,7 @@ int lu_factor_batch(float2 *A, long n, long lda, int nbat, long sa, int *ipiv, l
   S.linv = (void *)(base + L.off_li);
   S.ts = (unsigned long long *)(base + L.off_ts);
   const float2 *linv = (const float2 *)S.linv;
+  const bool g3m = mixed_gemm_3m();
   BStride bs;
   bs.a = sa; bs.p = sp; bs.s = sS;
   int gmaxi = (int)L.Gmax;
@@ -2598,7 +2639,7 @@ int lu_factor_batch(float2 *A, long n, long lda, int nbat, long sa, int *ipiv, l
       const long nr = K + jb - (j0 + w);
       if (nr > 0)
         gemm_sub(n - j0 - w, nr, (long)w, A + (j0 + w) + j0 * lda, lda, A + j0 + (j0 + w) * lda, lda,
-                 A + (j0 + w) + (j0 + w) * lda, lda, s, 0, nbat, sa);
+                 A + (j0 + w) + (j0 + w) * lda, lda, s, 0, nbat, sa, g3m);
     }
     const long c1 = K + jb;
     if (c1 < n) {
@@ -2606,7 +2647,7 @@ int lu_factor_batch(float2 *A, long n, long lda, int nbat, long sa, int *ipiv, l
       laswp_plans(A, lda, c1, n, 0, 0, plans + p0 * PLAN_LEN, bplans, s, nbat, bs);
       trsm_blk(A, lda, K, (int)jb, linv + p0 * PW * PW, A + K + c1 * lda, lda, n - c1, s, nbat, bs);
       gemm_sub(n - c1, n - c1, jb, A + c1 + K * lda, lda, A + K + c1 * lda, lda, A + c1 + c1 * lda, lda, s, 0, nbat,
-               sa);
+               sa, g3m);
     }
   }
   // interchanges of each block on the columns left of it
