@@ -26,6 +26,9 @@ namespace scuffx {
 __constant__ double c_tcr4[18] = SCUFFX_TCR4_INIT;
 
 #define SCUFFX_INV4PI 0.079577471545947667884   // 1/(4 pi)
+// first word of a band record whose pair is near (its values live in the
+// near buffer): a NaN bit pattern no integral produces
+#define SCUFFX_NEAR_MARK 0x7FF4DEADBEEF0001ULL
 
 // binary search of sorted nearCol[lo..hi) for pb; index or -1
 __device__ inline int near_find(const int *rowPtr, const int *col, int pa, int pb)
@@ -141,7 +144,11 @@ k_far(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   if (pa >= rowLimit) return;
   const int pb = slotPanels[slot];
   if (same && pb == pa) return;
-  if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) return;
+  if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) {
+    const int stride = needH1 ? 4 : 2;
+    HB[((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride)] = __longlong_as_double((long long)SCUFFX_NEAR_MARK);
+    return;
+  }
   double ca[3] = {PCa[4 * pa], PCa[4 * pa + 1], PCa[4 * pa + 2]};
   double cb[3] = {PCb[4 * pb], PCb[4 * pb + 1], PCb[4 * pb + 2]};
   const double *A0 = Va + 3 * PVIa[3 * pa], *A1 = Va + 3 * PVIa[3 * pa + 1], *A2 = Va + 3 * PVIa[3 * pa + 2];
@@ -241,7 +248,11 @@ k_far32(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   if (pa >= rowLimit) return;
   const int pb = slotPanels[slot];
   if (same && pb == pa) return;
-  if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) return;
+  if (near_find(nearRowPtr, nearCol, pa, pb) >= 0) {
+    const int stride = needH1 ? 4 : 2;
+    HB[((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride)] = __longlong_as_double((long long)SCUFFX_NEAR_MARK);
+    return;
+  }
   const double *A0 = Va + 3 * PVIa[3 * pa], *A1 = Va + 3 * PVIa[3 * pa + 1], *A2 = Va + 3 * PVIa[3 * pa + 2];
   const double *B0 = Vb + 3 * PVIb[3 * pb], *B1 = Vb + 3 * PVIb[3 * pb + 1], *B2 = Vb + 3 * PVIb[3 * pb + 2];
   float xi[18], eta[18], D[3];
@@ -437,16 +448,18 @@ __device__ inline void fetch_h(int pa, int qa, int pb, int qb, int slot, int r, 
     *h1 = mk(0, 0);
     return;
   }
-  int ni = near_find(nearRowPtr, nearCol, pa, pb);
-  if (ni >= 0) {
-    int id = nearIdx[ni];
+  // the band record, unless k_far marked the pair near (its first word)
+  const double *band = HB + ((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride);
+  if ((unsigned long long)__double_as_longlong(band[0]) == SCUFFX_NEAR_MARK) {
+    int ni = near_find(nearRowPtr, nearCol, pa, pb);
+    int id = ni >= 0 ? nearIdx[ni] : -1;
     if (id < 0) {   // a near pair outside the upper-triangle walk: never requested
       *h0 = mk(0, 0); *h1 = mk(0, 0);
       return;
     }
     rec = HN + ((long long)id * nreg + r) * 9 * stride + (3 * qa + qb) * stride;
   } else {
-    rec = HB + ((long)slot * rowsCap + pa) * (long)(nreg * 9 * stride) + (r * 9 + 3 * qa + qb) * stride;
+    rec = band + (r * 9 + 3 * qa + qb) * stride;
   }
   *h0 = mk(rec[0], rec[1]);
   *h1 = stride == 4 ? mk(rec[2], rec[3]) : mk(0, 0);

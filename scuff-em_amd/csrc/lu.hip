@@ -1954,16 +1954,53 @@ k_trsv_inv(const T *LU, long lda, long n, long kb, int bs, const T *invb, T *Bin
 // Workgroup g owns the 64 rows of block g: it subtracts the off-diagonal
 // blocks' products with every earlier solution block as that block is
 // published (forward: blocks k < g of L; backward: k > g of U), then
-// applies the inverse diagonal block and publishes its own 64 values
-// followed by a flag.  Values and flags are agent-scope relaxed atomics,
-// the flag stored after the values' stores have completed; every wait is
-// bounded (error word, as in the panel).  Thread t handles row t % 64 and
-// the 16 columns 16 (t / 64) .. of a block; the four partial sums meet in
-// LDS.  The next block's 16 values per thread are loaded before the wait.
+// applies the inverse diagonal block and publishes its own 64 values.  The
+// output starts filled with a sentinel (a NaN bit pattern in every 64-bit
+// word that no solve produces), and every entry is written once, by an
+// agent-scope relaxed atomic store: a reader that sees a word other than
+// the sentinel sees its final value, so no flag or store drain is needed
+// (one memory round trip per block step).  Every wait is bounded (error
+// word, as in the panel).  Thread t handles row t % 64 and the 16 columns
+// 16 (t / 64) .. of a block; the four partial sums meet in LDS.  The next
+// block's 16 values per thread are loaded before the wait.  The forward
+// solve, which reads its right-hand side b once per row, leaves the
+// sentinel in b (mark), where the backward solve writes its output.
+#define SCUFFX_CHAIN_SENT 0x7FF4DEADBEEF0003ULL
+
+// a loaded value is final once none of its 64-bit words is the sentinel
+template <typename T>
+__device__ inline bool chain_val_set(const T &v)
+{
+  unsigned long long w[sizeof(T) / 8];
+  __builtin_memcpy(w, &v, sizeof(T));
+  bool set = true;
+#pragma unroll
+  for (int j = 0; j < (int)(sizeof(T) / 8); j++) set = set && (w[j] != SCUFFX_CHAIN_SENT);
+  return set;
+}
+
+template <typename T>
+__device__ inline void chain_mark(T *p)
+{
+  unsigned long long *w = (unsigned long long *)p;
+#pragma unroll
+  for (int j = 0; j < (int)(sizeof(T) / 8); j++) st_coh(w + j, SCUFFX_CHAIN_SENT);
+}
+
+template <typename T>
+__global__ void k_chain_fill(T *x, long n)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    unsigned long long *w = (unsigned long long *)(x + i);
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(T) / 8); j++) w[j] = SCUFFX_CHAIN_SENT;
+  }
+}
+
 template <typename T, bool LOWER>
 __global__ void __launch_bounds__(256)
-k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout, unsigned long long *flags,
-             unsigned *err, unsigned long long epoch)
+k_trsv_chain(const T *LU, long lda, long n, const T *inv, T *bin, T *xout, int mark, unsigned *err)
 {
   __shared__ T part[4][TB];
   __shared__ T bx[TB];
@@ -1973,7 +2010,10 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
   const long row = g * TB + r;
   const int bs = (int)std::min<long>(TB, n - g * TB);
   T b = zero_v<T>();
-  if (q == 0 && r < bs) b = bin[row];
+  if (q == 0 && r < bs) {
+    b = bin[row];
+    if (mark) chain_mark(bin + row);
+  }
   // blocks k in solve order before g
   const long nk = LOWER ? g : nblk - 1 - g;
   auto blk = [&](long i) { return LOWER ? i : nblk - 1 - i; };
@@ -1996,24 +2036,24 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
   else load_inv(a);
   for (long i = 0; i < nk; i++) {
     const long k = blk(i);
-    // wait for block k's values (wave 0 polls; bounded)
-    if (t < 64) {
-      unsigned spins = 0;
-      for (;;) {
-        const unsigned long long f = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__ballot(f == epoch) != 0) break;
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (++spins > GBAR_SPIN) { if (t == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __syncthreads();
+    // this thread's 16 values of block k, once published (bounded)
+    // (all 16 loads in flight at once; the whole set again while any
+    // still holds the sentinel)
     T x[16];
+    unsigned spins = 0;
+    for (;;) {
+      bool ready = true;
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const long col = k * TB + q * 16 + u;
-      if (col < n) x[u] = ld_coh(xout + col);
-      else { x[u] = zero_v<T>(); }
+      for (int u = 0; u < 16; u++) {
+        const long col = k * TB + q * 16 + u;
+        x[u] = col < n ? ld_coh(xout + col) : zero_v<T>();
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) ready = ready && chain_val_set(x[u]);
+      if (ready) break;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (++spins > GBAR_SPIN) { __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+      __builtin_amdgcn_s_sleep(1);
     }
     T acc = zero_v<T>();
 #pragma unroll
@@ -2047,9 +2087,6 @@ k_trsv_chain(const T *LU, long lda, long n, const T *inv, const T *bin, T *xout,
     xg = cadd(xg, part[1][r]); xg = cadd(xg, part[2][r]); xg = cadd(xg, part[3][r]);
     st_coh(xout + row, xg);
   }
-  drain();
-  __syncthreads();
-  if (t == 0) __hip_atomic_store(flags + g, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_zero_u64(unsigned long long *w, long n)
@@ -2077,20 +2114,18 @@ static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *inv
   char *w = words ? (char *)words : (char *)ws_get(WS_CHAIN, sizeof(unsigned long long) * (2 * nblk + 8));
   if (!w) return -1;
   unsigned *err = (unsigned *)w;
-  unsigned long long *fl = (unsigned long long *)(w + 64), *fu = fl + nblk;
-  // cleared by a kernel (not a memset node, so that a captured sequence
-  // clears it on every replay)
-  const long nwords = 2 * nblk + 8;
-  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, (unsigned long long *)w,
-                     nwords);
-  unsigned long long epoch = 1;
-  void *al[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invL, (void *)&B, (void *)&W, (void *)&fl,
-                (void *)&err, (void *)&epoch};
+  // cleared and filled by kernels (not memset nodes, so that a captured
+  // sequence does the same on every replay)
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)w, 8L);
+  hipLaunchKernelGGL(k_chain_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, W, n);
+  int mark = 1, nomark = 0;
+  void *al[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invL, (void *)&B, (void *)&W, (void *)&mark,
+                (void *)&err};
   if (hipLaunchCooperativeKernel((const void *)k_trsv_chain<T, true>, dim3((unsigned)nblk), dim3(256), al, 0, s) !=
       hipSuccess)
     return -3;
-  void *au[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invU, (void *)&W, (void *)&B, (void *)&fu,
-                (void *)&err, (void *)&epoch};
+  void *au[] = {(void *)&LU, (void *)&lda, (void *)&n, (void *)&invU, (void *)&W, (void *)&B, (void *)&nomark,
+                (void *)&err};
   if (hipLaunchCooperativeKernel((const void *)k_trsv_chain<T, false>, dim3((unsigned)nblk), dim3(256), au, 0, s) !=
       hipSuccess)
     return -3;
