@@ -756,7 +756,7 @@ __device__ inline unsigned wave_max_key(unsigned k)
 // j & 1 again only after every wave has passed step j + 1's barrier, i.e.
 // has read it).
 __device__ __attribute__((always_inline)) inline int ts_lu(float2 (&r)[PW], int nr, int w, int *spiv, int *srank,
-                                                           unsigned (*wk)[4], int (*wi)[4], float2 (*cand)[4][PW],
+                                                           unsigned long long (*wkv)[4], float2 (*cand)[4][PW],
                                                            int *zero)
 {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -774,19 +774,17 @@ __device__ __attribute__((always_inline)) inline int ts_lu(float2 (&r)[PW], int 
       if (lane == wl) {
 #pragma unroll
         for (int q = j; q < PW; q++) cand[j & 1][wave][q] = r[q];
-        wi[j & 1][wave] = t;
+        // the wave's key and row in one word: the largest word has the
+        // largest key and, among equal keys, the lowest row
+        wkv[j & 1][wave] = ((unsigned long long)wm << 32) | (unsigned)(255 - t);
       }
-      if (lane == 0) {
-        wk[j & 1][wave] = wm;
-        if (wl < 0) wi[j & 1][wave] = -1;
-      }
+      if (lane == 0 && wl < 0) wkv[j & 1][wave] = 0ull;
       __syncthreads();
-      unsigned b0 = wk[j & 1][0];
-      int p = wi[j & 1][0];
-      for (int q = 1; q < 4; q++) {
-        const unsigned ob = wk[j & 1][q];
-        if (ob > b0) { b0 = ob; p = wi[j & 1][q]; }
-      }
+      unsigned long long best = wkv[j & 1][0];
+#pragma unroll
+      for (int q = 1; q < 4; q++) best = max(best, wkv[j & 1][q]);
+      const unsigned b0 = (unsigned)(best >> 32);
+      const int p = 255 - (int)(unsigned)best;
       const int pw = p >> 6;
       if (t == p) { used = true; srank[t] = j; }
       if (t == 0) spiv[j] = p;
@@ -846,8 +844,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   __shared__ float2 sF[PW][PW + 1];     // final block, row-major
   __shared__ float2 cand[2][NR / 64][PW];
   __shared__ float2 sRd[PW];
-  __shared__ unsigned wk[2][NR / 64];
-  __shared__ int wi[2][NR / 64];
+  __shared__ unsigned long long wkv[2][NR / 64];
   __shared__ int spiv[PW], srank[NR], ssrc[NR], sfrom[NR];
   __shared__ long sidx[NR];
   // the panel is the critical path and the trailing GEMM shares its CUs:
@@ -955,7 +952,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         ssrc[t] = ref;
       }
     }
-    const int np = ts_lu(r, nr, w, spiv, srank, wk, wi, cand, &zero);
+    const int np = ts_lu(r, nr, w, spiv, srank, wkv, cand, &zero);
     if (l == nl) {
       // workgroup 0: the final block in pivot order
       const int k = t < nr ? srank[t] : -1;
