@@ -102,26 +102,82 @@ __device__ inline void pair_moments(const double *xi, const double *eta, const d
   }
 }
 
+// one (Qa, Qb) source-vertex combination of a pair's moments
+__device__ inline void combo_ab(const Moments &m, const double *qa, const double *qb, cplx c4, int needH1, cplx *h0,
+                                cplx *h1)
+{
+  cplx h = m.Sxy;
+  for (int c = 0; c < 3; c++) h = h - qb[c] * m.Sx[c] - qa[c] * m.Sy[c];
+  h += (mk(qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2], 0) + c4) * m.S1;
+  *h0 = h;
+  if (needH1) {
+    double qq[3];
+    cross3(qa, qb, qq);
+    cplx t = m.s;
+    for (int c = 0; c < 3; c++) t = t - qb[c] * m.u[c] - qa[c] * m.v[c] + qq[c] * m.w[c];
+    *h1 = t;
+  }
+}
+
 __device__ inline void combos_from_moments(const Moments &m, const double *Qa, const double *Qb,
                                            cplx k, int needH1, cplx *H0, cplx *H1)
 {
   cplx ik = IMUL(k);
   cplx c4 = 4.0 / (ik * ik);
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) {
-      const double *qa = Qa + 3 * a, *qb = Qb + 3 * b;
-      cplx h = m.Sxy;
-      for (int c = 0; c < 3; c++) h = h - qb[c] * m.Sx[c] - qa[c] * m.Sy[c];
-      h += (mk(qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2], 0) + c4) * m.S1;
-      H0[3 * a + b] = h;
+    for (int b = 0; b < 3; b++) combo_ab(m, Qa + 3 * a, Qb + 3 * b, c4, needH1, H0 + 3 * a + b, H1 + 3 * a + b);
+}
+
+// TCR4 point i of a panel given as p = {V0 - centroid, V1 - V0, V2 - V0}
+// (3 each), panel-local: p0 + s pA + u pB, rounded term by term
+__device__ inline void tcr4_point(const double *p, int i, double *x)
+{
+  const double s = c_tcr4[3 * i], u = c_tcr4[3 * i + 1];
+  for (int c = 0; c < 3; c++) x[c] = p[c] + s * p[3 + c] + u * p[6 + c];
+}
+
+// pair_moments<0> with the points formed on the fly and the 6 x 6 point
+// loop kept rolled: the same sums in the same order, in about half the
+// registers of the unrolled form (two waves per SIMD instead of one)
+__device__ inline void pair_moments_far(const double *pa, const double *pb, const double *D, cplx k, int needH1,
+                                        Moments &m)
+{
+#pragma clang fp contract(fast)
+  m.S1 = mk(0, 0); m.Sxy = mk(0, 0); m.s = mk(0, 0);
+  for (int c = 0; c < 3; c++) { m.Sx[c] = mk(0, 0); m.Sy[c] = mk(0, 0); m.u[c] = mk(0, 0); m.v[c] = mk(0, 0); m.w[c] = mk(0, 0); }
+  cplx ik = IMUL(k);
+#pragma unroll 1
+  for (int i = 0; i < 6; i++) {
+    const double wi = c_tcr4[3 * i + 2];
+    double x[3];
+    tcr4_point(pa, i, x);
+#pragma unroll 1
+    for (int j = 0; j < 6; j++) {
+      const double wj = c_tcr4[3 * j + 2];
+      double y[3];
+      tcr4_point(pb, j, y);
+      double R[3] = {D[0] + x[0] - y[0], D[1] + x[1] - y[1], D[2] + x[2] - y[2]};
+      double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      double r = sqrt(r2);
+      double ir = 1.0 / r;
+      double e = exp(-k.i * r), sn, cs;
+      sincos(k.r * r, &sn, &cs);
+      cplx Phi = mk(e * cs, e * sn) * (SCUFFX_INV4PI * ir);
+      Phi = Phi * (wi * wj);
+      m.S1 += Phi;
+      for (int c = 0; c < 3; c++) { m.Sx[c] += x[c] * Phi; m.Sy[c] += y[c] * Phi; }
+      m.Sxy += (x[0] * y[0] + x[1] * y[1] + x[2] * y[2]) * Phi;
       if (needH1) {
-        double qq[3];
-        cross3(qa, qb, qq);
-        cplx t = m.s;
-        for (int c = 0; c < 3; c++) t = t - qb[c] * m.u[c] - qa[c] * m.v[c] + qq[c] * m.w[c];
-        H1[3 * a + b] = t;
+        cplx Psi = Phi * (ik - mk(ir, 0)) * ir;
+        double xy[3], Rx[3], yR[3];
+        cross3(x, y, xy);
+        cross3(R, x, Rx);
+        cross3(y, R, yR);
+        m.s += (xy[0] * R[0] + xy[1] * R[1] + xy[2] * R[2]) * Psi;
+        for (int c = 0; c < 3; c++) { m.u[c] += Rx[c] * Psi; m.v[c] += yR[c] * Psi; m.w[c] += R[c] * Psi; }
       }
     }
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -153,16 +209,11 @@ k_far(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   double cb[3] = {PCb[4 * pb], PCb[4 * pb + 1], PCb[4 * pb + 2]};
   const double *A0 = Va + 3 * PVIa[3 * pa], *A1 = Va + 3 * PVIa[3 * pa + 1], *A2 = Va + 3 * PVIa[3 * pa + 2];
   const double *B0 = Vb + 3 * PVIb[3 * pb], *B1 = Vb + 3 * PVIb[3 * pb + 1], *B2 = Vb + 3 * PVIb[3 * pb + 2];
-  double xi[18], eta[18], Qa[9], Qb[9], D[3];
+  // panels as {V0 - centroid, V1 - V0, V2 - V0}; D = centroid difference
+  double PA[9], PB[9], D[3];
   for (int c = 0; c < 3; c++) {
-    double a0 = A0[c] - ca[c], aA = A1[c] - A0[c], aB = A2[c] - A0[c];
-    double b0 = B0[c] - cb[c], bA = B1[c] - B0[c], bB = B2[c] - B0[c];
-    for (int i = 0; i < 6; i++) {
-      xi[3 * i + c] = a0 + c_tcr4[3 * i] * aA + c_tcr4[3 * i + 1] * aB;
-      eta[3 * i + c] = b0 + c_tcr4[3 * i] * bA + c_tcr4[3 * i + 1] * bB;
-    }
-    Qa[c] = A0[c] - ca[c]; Qa[3 + c] = A1[c] - ca[c]; Qa[6 + c] = A2[c] - ca[c];
-    Qb[c] = B0[c] - cb[c]; Qb[3 + c] = B1[c] - cb[c]; Qb[6 + c] = B2[c] - cb[c];
+    PA[c] = A0[c] - ca[c]; PA[3 + c] = A1[c] - A0[c]; PA[6 + c] = A2[c] - A0[c];
+    PB[c] = B0[c] - cb[c]; PB[3 + c] = B1[c] - B0[c]; PB[6 + c] = B2[c] - B0[c];
     D[c] = ca[c] - cb[c];
   }
   const int stride = needH1 ? 4 : 2;
@@ -170,13 +221,21 @@ k_far(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   for (int r = 0; r < nreg; r++) {
     Moments m;
     cplx k = regs[r].k;
-    pair_moments<0>(xi, eta, D, k, needH1, m);
-    cplx H0[9], H1[9];
-    combos_from_moments(m, Qa, Qb, k, needH1, H0, H1);
+    pair_moments_far(PA, PB, D, k, needH1, m);
+    // source vertices relative to the centroids
+    double Qa[9], Qb[9];
+    for (int c = 0; c < 3; c++) {
+      Qa[c] = A0[c] - ca[c]; Qa[3 + c] = A1[c] - ca[c]; Qa[6 + c] = A2[c] - ca[c];
+      Qb[c] = B0[c] - cb[c]; Qb[3 + c] = B1[c] - cb[c]; Qb[6 + c] = B2[c] - cb[c];
+    }
+    cplx ik = IMUL(k);
+    cplx c4 = 4.0 / (ik * ik);
     for (int q = 0; q < 9; q++) {
+      cplx h0, h1;
+      combo_ab(m, Qa + 3 * (q / 3), Qb + 3 * (q % 3), c4, needH1, &h0, &h1);
       double *o = out + (r * 9 + q) * stride;
-      o[0] = H0[q].r; o[1] = H0[q].i;
-      if (needH1) { o[2] = H1[q].r; o[3] = H1[q].i; }
+      o[0] = h0.r; o[1] = h0.i;
+      if (needH1) { o[2] = h1.r; o[3] = h1.i; }
     }
   }
 }
@@ -203,24 +262,36 @@ struct Moments32 {
   fcx s, u[3], v[3], w[3];
 };
 
-__device__ inline void pair_moments32(const float *xi, const float *eta, const float *D, fcx k, int needH1,
+// the far-pair moments in FP32 from panels given as in pair_moments_far:
+// each point formed in FP64, as before the cast, inside a rolled 6 x 6
+// point loop
+template <bool REALK>
+__device__ inline void pair_moments32(const double *pa, const double *pb, const float *D, fcx k, int needH1,
                                       Moments32 &m)
 {
 #pragma clang fp contract(fast)
   m.S1 = fmk(0, 0); m.Sxy = fmk(0, 0); m.s = fmk(0, 0);
   for (int c = 0; c < 3; c++) { m.Sx[c] = fmk(0, 0); m.Sy[c] = fmk(0, 0); m.u[c] = fmk(0, 0); m.v[c] = fmk(0, 0); m.w[c] = fmk(0, 0); }
   const fcx ik = fmk(-k.i, k.r);
+#pragma unroll 1
   for (int i = 0; i < 6; i++) {
     const float wi = (float)c_tcr4[3 * i + 2];
-    const float *x = xi + 3 * i;
+    double xd[3];
+    tcr4_point(pa, i, xd);
+    const float x[3] = {(float)xd[0], (float)xd[1], (float)xd[2]};
+#pragma unroll 1
     for (int j = 0; j < 6; j++) {
       const float wj = (float)c_tcr4[3 * j + 2];
-      const float *y = eta + 3 * j;
+      double yd[3];
+      tcr4_point(pb, j, yd);
+      const float y[3] = {(float)yd[0], (float)yd[1], (float)yd[2]};
       const float R[3] = {D[0] + x[0] - y[0], D[1] + x[1] - y[1], D[2] + x[2] - y[2]};
       const float r = sqrtf(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]), ir = 1.0f / r;
       float sn, cs;
       sincosf(k.r * r, &sn, &cs);
-      const float e = expf(-k.i * r) * (float)SCUFFX_INV4PI * ir * (wi * wj);
+      // (REALK: a real wavenumber, as in a lossless region at real
+      // frequency, needs no exponential: expf(-0) is 1 exactly)
+      const float e = (REALK ? 1.0f : expf(-k.i * r)) * (float)SCUFFX_INV4PI * ir * (wi * wj);
       const fcx Phi = fmk(e * cs, e * sn);
       m.S1 += Phi;
       for (int c = 0; c < 3; c++) { m.Sx[c] += x[c] * Phi; m.Sy[c] += y[c] * Phi; }
@@ -255,18 +326,12 @@ k_far32(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   }
   const double *A0 = Va + 3 * PVIa[3 * pa], *A1 = Va + 3 * PVIa[3 * pa + 1], *A2 = Va + 3 * PVIa[3 * pa + 2];
   const double *B0 = Vb + 3 * PVIb[3 * pb], *B1 = Vb + 3 * PVIb[3 * pb + 1], *B2 = Vb + 3 * PVIb[3 * pb + 2];
-  float xi[18], eta[18], D[3];
-  double Qa[9], Qb[9];
+  double PA[9], PB[9];
+  float D[3];
   for (int c = 0; c < 3; c++) {
     const double ca = PCa[4 * pa + c], cb = PCb[4 * pb + c];
-    const double a0 = A0[c] - ca, aA = A1[c] - A0[c], aB = A2[c] - A0[c];
-    const double b0 = B0[c] - cb, bA = B1[c] - B0[c], bB = B2[c] - B0[c];
-    for (int i = 0; i < 6; i++) {
-      xi[3 * i + c] = (float)(a0 + c_tcr4[3 * i] * aA + c_tcr4[3 * i + 1] * aB);
-      eta[3 * i + c] = (float)(b0 + c_tcr4[3 * i] * bA + c_tcr4[3 * i + 1] * bB);
-    }
-    Qa[c] = A0[c] - ca; Qa[3 + c] = A1[c] - ca; Qa[6 + c] = A2[c] - ca;
-    Qb[c] = B0[c] - cb; Qb[3 + c] = B1[c] - cb; Qb[6 + c] = B2[c] - cb;
+    PA[c] = A0[c] - ca; PA[3 + c] = A1[c] - A0[c]; PA[6 + c] = A2[c] - A0[c];
+    PB[c] = B0[c] - cb; PB[3 + c] = B1[c] - B0[c]; PB[6 + c] = B2[c] - B0[c];
     D[c] = (float)(ca - cb);
   }
   const int stride = needH1 ? 4 : 2;
@@ -274,7 +339,9 @@ k_far32(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
   for (int r = 0; r < nreg; r++) {
     const cplx k = regs[r].k;
     Moments32 m32;
-    pair_moments32(xi, eta, D, fmk((float)k.r, (float)k.i), needH1, m32);
+    const fcx k32 = fmk((float)k.r, (float)k.i);
+    if (k32.i == 0) pair_moments32<true>(PA, PB, D, k32, needH1, m32);
+    else pair_moments32<false>(PA, PB, D, k32, needH1, m32);
     // the 9 combinations from the single-precision moments
     Moments m;
     m.S1 = mk(m32.S1.r, m32.S1.i); m.Sxy = mk(m32.Sxy.r, m32.Sxy.i); m.s = mk(m32.s.r, m32.s.i);
@@ -282,12 +349,20 @@ k_far32(const double *Va, const int *PVIa, const double *PCa, int rowLimit,
       m.Sx[c] = mk(m32.Sx[c].r, m32.Sx[c].i); m.Sy[c] = mk(m32.Sy[c].r, m32.Sy[c].i);
       m.u[c] = mk(m32.u[c].r, m32.u[c].i); m.v[c] = mk(m32.v[c].r, m32.v[c].i); m.w[c] = mk(m32.w[c].r, m32.w[c].i);
     }
-    cplx H0[9], H1[9];
-    combos_from_moments(m, Qa, Qb, k, needH1, H0, H1);
+    double Qa[9], Qb[9];
+    for (int c = 0; c < 3; c++) {
+      const double ca = PCa[4 * pa + c], cb = PCb[4 * pb + c];
+      Qa[c] = A0[c] - ca; Qa[3 + c] = A1[c] - ca; Qa[6 + c] = A2[c] - ca;
+      Qb[c] = B0[c] - cb; Qb[3 + c] = B1[c] - cb; Qb[6 + c] = B2[c] - cb;
+    }
+    cplx ik = IMUL(k);
+    cplx c4 = 4.0 / (ik * ik);
     for (int q = 0; q < 9; q++) {
+      cplx h0, h1;
+      combo_ab(m, Qa + 3 * (q / 3), Qb + 3 * (q % 3), c4, needH1, &h0, &h1);
       double *o = out + (r * 9 + q) * stride;
-      o[0] = H0[q].r; o[1] = H0[q].i;
-      if (needH1) { o[2] = H1[q].r; o[3] = H1[q].i; }
+      o[0] = h0.r; o[1] = h0.i;
+      if (needH1) { o[2] = h1.r; o[3] = h1.i; }
     }
   }
 }
