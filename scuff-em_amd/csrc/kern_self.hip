@@ -18,6 +18,7 @@ namespace scuffx {
 #define TD_NWP 5
 #define TD_MAXM 9            // 2^(9+1)+1 = 1025 nodes reached by MaxEval=1000
 #define TD_NODES 1025
+#define TD_NPRE 64           // cache entries computed up front, one per lane
 
 __constant__ double c_ccx[512];            // nodes x_i, layout as the oracle
 __constant__ double c_ccw[TD_MAXM + 1 + (1 << (TD_MAXM + 1))];
@@ -126,26 +127,32 @@ __device__ void ct_upsilon(double M13, double M14, double M23, double M24, doubl
 
 struct CTJob { double A2, AdB, B2; CTPoly P[2]; cplx k; };
 
-// integrand at y: two complex values (P=1, P=PMCHWG1) as 4 doubles
+// integrand at y: two complex values (P=1, P=PMCHWG1) as 4 doubles.  The
+// job (J, in LDS) is shared by the wave; the kernel values K_n(X_d) =
+// e^{ikX} ExpRel_n(-ikX) / (n X) are formed where the sum uses them (the
+// same operations as tabulating them first, without an array indexed by
+// the run-time power range, which would live in scratch memory), and
+// e^{ikX_d} once for both polynomials.
 __device__ void ct_integrand(const CTJob &J, double y, double *f)
 {
   double X[3], u1, u2;
   u1 = 1.0; u2 = y;       X[0] = sqrt(J.A2 * u1 * u1 + 2.0 * J.AdB * u1 * u2 + J.B2 * u2 * u2);
   u1 = y;   u2 = (y - 1); X[1] = sqrt(J.A2 * u1 * u1 + 2.0 * J.AdB * u1 * u2 + J.B2 * u2 * u2);
   u1 = y;   u2 = 1.0;     X[2] = sqrt(J.A2 * u1 * u1 + 2.0 * J.AdB * u1 * u2 + J.B2 * u2 * u2);
-  cplx IK = IMUL(J.k);
+  const cplx IK = IMUL(J.k);
+  cplx IKX[3], E[3];
+  for (int d = 0; d < 3; d++) {
+    IKX[d] = IK * X[d];
+    E[d] = cexp_(IKX[d]);
+  }
   for (int pk = 0; pk < 2; pk++) {
     const CTPoly &P = J.P[pk];
-    cplx K[3][TD_NWP + 2];
-    for (int d = 0; d < 3; d++) {
-      cplx IKX = IK * X[d], e = cexp_(IKX);
-      for (int n = P.nMin + 1; n <= P.nMax + 1; n++)
-        K[d][n] = e * exprel_v3(n, -IKX) / (n * X[d]);
-    }
     cplx s = mk(0, 0);
     for (int n = P.nMin; n <= P.nMax; n++)
-      for (int d = 0; d < 3; d++)
-        s += (P.U[d][n][0] + y * P.U[d][n][1]) * K[d][n + 1];
+      for (int d = 0; d < 3; d++) {
+        const cplx K = E[d] * exprel_v3(n + 1, -IKX[d]) / ((n + 1) * X[d]);
+        s += (P.U[d][n][0] + y * P.U[d][n][1]) * K;
+      }
     s = s * (1.0 / (4.0 * M_PI));
     f[2 * pk] = s.r; f[2 * pk + 1] = s.i;
   }
@@ -184,17 +191,35 @@ k_self(const double *V, const int *PVI, int nP, int nreg, const RegionK *regs,
   const double *Q = V + 3 * PVI[3 * p + qa], *QP = V + 3 * PVI[3 * p + qb];
   double A[3], B[3], D[3], DP[3];
   sub3(V2, V1, A); sub3(V3, V2, B); sub3(V1, Q, D); sub3(V1, QP, DP);
-  CTJob J;
-  J.A2 = dot3(A, A); J.AdB = dot3(A, B); J.B2 = dot3(B, B);
-  J.k = regs[r].k;
-  ct_upsilon(0, 0, 0, 0, 0, 0, 0, 0, 1.0, J.P[0]);
-  ct_upsilon(dot3(A, A), dot3(A, B), dot3(B, A), dot3(B, B), dot3(A, DP), dot3(B, DP),
-             dot3(D, A), dot3(D, B), dot3(D, DP), J.P[1]);
+  // the job's constants in LDS (read by every lane at the same address)
+  __shared__ CTJob J;
+  if (lane == 0) {
+    J.A2 = dot3(A, A); J.AdB = dot3(A, B); J.B2 = dot3(B, B);
+    J.k = regs[r].k;
+    ct_upsilon(0, 0, 0, 0, 0, 0, 0, 0, 1.0, J.P[0]);
+    ct_upsilon(dot3(A, A), dot3(A, B), dot3(B, A), dot3(B, B), dot3(A, DP), dot3(B, DP),
+               dot3(D, A), dot3(D, B), dot3(D, DP), J.P[1]);
+  }
+  __syncthreads();
   double *vals = scratch + (long long)t * TD_NODES * 4;
   const double c = 0.5, rr = 0.5, Vol = 0.5;
-  if (lane < 3) {
-    const double y = lane == 0 ? c : lane == 1 ? c + rr * c_ccx[0] : c - rr * c_ccx[0];
-    ct_integrand(J, y, vals + 4 * lane);
+  // The first TD_NPRE cache entries (centre, the pair at x_0, then
+  // 2^(l-1) pairs per level l: rule orders 0 .. 4 and all but the last
+  // entry of order 5) are filled at once, one per lane, before the first
+  // convergence test: the lanes a low order leaves idle evaluate the next
+  // orders' points meanwhile.  The tests below read the same values in
+  // the same order as if each order's points were computed after the
+  // previous order's test; only the number of sequential passes shrinks.
+  for (int e = lane; e < TD_NPRE; e += 64) {
+    double y;
+    if (e == 0) y = c;
+    else if (e < 3) y = e == 1 ? c + rr * c_ccx[0] : c - rr * c_ccx[0];
+    else {
+      const int l = 31 - __clz(e - 1), k = e - (1 << l) - 1, nl = 1 << (l - 1);
+      const double x = c_ccx[nl + (k >> 1)];
+      y = (k & 1) ? c - rr * x : c + rr * x;
+    }
+    ct_integrand(J, y, vals + 4 * e);
   }
   __syncthreads();
   int m = 0, nstored = 3;
@@ -218,6 +243,7 @@ k_self(const double *V, const int *PVI, int nP, int nreg, const RegionK *regs,
     m += 1;
     const int nl = 1 << (m - 1);
     for (int k = lane; k < 2 * nl; k += 64) {
+      if (nstored + k < TD_NPRE) continue;   // computed up front
       const double x = c_ccx[nl + (k >> 1)];
       ct_integrand(J, (k & 1) ? c - rr * x : c + rr * x, vals + 4 * (nstored + k));
     }
