@@ -25,8 +25,9 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
 // the intermediate vector W (n), the inverses of the 64x64 diagonal blocks
 // of L and U (nblk * 64 * 64 each, nblk = ceil(n / 64)), the composed row
 // permutation (n ints) and the one-launch solve's words (2 nblk + 8).
-// inv_ready: invL / invU already hold the inverses for these factors (set
-// by the first solve; clear it for new factors).
+// inv_ready: invL / invU and perm already hold the inverses and the
+// composed row permutation for these factors (set by the first solve;
+// clear it for new factors).
 template <typename T>
 struct SolveWs {
   T *W = nullptr, *invL = nullptr, *invU = nullptr;

@@ -1828,10 +1828,11 @@ __global__ void __launch_bounds__(256) k_perm_apply(const T *in, long ldi, T *ou
 
 // applies the interchanges of ipiv to the nrhs columns of B (forward, as
 // laswp with increment 1) or their inverse (reverse order); tmp holds n x
-// nrhs elements
+// nrhs elements; perm_ready: perm_buf already holds the composed
+// permutation of ipiv
 template <typename T>
 static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s,
-                           int *perm_buf = nullptr);
+                           int *perm_buf = nullptr, bool perm_ready = false);
 
 // Triangular solves with few right-hand sides, in 64-row block steps.
 // The diagonal 64x64 blocks of L (unit lower) and U (upper) are inverted
@@ -2134,7 +2135,7 @@ static int trsv_chain(const T *LU, long lda, long n, const T *invL, const T *inv
 
 template <typename T>
 static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, bool fwd, T *tmp, hipStream_t s,
-                           int *perm_buf)
+                           int *perm_buf, bool perm_ready)
 {
   int *perm = n > PERM_LDS_MAX ? nullptr : perm_buf ? perm_buf : (int *)ws_get(WS_PERM, sizeof(int) * n);
   if (!perm || !tmp) {
@@ -2143,7 +2144,7 @@ static void apply_row_perm(T *B, long ldb, long n, long nrhs, const int *ipiv, b
     else hipLaunchKernelGGL(k_laswp_rev<T>, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, B, ldb, n, nrhs, ipiv);
     return;
   }
-  hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(64), 0, s, ipiv, n, perm, 0L, 0L);
+  if (!(perm_ready && perm == perm_buf)) hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(64), 0, s, ipiv, n, perm, 0L, 0L);
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)nrhs);
   if (fwd) hipLaunchKernelGGL((k_perm_apply<T, true>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n, 0L, 0L, 0L);
   else hipLaunchKernelGGL((k_perm_apply<T, false>), grid, dim3(256), 0, s, B, ldb, tmp, n, perm, n, 0L, 0L, 0L);
@@ -2186,12 +2187,15 @@ int lu_solve(const T *LU, long n, long lda, const int *d_ipiv, char trans, long 
     T *invL = ws ? ws->invL : (T *)ws_get(WS_SOLVE_INVL, sizeof(T) * nblk * TB * TB);
     T *invU = ws ? ws->invU : (T *)ws_get(WS_SOLVE_INVU, sizeof(T) * nblk * TB * TB);
     if (!W || !invL || !invU) return -2;
-    if (!ws || !ws->inv_ready) {
+    // a solve after the first with the same factors reuses the block
+    // inverses and the composed permutation
+    const bool cached = ws && ws->inv_ready;
+    if (!cached) {
       hipLaunchKernelGGL((k_diag_inv<T, true>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invL, 0L, 0L);
       hipLaunchKernelGGL((k_diag_inv<T, false>), dim3((unsigned)nblk), dim3(TB), 0, stream, LU, lda, n, invU, 0L, 0L);
       if (ws) ws->inv_ready = true;
     }
-    apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream, ws ? ws->perm : nullptr);
+    apply_row_perm(B, ldb, n, nrhs, d_ipiv, true, W, stream, ws ? ws->perm : nullptr, cached);
     if (nrhs == 1) {
       int rc = trsv_chain(LU, lda, n, invL, invU, B, W, stream, ws ? ws->chain : nullptr);
       if (rc != -1) return rc;
