@@ -160,7 +160,10 @@ __device__ inline void pair_moments_far(const double *pa, const double *pb, cons
       double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
       double r = sqrt(r2);
       double ir = 1.0 / r;
-      double e = exp(-k.i * r), sn, cs;
+      // (a real wavenumber, as in a lossless region at real frequency,
+      // skips the exponential: exp(-0) is 1 exactly)
+      double e = 1.0, sn, cs;
+      if (k.i != 0) e = exp(-k.i * r);
       sincos(k.r * r, &sn, &cs);
       cplx Phi = mk(e * cs, e * sn) * (SCUFFX_INV4PI * ir);
       Phi = Phi * (wi * wj);
