@@ -1419,8 +1419,10 @@ static void trsm_block_inv(const T *A, long lda, long r0, int nb, const T *linv,
 // 256-column block (jb <= 256) in one launch, with the panels' inverses:
 // per 32-row panel i, B_i <- Linv_i B_i, then the rows below minus
 // L(below, panel i) B_i (what trsm_block_inv does in 15 launches for the
-// next block's columns).  One wave per column; lane l holds rows l, l + 64,
-// l + 128, l + 192; the panel rows are read across lanes by lane reads.
+// next block's columns).  One workgroup per column: wave v holds rows
+// 64 v + lane; the wave that holds panel i's rows forms x_i = Linv_i B_i
+// by lane reads and passes it to the others through LDS, so the rows
+// below are updated by four waves at once.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb, long ncol, BStride bs)
@@ -1431,68 +1433,55 @@ k_trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B, long ldb,
     B += bi * bs.a;
     linv = bshift(linv, bi * bs.s);
   }
-  const int lane = threadIdx.x & 63;
-  const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ T xs[2][PW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long c = blockIdx.x;
   if (c >= ncol) return;
-  T b[4];
-#pragma unroll
-  for (int s = 0; s < 4; s++) {
-    b[s] = zero_v<T>();
-    if (lane + 64 * s < jb) b[s] = B[lane + 64 * s + c * ldb];
-  }
+  const int row = lane + 64 * wv;
+  T b = zero_v<T>();
+  if (row < jb) b = B[row + c * ldb];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const int i0 = PW * i;
     if (i0 < jb) {
       const int w = min(PW, jb - i0), si = i / 2, l0 = PW * (i & 1);
-      // x = Linv_i b_i on the lanes holding the panel's rows
-      const T *Li = linv + (long)i * PW * PW;
-      const bool mine = lane >= l0 && lane < l0 + w;
-      // this lane's row of Linv_i, all loads in flight before the products
-      T lr[PW];
-      const int q = mine ? lane - l0 : 0;
+      if (wv == si) {
+        // x = Linv_i b_i on the lanes holding the panel's rows
+        const T *Li = linv + (long)i * PW * PW;
+        const bool mine = lane >= l0 && lane < l0 + w;
+        // this lane's row of Linv_i, all loads in flight before the products
+        T lr[PW];
+        const int q = mine ? lane - l0 : 0;
 #pragma unroll
-      for (int k = 0; k < PW; k++) lr[k] = Li[q + k * PW];
-      T acc = zero_v<T>();
-#pragma unroll
-      for (int k = 0; k < PW; k++) {
-        if (k < w) {
-          T bk;
-          bk.x = readlane_f(b[si].x, l0 + k);
-          bk.y = readlane_f(b[si].y, l0 + k);
-          acc = cadd(acc, cmul(lr[k], bk));
-        }
-      }
-      if (mine) b[si] = acc;
-      // rows below the panel: minus L(rows, panel) x
-      if (i0 + w < jb) {
-        T x[PW];
+        for (int k = 0; k < PW; k++) lr[k] = Li[q + k * PW];
+        T acc = zero_v<T>();
 #pragma unroll
         for (int k = 0; k < PW; k++) {
-          x[k].x = readlane_f(b[si].x, l0 + (k < w ? k : 0));
-          x[k].y = readlane_f(b[si].y, l0 + (k < w ? k : 0));
-        }
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-          const int row = lane + 64 * s;
-          if (row >= i0 + w && row < jb) {
-            const T *Lr = A + (r0 + row) + (r0 + i0) * lda;
-            T lv[PW];
-#pragma unroll
-            for (int k = 0; k < PW; k++) lv[k] = Lr[(k < w ? k : 0) * lda];
-            T a = b[s];
-#pragma unroll
-            for (int k = 0; k < PW; k++)
-              if (k < w) a = csub(a, cmul(lv[k], x[k]));
-            b[s] = a;
+          if (k < w) {
+            T bk;
+            bk.x = readlane_f(b.x, l0 + k);
+            bk.y = readlane_f(b.y, l0 + k);
+            acc = cadd(acc, cmul(lr[k], bk));
           }
         }
+        if (mine) { b = acc; xs[i & 1][lane - l0] = acc; }
+      }
+      __syncthreads();
+      // rows below the panel: minus L(rows, panel) x
+      if (i0 + w < jb && row >= i0 + w && row < jb) {
+        const T *Lr = A + (r0 + row) + (r0 + i0) * lda;
+        T lv[PW];
+#pragma unroll
+        for (int k = 0; k < PW; k++) lv[k] = Lr[(k < w ? k : 0) * lda];
+        T a = b;
+#pragma unroll
+        for (int k = 0; k < PW; k++)
+          if (k < w) a = csub(a, cmul(lv[k], xs[i & 1][k]));
+        b = a;
       }
     }
   }
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-    if (lane + 64 * s < jb) B[lane + 64 * s + c * ldb] = b[s];
+  if (row < jb) B[row + c * ldb] = b;
 }
 
 template <typename T>
@@ -1501,8 +1490,8 @@ static void trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B,
 {
   if (ncol <= 0 || jb <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
-  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)((ncol + 3) / 4), (unsigned)nbat), dim3(256), 0, s, A, lda, r0, jb,
-                     linv, B, ldb, ncol, bs);
+  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)ncol, (unsigned)nbat), dim3(256), 0, s, A, lda, r0, jb, linv, B, ldb,
+                     ncol, bs);
   prof_end(s, tok, 4.0 * (double)jb * (double)jb * (double)ncol);
 }
 
