@@ -900,19 +900,24 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   // own rows into registers (A keeps the original values until the last
   // phase, which re-reads them); workgroup 0 also publishes rows r0 .. r0 +
   // w - 1 (those the interchanges displace)
+  // (columns from w on hold zeros; their loads repeat column w - 1, so
+  // that the loads are not split by per-column branches)
   float2 r[PW];
 #pragma unroll
-  for (int q = 0; q < PW; q++) {
-    r[q] = zero_v<float2>();
-    if (t < nrow && q < w) r[q] = A[i + (r0 + q) * lda];
+  for (int q = 0; q < PW; q++) r[q] = zero_v<float2>();
+  if (t < nrow) {
+#pragma unroll
+    for (int q = 0; q < PW; q++) {
+      const float2 v = A[i + (r0 + min(q, w - 1)) * lda];
+      r[q] = q < w ? v : zero_v<float2>();
+    }
   }
   // every row's original values to the copy area (read by the merges and,
   // for the rows the interchanges displace from the diagonal block, by the
   // last phase); drained before this workgroup's first flag
   if (t < nrow) {
 #pragma unroll
-    for (int q = 0; q < PW; q++)
-      if (q < w) st_coh((float2 *)(copy + ((long)g * NR + t) * PW + q), r[q]);
+    for (int q = 0; q < PW; q++) st_coh((float2 *)(copy + ((long)g * NR + t) * PW + q), r[q]);
   }
   sidx[t] = lo + t;
   // stage 0: this workgroup's rows; stage l >= 1: workgroups g % TS_FAN^l
@@ -946,8 +951,7 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         const int ref = (int)ld_coh(cs + TS_REF + rr);
         const unsigned long long *os = copy + (long)ref * PW;   // the row's original values
 #pragma unroll
-        for (int q = 0; q < PW; q++)
-          if (q < w) r[q] = ld_coh((const float2 *)(os + q));
+        for (int q = 0; q < PW; q++) r[q] = ld_coh((const float2 *)(os + q));
         sidx[t] = (long)ld_coh(cs + TS_IDX + rr);
         ssrc[t] = ref;
       }
@@ -1066,14 +1070,12 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         float2 *Ai = A + i + r0 * lda;
         asm volatile("" : "+v"(Ai));
 #pragma unroll
-        for (int q = 0; q < PW; q++)
-          if (q < w) v[q] = Ai[q * lda];
+        for (int q = 0; q < PW; q++) v[q] = Ai[min(q, w - 1) * lda];
       } else if (c >= r0 && c < r0 + w) {
         // a row displaced from the diagonal block (no other source exists;
         // anything else is reported as a failed panel)
 #pragma unroll
-        for (int q = 0; q < PW; q++)
-          if (q < w) v[q] = ld_coh((const float2 *)(copy + (c - r0) * PW + q));
+        for (int q = 0; q < PW; q++) v[q] = ld_coh((const float2 *)(copy + (c - r0) * PW + q));
       } else {
         __hip_atomic_store(err, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
