@@ -872,9 +872,14 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
   const int nl = ts_levels(G);
   if (g == G) {
     ts_wait(root + TS_FIN, 0, 1, seq, err);
-    for (int e = t; e < PW * PW; e += NR) {
-      const int rr = e / PW, q = e % PW;
-      if (rr < w && q < w) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
+    // (all of this thread's loads in flight before the first LDS store)
+    float2 fv[PW * PW / NR];
+#pragma unroll
+    for (int u = 0; u < PW * PW / NR; u++) fv[u] = ld_coh((const float2 *)(root + t + u * NR));
+#pragma unroll
+    for (int u = 0; u < PW * PW / NR; u++) {
+      const int e = t + u * NR, rr = e / PW, q = e % PW;
+      if (rr < w && q < w) sF[rr][q] = fv[u];
     }
     __syncthreads();
     // the inverse of the unit-lower w x w block, column t by substitution
@@ -1029,25 +1034,48 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
     if (t == 0) st_coh(root + TS_FIN, seq);
   }
 
-  // every workgroup: its rows of the permuted panel
+  // every workgroup: its rows of the permuted panel.  The row's original
+  // values (the panel never writes them before this phase, and only this
+  // thread writes its row after it) are loaded before the root's flag is
+  // awaited; a row displaced from the diagonal block reloads below.
+  float2 v[PW];
+#pragma unroll
+  for (int q = 0; q < PW; q++) { v[q] = zero_v<float2>(); }
+  if (t < nrow && i >= r0 + w) {
+    // plain loads (opaque address, as above); columns from w on repeat
+    // column w - 1 and are never used
+    float2 *Ai = A + i + r0 * lda;
+    asm volatile("" : "+v"(Ai));
+#pragma unroll
+    for (int q = 0; q < PW; q++) v[q] = Ai[min(q, w - 1) * lda];
+  }
   ts_wait(root + TS_FIN, 0, 1, seq, err);
   // workgroup 0 (the diagonal block's rows) reads the whole final
-  // block, the others U11 from replica g % 8
-  for (int e = t; e < PW * PW; e += NR) {
-    const int rr = e / PW, q = e % PW;
-    if (rr < w && q < w) {
-      if (g == 0) sF[rr][q] = ld_coh((const float2 *)(root + rr * PW + q));
-      else if (q >= rr) sF[rr][q] = ld_coh((const float2 *)(root + TS_UREP + (g & 7) * TS_REP + rr * PW + q));
-    }
+  // block, the others U11 from replica g % 8; with the plan words, all
+  // loads in flight at once
+  float2 fv[PW * PW / NR];
+  {
+    const unsigned long long *fb = g == 0 ? root : root + TS_UREP + (g & 7) * TS_REP;
+#pragma unroll
+    for (int u = 0; u < PW * PW / NR; u++) fv[u] = ld_coh((const float2 *)(fb + t + u * NR));
+  }
+  int npl = 0;
+  unsigned long long pe = 0;
+  if (t < 2 * PW) {
+    npl = (int)ld_coh(root + TS_NPLAN);
+    pe = ld_coh(root + TS_PLAN + t);
+  }
+#pragma unroll
+  for (int u = 0; u < PW * PW / NR; u++) {
+    const int e = t + u * NR, rr = e / PW, q = e % PW;
+    if (rr < w && q < w && (g == 0 || q >= rr)) sF[rr][q] = fv[u];
   }
   // displaced rows of this workgroup: sfrom[row - lo] = original row
   sfrom[t] = -1;
   __syncthreads();
   if (t < 2 * PW) {
-    const int np = (int)ld_coh(root + TS_NPLAN);
-    const unsigned long long pe = ld_coh(root + TS_PLAN + t);
     const long dst = (long)(pe >> 32);
-    if (t < np && dst >= lo && dst < lo + nrow) sfrom[dst - lo] = (int)(pe & 0xFFFFFFFFull);
+    if (t < npl && dst >= lo && dst < lo + nrow) sfrom[dst - lo] = (int)(pe & 0xFFFFFFFFull);
   }
   __syncthreads();
   if (t < w) {
@@ -1062,15 +1090,8 @@ k_panel_ts(float2 *A, long lda, long r0, long m, int w, int *ipiv, int *info, Pa
         if (q < w) A[i + (r0 + q) * lda] = sF[i - r0][q];
     } else {
       const long c = sfrom[t] >= 0 ? sfrom[t] : i;
-      float2 v[PW];
-#pragma unroll
-      for (int q = 0; q < PW; q++) { v[q] = zero_v<float2>(); }
       if (c == i) {
-        // plain loads of the unchanged row (opaque address, as above)
-        float2 *Ai = A + i + r0 * lda;
-        asm volatile("" : "+v"(Ai));
-#pragma unroll
-        for (int q = 0; q < PW; q++) v[q] = Ai[min(q, w - 1) * lda];
+        // the values loaded above
       } else if (c >= r0 && c < r0 + w) {
         // a row displaced from the diagonal block (no other source exists;
         // anything else is reported as a failed panel)
