@@ -1692,12 +1692,21 @@ int lu_factor(T *A, long n, long lda, int *d_ipiv, int *info, hipStream_t stream
                0, g3m);
       hipEventRecord(evR, side);
     }
+    // this block's interchanges on the columns left of it.  Nothing reads
+    // those rows again during the factorization (the trailing updates read
+    // only the current block's L21), so with two streams they run on the
+    // side stream behind this block's update, in block order, instead of
+    // all at the end of the caller's stream
+    if (side != stream && K > 0) {
+      if (c2 >= n) hipStreamWaitEvent(side, evP, 0);   // no trailing update queued for this block
+      laswp_plans(A, lda, 0, K, 0, 0, plans + p0 * PLAN_LEN, bplans, side);
+    }
   }
-  // join the side stream, then the deferred interchanges on the columns
-  // left of each block
+  // join the side stream (one stream: the deferred interchanges on the
+  // columns left of each block, now)
   hipEventRecord(evR, side);
   hipStreamWaitEvent(stream, evR, 0);
-  if (rc == 0)
+  if (rc == 0 && side == stream)
     for (long K = nb; K < n; K += nb) {
       const long jb = std::min<long>(nb, n - K);
       laswp_plans(A, lda, 0, K, 0, 0, plans + (K / PW) * PLAN_LEN, (int)((jb + PW - 1) / PW), stream);
