@@ -563,13 +563,35 @@ k_stamp(int nEa, const int *EIa, const double *ELa, const int *bandCols, const i
   cplx m00 = mk(0, 0), m01 = mk(0, 0), m11 = mk(0, 0);
   for (int r = 0; r < nreg; r++) {
     cplx G[4], C[4];
-    for (int s = 0; s < 2; s++)
-      for (int t = 0; t < 2; t++) {
-        G[2 * s + t] = mk(0, 0); C[2 * s + t] = mk(0, 0);
-        if (pA[s] < 0 || pB[t] < 0) continue;
+    // the four far records and their near marks loaded together, from
+    // valid addresses whatever the pair turns out to be (a load behind the
+    // mark test would wait for it, four times per region); self and near
+    // pairs, the exceptions, then read their own buffers
+    double mw[4], rv[4][4];
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      const int s = st >> 1, t = st & 1;
+      const bool ok = pA[s] >= 0 && pB[t] >= 0;
+      const double *band = HB + ((long)(ok ? sB[t] : 0) * rowsCap + (ok ? pA[s] : 0)) * (long)(nreg * 9 * stride);
+      const double *rec = band + (r * 9 + (ok ? 3 * qA[s] + qB[t] : 0)) * stride;
+      mw[st] = band[0];
+      rv[st][0] = rec[0]; rv[st][1] = rec[1];
+      rv[st][2] = stride == 4 ? rec[2] : 0.0;
+      rv[st][3] = stride == 4 ? rec[3] : 0.0;
+    }
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      const int s = st >> 1, t = st & 1;
+      G[st] = mk(0, 0); C[st] = mk(0, 0);
+      if (pA[s] < 0 || pB[t] < 0) continue;
+      if ((same && pA[s] == pB[t]) || (unsigned long long)__double_as_longlong(mw[st]) == SCUFFX_NEAR_MARK)
         fetch_h(pA[s], qA[s], pB[t], qB[t], sB[t], r, nreg, stride, same, HB, rowsCap, HN, HS,
-                nearRowPtr, nearCol, nearIdx, &G[2 * s + t], &C[2 * s + t]);
+                nearRowPtr, nearCol, nearIdx, &G[st], &C[st]);
+      else {
+        G[st] = mk(rv[st][0], rv[st][1]);
+        C[st] = stride == 4 ? mk(rv[st][2], rv[st][3]) : mk(0, 0);
       }
+    }
     cplx k = regs[r].k;
     cplx GC0 = LL * (G[0] - G[1] - G[2] + G[3]);
     cplx GC1 = (mk(LL, 0) / IMUL(k)) * (C[0] - C[1] - C[2] + C[3]);
