@@ -619,6 +619,9 @@ k_fippi(const FippiJob *jobs, int njobs, const int *jobList, double *records,
   __shared__ double eA[2][NF], rA[2][NF];
   __shared__ int s_job, s_pop, s_slotB;
   __shared__ int s_stop;
+  // the job's constants, the same for every lane: in LDS, not in each
+  // lane's registers
+  __shared__ FipW sW;
   const int lane = threadIdx.x;
   double *data = heapData + (long long)blockIdx.x * cap * REC;
   HeapRef H;
@@ -632,8 +635,9 @@ k_fippi(const FippiJob *jobs, int njobs, const int *jobList, double *records,
     if (jj >= njobs) break;
     int job = jobList ? jobList[jj] : jj;
     const FippiJob &J = jobs[job];
-    FipW W;
-    setup_w(J, W);
+    if (lane == 0) setup_w(J, sW);
+    __syncthreads();
+    const FipW &W = sW;
     const int dim = (J.ncv == 0) ? 3 : (W.ce ? 1 : 2);
     const int npts = (dim == 1) ? 15 : 1 + 4 * dim + 2 * dim * (dim - 1) + (1 << dim);
     const long maxEval = (J.ncv == 0) ? 1000000 : 10000;
