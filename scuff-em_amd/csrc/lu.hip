@@ -1443,7 +1443,7 @@ static void trsm_block_inv(const T *A, long lda, long r0, int nb, const T *linv,
 // per 32-row panel i, B_i <- Linv_i B_i, then the rows below minus
 // L(below, panel i) B_i (what trsm_block_inv does in 15 launches for the
 // next block's columns).  One workgroup per column: wave v holds rows
-// 64 v + lane; the wave that holds panel i's rows forms x_i = Linv_i B_i
+// 64 v + lane (jb / 64 waves); the wave that holds panel i's rows forms x_i = Linv_i B_i
 // by lane reads and passes it to the others through LDS, so the rows
 // below are updated by four waves at once.
 template <typename T>
@@ -1513,8 +1513,10 @@ static void trsm_blk(const T *A, long lda, long r0, int jb, const T *linv, T *B,
 {
   if (ncol <= 0 || jb <= 0) return;
   int tok = prof_begin(s, std::is_same<T, double2>::value ? PK_TRSM_C128 : PK_TRSM_C64);
-  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)ncol, (unsigned)nbat), dim3(256), 0, s, A, lda, r0, jb, linv, B, ldb,
-                     ncol, bs);
+  // one wave per 64 rows of the block: two for a 128-column block
+  const unsigned nthr = jb <= 128 ? 128u : 256u;
+  hipLaunchKernelGGL(k_trsm_blk<T>, dim3((unsigned)ncol, (unsigned)nbat), dim3(nthr), 0, s, A, lda, r0, jb, linv, B,
+                     ldb, ncol, bs);
   prof_end(s, tok, 4.0 * (double)jb * (double)jb * (double)ncol);
 }
 
