@@ -576,8 +576,8 @@ k_stamp(int nEa, const int *EIa, const double *ELa, const int *bandCols, const i
       const double *rec = band + (r * 9 + (ok ? 3 * qA[s] + qB[t] : 0)) * stride;
       mw[st] = band[0];
       rv[st][0] = rec[0]; rv[st][1] = rec[1];
-      rv[st][2] = stride == 4 ? rec[2] : 0.0;
-      rv[st][3] = stride == 4 ? rec[3] : 0.0;
+      if (stride == 4) { rv[st][2] = rec[2]; rv[st][3] = rec[3]; }   // (records of 2 doubles end at rec[1])
+      else { rv[st][2] = 0.0; rv[st][3] = 0.0; }
     }
 #pragma unroll
     for (int st = 0; st < 4; st++) {
